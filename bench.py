@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark of the LJ hot path on one MI355X.
+
+Workload (the value line): LJ fluid, N = 1,048,576 (FCC 64^3 x 4), reduced
+density 0.8442, T = 0.72, rc = 2.5 sigma (shifted energies), velocity Verlet
+dt = 0.005, Verlet list rebuilt and particles re-sorted into cell order every
+20 steps; FP32 pair forces on FP64 displacements with FP64 integration and
+reductions.  A "step" is one force + integration pass over all particles.
+
+Also reported on the same line: the plain FP64 form of the same run, the
+mixed form's force error against FP64 and its energy drift, the batched
+replica workload (256 LJ replicas, N log-uniform in [500, 20000]) as one
+batched launch per step against each replica run alone in FP64, the
+roofline of the pair kernel (HIP events on the library's stream) and the
+CPU restatement of the reference loop timed on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import particlebase_amd as pb  # noqa: E402
+from particlebase_amd import workloads  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+ALG_BYTES = 98.0          # FP64 r, v read + written (96 B) + u16 type read
+RC2 = 6.25
+DT = 0.005
+METRIC = "LJ atom-steps/s at 1M atoms (roofline %) + batched-vs-plain speedup at 1e-5 err"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_system(ctx, w_or_batch):
+    if isinstance(w_or_batch, workloads.Batch):
+        r, v, m, t, cells = w_or_batch.concat()
+        s = pb.System(ctx, w_or_batch.counts)
+        s.set_cell(cells)
+        s.import_particles(r, m, v=v, type=t)
+    else:
+        w = w_or_batch
+        s = pb.System(ctx, [w.n])
+        s.set_cell(w.cell)
+        s.import_particles(w.r, w.m, v=w.v, type=w.type)
+    s.lj_set_params([1.0, 1.0], 1)
+    s.lj_set_cutoff(RC2, shifted=True)
+    s.clear_force(True)
+    return s
+
+
+def timed_run(ctx, s, steps, precision, skin, rebuild_every, energy_every, timing=True):
+    ctx.sync()
+    t0 = time.perf_counter()
+    e, rc = s.lj_run(DT, steps, precision=precision, skin=skin, rebuild_every=rebuild_every,
+                     energy_every=energy_every, timing=timing)
+    ctx.sync()
+    return time.perf_counter() - t0, e
+
+
+def drift_per_1e4(e, n, energy_every):
+    et = (e[:, 0, 0] + e[:, 0, 1]) / n
+    if et.size < 4:
+        return None
+    t = np.arange(et.size) * energy_every
+    slope = np.polyfit(t, et, 1)[0]
+    return float(slope * 1e4)
+
+
+def cpu_baseline(w, budget_s=20.0):
+    """The oracle's restatement of the reference loop (rebin, 27-cell forces,
+    VV, wrap; FP64, OpenMP) on this host, on a bounded sample of steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pb_oracle as orc
+    cores = min(os.cpu_count() or 1, 16)
+    p = orc.lj_params([1.0, 1.0], 1, RC2, True)
+    minv = 1.0 / w.m
+    t0 = time.perf_counter()
+    orc.lj_run(w.r, w.v, np.zeros_like(w.r), minv, w.type, w.cell, 2.5, RC2, p, DT, 1, cores)
+    t1 = time.perf_counter() - t0
+    steps = max(1, min(50, int(budget_s / max(t1, 1e-3)) - 1))
+    t0 = time.perf_counter()
+    orc.lj_run(w.r, w.v, np.zeros_like(w.r), minv, w.type, w.cell, 2.5, RC2, p, DT, steps, cores)
+    el = time.perf_counter() - t0
+    return {"value": w.n * steps / el, "unit": "atom-steps/s", "cores": cores, "kind": "port",
+            "sample": f"{steps} reference-loop steps of the N={w.n} LJ fluid "
+                      f"({el:.1f} s, FP64, OpenMP {cores} threads)"}
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "lj_step_1M_traffic.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            d = json.load(f)
+        return d.get("hbm_bytes_per_launch")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--skin", type=float, default=0.3)
+    ap.add_argument("--rebuild-every", type=int, default=20)
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--plain-steps", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="small sizes (debug)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {a.gpus} is not supported "
+              f"(the multi-GPU decomposition is a design, not part of this build)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    ctx = pb.Context(0)
+    name, cus = ctx.info()
+    log(f"device {name} CUs={cus}")
+    w = workloads.lj_fcc(16 if a.quick else 64, 0.8442, 0.72, seed=2)
+    log(f"workload N={w.n} L={w.L:.4f}")
+
+    # ---- mixed (chosen) form
+    s = make_system(ctx, w)
+    timed_run(ctx, s, a.warmup, pb.PB_MIXED, a.skin, a.rebuild_every, 0, timing=False)
+    el, e = timed_run(ctx, s, a.steps, pb.PB_MIXED, a.skin, a.rebuild_every, 10)
+    tm = s.lj_run_timing()
+    stats = s.lj_run_stats()
+    value = w.n * a.steps / el
+    kern_ms = tm["step_ms"] / max(tm["nstep"], 1)
+    achieved = ALG_BYTES * w.n / (kern_ms * 1e-3) / 1e9
+    drift = drift_per_1e4(e, w.n, 10)
+    log(f"mixed: {value:.4g} atom-steps/s, {1e3 * el / a.steps:.3f} ms/step, "
+        f"step kernel {kern_ms * 1e3:.1f} us, rebuild {tm['rebuild_ms'] / max(tm['nrebuild'], 1):.3f} ms,"
+        f" {stats}")
+    # force error of the mixed pair kernel against FP64 on the final state
+    s.lj_force(pb.PB_FP64, energies=False)
+    f64 = s.get("F")
+    s.lj_force(pb.PB_MIXED, energies=False)
+    f32 = s.get("F")
+    frms = float(np.sqrt((f64 ** 2).sum(axis=1).mean()))
+    ferr = float(np.abs(f32 - f64).max() / frms)
+    del s
+
+    # ---- plain FP64 form
+    s = make_system(ctx, w)
+    timed_run(ctx, s, a.warmup, pb.PB_FP64, a.skin, a.rebuild_every, 0, timing=False)
+    el64, e64 = timed_run(ctx, s, a.steps, pb.PB_FP64, a.skin, a.rebuild_every, 10)
+    tm64 = s.lj_run_timing()
+    value64 = w.n * a.steps / el64
+    kern64 = tm64["step_ms"] / max(tm64["nstep"], 1)
+    log(f"fp64: {value64:.4g} atom-steps/s, step kernel {kern64 * 1e3:.1f} us")
+    del s
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * el / a.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 pair forces / f64 integration+reductions", "data": "synthetic",
+        "config": {"workload": f"LJ fluid N={w.n} (FCC {round((w.n / 4) ** (1 / 3))}^3x4), "
+                               f"rho 0.8442, T 0.72, rc 2.5, dt 0.005, Verlet skin {a.skin}, "
+                               f"re-sort every {a.rebuild_every} steps",
+                   "global_batch": w.n, "seq_len": 1, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(),
+                     "kernel": "k_lj_step (mixed, fused force+VV)",
+                     "kernel_us": kern_ms * 1e3, "alg_bytes_per_atom": ALG_BYTES},
+        "plain_fp64": {"value": value64, "ms_per_step": 1e3 * el64 / a.steps,
+                       "kernel_us": kern64 * 1e3,
+                       "roofline_frac": ALG_BYTES * w.n / (kern64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                       "energy_drift_per_1e4_steps": drift_per_1e4(e64, w.n, 10)},
+        "mixed": {"force_err_max_over_frms": ferr, "energy_drift_per_1e4_steps": drift,
+                  "max_nn": stats["max_nn"], "skin_violation": stats["skin_violation"],
+                  "rebuild_ms_avg": tm["rebuild_ms"] / max(tm["nrebuild"], 1)},
+    }
+
+    # ---- batched replicas vs plain per-replica FP64
+    if not a.no_batch:
+        reps = workloads.lj_replicas(16 if a.quick else 256, 500, 20000, seed=5)
+        ntot = int(reps.counts.sum())
+        sb = make_system(ctx, reps)
+        timed_run(ctx, sb, 10, pb.PB_MIXED, a.skin, a.rebuild_every, 0, timing=False)
+        elb, eb = timed_run(ctx, sb, a.batch_steps, pb.PB_MIXED, a.skin, a.rebuild_every, 0)
+        vb = ntot * a.batch_steps / elb
+        del sb
+        log(f"batched: {len(reps.systems)} replicas, {ntot} atoms, {vb:.4g} atom-steps/s")
+        plain_systems = [make_system(ctx, x) for x in reps.systems]
+        for ps in plain_systems:
+            ps.lj_run(DT, 2, precision=pb.PB_FP64, skin=a.skin, rebuild_every=a.rebuild_every,
+                      energy_every=0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for ps in plain_systems:
+            ps.lj_run(DT, a.plain_steps, precision=pb.PB_FP64, skin=a.skin,
+                      rebuild_every=a.rebuild_every, energy_every=0)
+        ctx.sync()
+        elp = time.perf_counter() - t0
+        vp = ntot * a.plain_steps / elp
+        del plain_systems
+        log(f"plain sequential FP64: {vp:.4g} atom-steps/s")
+        out["batched"] = {"replicas": len(reps.systems), "atoms": ntot, "value": vb,
+                          "plain_sequential_fp64": vp, "speedup": vb / vp,
+                          "steps": a.batch_steps, "plain_steps": a.plain_steps}
+
+    if not a.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(w, budget_s=5.0 if a.quick else 20.0)
+        log(f"cpu: {out['cpu_baseline']}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,209 @@
+/*
+ * pbamd.h -- C ABI of the MI355X-native short-range particle engine.
+ *
+ * Plain pointers and sizes only.  Every entry point returns an int status
+ * (PB_OK = 0, negative on error; the message is in pb_ctx_last_error) except
+ * the few that return a value.  One context = one HIP device + one stream;
+ * calls on a context are serialised by the caller (the reference's objects
+ * are not thread-safe either).  Device memory is owned by the library, host
+ * buffers by the caller.  Per-particle host arrays are in the caller's
+ * original particle order; vectors are xyz-interleaved (3 doubles a particle).
+ *
+ * Each declaration names the reference interface it replaces (file:line in
+ * particlelover/ParticleBase).  A "system" is one cudaParticleBase object,
+ * or a batch of independent periodic replicas that share one launch.
+ */
+#ifndef PBAMD_H
+#define PBAMD_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------- status codes */
+#define PB_OK               0
+#define PB_ERR_ARG         -1   /* bad argument (sizes, nulls, ranges) */
+#define PB_ERR_HIP         -2   /* a HIP runtime call failed */
+#define PB_ERR_STATE       -3   /* call out of order (e.g. no cells set up) */
+#define PB_ERR_CELLS       -4   /* cell grid invalid (nb <= 2, len < cutoff) */
+#define PB_ERR_NLIST       -5   /* neighbour list overflowed its capacity */
+#define PB_ERR_SKIN        -6   /* a particle moved more than skin/2 between
+                                   list rebuilds: pairs may have been missed */
+
+/* precision modes of the pair kernels */
+#define PB_FP64            0    /* pair math, integration, reductions in FP64 */
+#define PB_MIXED           1    /* FP32 pair math on FP64 displacements;
+                                   FP64 accumulation, integration, reductions */
+
+/* fields for pb_sys_get / pb_sys_set (original particle order) */
+#define PB_FIELD_R         0    /* double[3n] position      (r,   cudaParticleBase.hh:33) */
+#define PB_FIELD_V         1    /* double[3n] velocity      (v,   :35) */
+#define PB_FIELD_A         2    /* double[3n] acceleration; Fold in VV (a, :37; cudaParticleVV.cu:7) */
+#define PB_FIELD_F         3    /* double[3n] force         (F,   :41) */
+#define PB_FIELD_MINV      4    /* double[n]  1/mass        (minv,:39) */
+#define PB_FIELD_TYPE      5    /* uint16[n]  type id       (typeID, :45) */
+#define PB_FIELD_MOVE      6    /* uint8[n]   movable flag  (move, :47) */
+#define PB_FIELD_W         7    /* double[3n] angular velocity (cudaParticleRotation.hh:22) */
+#define PB_FIELD_L         8    /* double[3n] angular momentum (:24) */
+#define PB_FIELD_T         9    /* double[3n] torque           (:26) */
+#define PB_FIELD_RADIUS   10    /* double[n]  radius (DEM r.w, cudaParticleDEM.cu:221-252) */
+
+typedef struct pb_ctx pb_ctx;
+typedef struct pb_sys pb_sys;
+
+/* --------------------------------------------------------------- context */
+/* Replaces CUDAenv<T>'s per-device setup (CUDAenv.hh:93-156) for one GPU. */
+int         pb_ctx_create(int device, pb_ctx **out);
+void        pb_ctx_destroy(pb_ctx *ctx);
+/* Replaces cudaFrame::ErrorInfo (cudaFrame.cu:20-32): message of the last
+ * failing call on this context ("" if none). */
+const char *pb_ctx_last_error(const pb_ctx *ctx);
+int         pb_ctx_sync(pb_ctx *ctx);
+/* Device name and compute-unit count (CUDAenv::setThnum, CUDAenv.hh:36-50). */
+int         pb_ctx_info(pb_ctx *ctx, char *name, size_t name_len, int *cu_count);
+
+/* ---------------------------------------------------------------- system */
+/* cudaParticleBase::setup(int n) (cudaParticleBase.cu:18-66).  nrep = 1 is
+ * the reference's single object; nrep > 1 is a batch of independent
+ * replicas with counts[nrep] particles each, stored back to back. */
+int  pb_sys_create(pb_ctx *ctx, uint32_t nrep, const uint32_t *counts,
+                   pb_sys **out);
+void pb_sys_destroy(pb_sys *sys);
+uint32_t pb_sys_count(const pb_sys *sys);     /* total particles */
+uint32_t pb_sys_replicas(const pb_sys *sys);
+
+/* cudaFrame::setCell (cudaFrame.cu:85-91): cells[6*nrep] = xmin,xmax,ymin,
+ * ymax,zmin,zmax per replica. */
+int pb_sys_set_cell(pb_sys *sys, const double *cells);
+
+/* cudaParticleBase::import(vector<ParticleBase>) (cudaParticleBase.cu:
+ * 68-117), field by field of ParticleBase (ParticleBase.hh:10-18).
+ * v, a may be NULL (zero); is_fixed may be NULL (all movable). */
+int pb_sys_import(pb_sys *sys, const double *r, const double *v,
+                  const double *a, const double *m, const uint32_t *type,
+                  const uint8_t *is_fixed);
+
+/* getPosition / getAcceleration / getForce / getTypeID
+ * (cudaParticleBase.cu:119-149) and the matching uploads. */
+int pb_sys_get(pb_sys *sys, int field, void *host);
+int pb_sys_set(pb_sys *sys, int field, const void *host);
+
+/* cudaParticleBase::clearForce + cudaParticleVV::clearForce
+ * (cudaParticleBase.cu:158-163, cudaParticleVV.cu:10-15): F = 0 and
+ * (with_fold) a = Fold = 0. */
+int pb_clear_force(pb_sys *sys, int with_fold);
+/* calcAcceleration: a = F * minv (cudaParticleBase.cu:170-173). */
+int pb_calc_acceleration(pb_sys *sys);
+/* addForceX/Y/Z, addAccelerationX/Y/Z (cudaParticleBase.cu:208-242). */
+int pb_add_force(pb_sys *sys, int axis, double value);
+int pb_add_acceleration(pb_sys *sys, int axis, double value);
+/* treatPeriodicCondition / treatAbsoluteCondition / treatRefrectCondition
+ * (cudaParticleBase.cu:175-206; kernelfuncs.cu:87-172). */
+int pb_treat_periodic(pb_sys *sys);
+int pb_treat_absolute(pb_sys *sys);
+int pb_treat_reflect(pb_sys *sys);
+/* cudaParticleVV::TimeEvolution (cudaParticleVV.cu:17-34). */
+int pb_vv_step(pb_sys *sys, double dt);
+/* cudaParticleLF::calcVinit / TimeEvolution / rollback
+ * (cudaParticleLF.cu:4-22; kernelfuncs.cu:246-327). */
+int pb_lf_vinit(pb_sys *sys, double dt);
+int pb_lf_step(pb_sys *sys, double dt);
+int pb_lf_rollback(pb_sys *sys, double dt);
+/* cudaParticleMD::calcMV2 (cudaParticleMD.cu:124-133): per-replica
+ * sum m v^2 in FP64 (replaces the cuBLAS FP32 dot). out[nrep]. */
+int pb_mv2(pb_sys *sys, double *out);
+/* mulArray on v (scaleTemp, cudaParticleMD.cu:168-175). */
+int pb_scale_velocities(pb_sys *sys, double s);
+/* cudaParticleBase::inspectVelocity (cudaParticleBase.cu:244-282,
+ * inspectV_F4 kernelfuncs.cu:194-240).  vlim is truncated to uint32 as the
+ * reference's kernel signature does (kernelfuncs.h:116).  *result = 1, 0 or
+ * -1; vratio_max / imax (may be NULL) report the largest |v|/vlim. */
+int pb_inspect_velocity(pb_sys *sys, double vlim, double lim_u,
+                        double lim_l, int *result, double *vratio_max,
+                        uint32_t *imax);
+
+/* ------------------------------------------------------------ cell grid */
+/* cudaCutoffBlock::setupCutoffBlock (cudaCutoffBlock.cu:75-231): FP32 cell
+ * rules of the reference, single replica only.  single_particle selects the
+ * 125-neighbour single-occupancy mode (:115-138). */
+int pb_setup_cells(pb_sys *sys, float rmax, int periodic, int single_particle);
+int pb_get_cell_info(pb_sys *sys, uint32_t blocknum[3], float blocklen[6]);
+/* calcBID_F4 + sort + makeBindex, then cudaSortedPositions::calcBlockID
+ * (cudaCutoffBlock.cu:233-258, kernelfuncs.cu:807-932,
+ * cudaSortedPositions.cu:27-30): cell ids from FP32 positions exactly as
+ * the reference computes them; canonical stable order (ascending original
+ * index inside a cell). */
+int pb_bin(pb_sys *sys);
+/* download pid[n], bid (sorted)[n], bindex[ncell+1] (UINT32_MAX = empty). */
+int pb_get_bins(pb_sys *sys, uint32_t *pid, uint32_t *bid_sorted,
+                uint32_t *bindex);
+/* calc_BlockNeighbor table (cudaCutoffBlock.cu:17-65): ncell*27 or *125. */
+int pb_get_neighbor_table(pb_sys *sys, uint32_t *table);
+
+/* -------------------------------------------------------------------- LJ */
+/* cudaParticleMD::setLJparams (cudaParticleMD.cu:200-202,
+ * kernelinline.hh:120-134): sigma_eps[(i*ntypes+j)*2 + {0,1}] = sigma, eps. */
+int pb_lj_set_params(pb_sys *sys, const double *sigma_eps, int ntypes);
+/* cudaParticleMD::rmax2 (public field, cudaParticleMD.hh:160) plus the
+ * energy shift the benchmark configs use (shifted = 1: phi(rc) subtracted
+ * per pair in energies; forces are never shifted). */
+int pb_lj_set_cutoff(pb_sys *sys, double rc2, int shifted);
+/* cudaParticleMD::calcForce + a correct calcPotentialE
+ * (cudaParticleMD.cu:24-122): F from the current positions; pe, virial
+ * (FP64 per replica, may be NULL) = 1/2 sum phi, 1/2 sum f.r. */
+int pb_lj_force(pb_sys *sys, int precision, double *pe, double *virial);
+/* calcCoordMD_F4 counts via makeNList step 1 (cudaParticleMD.cu:343-366,
+ * kernelinline.hh:387-399): FP32-exact number of j != i with distance2p <
+ * rmax2t, over the 27 cells of the grid set by pb_setup_cells.
+ * counts[n] in original order. */
+int pb_coord_counts(pb_sys *sys, float rmax2t, uint32_t *counts);
+
+/* Options of the fused LJ run (no reference counterpart as one call; it is
+ * the loop of testMD1.cu:135-169 -- calcBlockID, calcForce, TimeEvolution,
+ * treatPeriodicCondition -- fused on the GPU). */
+typedef struct {
+  double   dt;
+  uint32_t nsteps;
+  int      precision;      /* PB_FP64 or PB_MIXED */
+  double   skin;           /* Verlet skin; list cutoff = rc + skin */
+  uint32_t rebuild_every;  /* list rebuild + cell-order re-sort interval */
+  uint32_t energy_every;   /* energies recorded every k steps (0 = never) */
+  int      use_graph;      /* capture the step sequence in a hipGraph */
+  uint32_t block_size;     /* threads per workgroup of the pair kernel (0 = auto) */
+  int      timing;         /* 1: time every step / rebuild with HIP events on the
+                              context's stream (pb_lj_last_run_timing) */
+} pb_run_opts;
+
+/* Fused cudaParticleMD loop: nsteps of [force -> VV -> periodic wrap] with a
+ * Verlet list re-built (and particles re-sorted into cell order) every
+ * rebuild_every steps.  Starts from and ends in the reference VV state
+ * (r(t), v(t-dt), Fold = F(t-dt)); call pb_clear_force(sys, 1) first for the
+ * reference's start.  energies (may be NULL) receives, for each recorded
+ * step s (s % energy_every == 0), nrep x {KE(v(t)), PE(r(t)), W(r(t))}. */
+int pb_lj_run(pb_sys *sys, const pb_run_opts *opts, double *energies);
+/* statistics of the last pb_lj_run: rebuild count, max list length, and
+ * whether a skin violation occurred (1) -- never expected for sane skins. */
+int pb_lj_last_run_stats(pb_sys *sys, uint32_t *rebuilds, uint32_t *max_nn,
+                         int *skin_violation);
+
+/* GPU time of the last pb_lj_run with opts.timing = 1: summed HIP-event time
+ * of the fused step kernels and of the list rebuilds, and their counts. */
+int pb_lj_last_run_timing(pb_sys *sys, double *step_ms, uint32_t *nstep,
+                          double *rebuild_ms, uint32_t *nrebuild);
+
+/* ------------------------------------------------------------------ g(r) */
+/* cudaCalcGR::setup + makePairInfo + calcgr (cudaCalcGR.hh:25-39,
+ * cudaCalcGR.cu:30-113, kerneltemplate.hh:423-512, kernelinline.hh:440-448):
+ * integer histogram over every unordered pair of each replica, FP32-exact
+ * bins.  rstep[nrep] per replica, rnum bins; hist[nrep*rnum] (uint64).
+ * hist[rep*rnum + 0] holds the pairs the reference drops (bin 0 and
+ * overflow). */
+int pb_gr_hist(pb_sys *sys, const float *rstep, uint32_t rnum,
+               uint64_t *hist);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,163 @@
+/*
+ * pb_oracle.h -- CPU restatement of the cudaParticleBase short-range path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (particlebase_amd/,
+ * include/) links, loads or calls this library.  It is used by tests/ as the
+ * checker, by __graft_entry__.smoke() as the checker, and by bench.py's
+ * cpu_baseline leg as the timed CPU path ("kind": "port").
+ *
+ * Two arithmetic modes are restated:
+ *   - "f32-exact": the reference's FP32 operation order (no FMA contraction,
+ *     correctly rounded div/sqrt), for the integer outputs that must match
+ *     bit for bit (cell ids, sort, neighbour counts, g(r) histograms).
+ *   - FP64: the tolerance oracle for forces, energies and trajectories.
+ *
+ * Every function names the reference file:line it restates.  The reference
+ * ships no golden vectors, fixtures or tests; each function is pinned in
+ * tests/test_oracle_*.py by closed forms, invariants, brute force on small
+ * inputs or textbook routines (see the "pinned by" line of each function).
+ */
+#ifndef PB_ORACLE_H
+#define PB_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBO_MAXTYPES 5   /* MAXELEMNUM, kernelinline.hh:68 */
+
+/* ---------------------------------------------------------------- cells */
+
+/* cudaCutoffBlock::setupCutoffBlock, cudaCutoffBlock.cu:97-113 (FP32).
+ * periodic: nb = (uint)(c/rmax), len = c/(float)nb; else len = rmax,
+ * nb = (uint)(c/rmax)+1.  blocklen[3..5] = box minima.
+ * Returns 0, or -1 if some nb <= 2 (the reference asserts nb > 2).
+ * pinned by: closed form in tests/test_oracle_cells.py */
+int pbo_setup_cells(const float cell[6], float rmax, int periodic,
+                    uint32_t blocknum[3], float blocklen[6]);
+
+/* calc_BlockNeighbor, cudaCutoffBlock.cu:17-65: neighbour cell table,
+ * (2w+1)^3 entries per cell in (i1,j1,k1) loop order, UINT32_MAX outside a
+ * non-periodic box.  table has ncell*(2w+1)^3 entries.
+ * pinned by: closed form + symmetry. */
+void pbo_neighbor_table(const uint32_t blocknum[3], int w, int periodic,
+                        uint32_t *table);
+
+/* calcBID_F4, kernelfuncs.cu:807-841, f32-exact, in identity (pid=i) order.
+ * r is xyz-interleaved FP32 (stride 3).
+ * pinned by: edge cases (xmin, xmax, nextafter, negative) + cell-interval
+ * containment. */
+void pbo_calc_bid(const float *r, uint32_t n, const float blocklen[6],
+                  const uint32_t blocknum[3], uint32_t *bid);
+
+/* Canonical replacement for sortByBID_M1/sortByBID/makeBindex
+ * (kernelfuncs.cu:844-932, cudaCutoffBlock.cu:241-256): stable sort of
+ * (bid, pid) from identity order; bindex[c] = first sorted slot of cell c or
+ * UINT32_MAX if c is empty (the reference's makeBindex leaves cell 0 at 0
+ * when it is empty; the correct semantics are used), bindex[ncell] = n.
+ * pinned by: numpy.argsort(kind="stable") and counts. */
+void pbo_sort_bins(const uint32_t *bid, uint32_t n, uint32_t ncell,
+                   uint32_t *pid, uint32_t *bid_sorted, uint32_t *bindex);
+
+/* ------------------------------------------------------------------ LJ */
+
+/* LJ parameters as the reference's setLJparams vector (kernelinline.hh:
+ * 120-134): sigma_eps[(i*ntypes+j)*2+0] = sigma, [..+1] = epsilon. */
+typedef struct {
+  int ntypes;
+  double s2[PBO_MAXTYPES * PBO_MAXTYPES];   /* sigma^2 */
+  double e4[PBO_MAXTYPES * PBO_MAXTYPES];   /* 4 epsilon */
+  double shift[PBO_MAXTYPES * PBO_MAXTYPES];/* phi(rc) per pair type (0 if unshifted) */
+} pbo_lj_params;
+
+void pbo_lj_make_params(const double *sigma_eps, int ntypes, double rc2,
+                        int shifted, pbo_lj_params *p);
+
+/* LJ forces (MDpairForce<potentialLJ>, kerneltemplate.hh:530-564;
+ * potentialLJ::F kernelinline.hh:86-96) in FP64 over the 27-cell stencil of
+ * calcF_IJpairWithBlock_F4 (kerneltemplate.hh:244-293), full neighbour sum
+ * (no Newton's third law), minimum image by the reference's +-L rule
+ * (kerneltemplate.hh:541-543), cutoff R2 < rc2.  F is written in original
+ * particle order (as reduce27_F4 does, kernelfuncs.cu:935-949).
+ * Also returns the correct PE = 1/2 sum phi (shifted if the params are) and
+ * the virial W = 1/2 sum f.r (the reference's calcPotentialE is defective;
+ * the correct quantity is returned).  r, F xyz-interleaved FP64.
+ * Cells: the caller passes the cell grid (blocknum, blocklen) to use.
+ * pinned by: two-body closed form, FCC F=0, sum F = 0, brute force below. */
+int pbo_lj_forces_cells(const double *r, const uint16_t *type, uint32_t n,
+                        const double cell[6], const uint32_t blocknum[3],
+                        const double blocklen[6], double rc2,
+                        const pbo_lj_params *p, double *F, double *pe,
+                        double *virial);
+
+/* O(N^2) minimum-image FP64 brute force: the independent check of the cell
+ * path above (same physics, no cells). */
+void pbo_lj_forces_brute(const double *r, const uint16_t *type, uint32_t n,
+                         const double cell[6], double rc2,
+                         const pbo_lj_params *p, double *F, double *pe,
+                         double *virial);
+
+/* propagateVelocityVerlet_F4, kernelfuncs.cu:333-367, FP64:
+ * v += (F+Fold) dt/2m ; r += v dt + F dt^2/2m ; Fold = F. */
+void pbo_vv_step(double *r, double *v, const double *F, double *Fold,
+                 const double *minv, uint32_t n, double dt);
+
+/* applyPeriodicCondition_F4, kernelfuncs.cu:87-139, FP64, with one defined
+ * deviation: a coordinate that rounds onto the upper face after wrapping is
+ * put on the lower face (the reference asserts there). */
+void pbo_wrap(double *r, uint32_t n, const double cell[6]);
+
+/* sum m v^2 in FP64 (calcMV2, cudaParticleMD.cu:124-133). */
+double pbo_mv2(const double *v, const double *minv, uint32_t n);
+
+/* The reference LJ loop of testMD1.cu:135-169 restated in FP64:
+ * per step: rebin (rmax cells), forces, VV, wrap.  Fold must be zero at
+ * start for the reference's half-kick start (cudaParticleVV.cu:10-15).
+ * energies (if not NULL) receives per step {KE(v(t)), PE(r(t)), W(r(t))}
+ * where v(t) is the VV velocity synchronous with the force at r(t).
+ * nthreads <= 0 uses all OpenMP threads.  Returns 0 or -1 on bad cells. */
+int pbo_lj_run(double *r, double *v, double *Fold, const double *minv,
+               const uint16_t *type, uint32_t n, const double cell[6],
+               double rmax_cells, double rc2, const pbo_lj_params *p,
+               double dt, uint32_t nsteps, double *energies, int nthreads);
+
+/* ------------------------------------------------- f32-exact integer paths */
+
+/* calcCoordMD_F4 (kernelinline.hh:387-399) driven by
+ * calcF_IJpairWithBlock_F4 over the 27-cell table (kerneltemplate.hh:
+ * 244-293) as in cudaParticleMD::makeNList step 1 (cudaParticleMD.cu:343-366):
+ * count j != i with distance2p (kernelinline.hh:47-62) < rmax2t, FP32.
+ * rmax2t is given (the caller forms (sqrt(rmax2)+thickness)^2 in FP32).
+ * counts in original order.  Returns -1 on bad cells.
+ * pinned by: O(N^2) brute force and FCC shell counts. */
+int pbo_coord_counts_f32(const float *r, uint32_t n, const float cell[6],
+                         float rmax_cells, float rmax2t, uint32_t *counts);
+
+/* distance2p, kernelinline.hh:47-62, FP32 exact op order. */
+float pbo_distance2p(float r1x, float r1y, float r1z, float r2x, float r2y,
+                     float r2z, float cx, float cy, float cz);
+
+/* g(r) integer histogram: calcGR_F4 (kernelinline.hh:440-448) over every
+ * unordered pair, the pair set of calcF_IJpairWithBlock5_F4 +
+ * makePairInfo (kerneltemplate.hh:423-512, cudaCalcGR.cu:30-84).
+ * bin = (uint)(sqrt(R2)*rstepinv + 0.5 [double]); bins >= rnum go to 0;
+ * hist[0] is therefore the overflow count (the reference drops it).
+ * rstepinv/rnum as cudaCalcGR::setup (cudaCalcGR.hh:25-34).
+ * pinned by: FCC shell counts (12,6,24,12,24) and brute-force pair count. */
+void pbo_gr_hist_f32(const float *r, uint32_t n, const float cell[6],
+                     float rstepinv, uint32_t rnum, uint64_t *hist);
+
+/* cudaCalcGR::setup scalars: rstepinv = (float)(1.0/rstep),
+ * rnum = (uint)(rmax*rstepinv) in FP32. */
+void pbo_gr_setup(float rstep, float rmax, float *rstepinv, uint32_t *rnum);
+
+/* igr2gr / kigr2gr (cudaCalcGR.cu:14-28, 115-132) in FP32 op order. */
+void pbo_gr_normalize(const uint64_t *igr, uint32_t rnum, float rstepinv,
+                      const float cell[6], uint32_t n, float *gr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

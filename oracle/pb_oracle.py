@@ -1,0 +1,252 @@
+"""ctypes wrapper of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, __graft_entry__.smoke() (as the checker) and
+bench.py's cpu_baseline leg.  The product (particlebase_amd/) never imports
+it.  See pb_oracle.h for the reference file:line each function restates.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+MAXT = 5
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        P = C.POINTER
+        u32, d, f, i32 = C.c_uint32, C.c_double, C.c_float, C.c_int
+        L.pbo_setup_cells.argtypes = [P(f), f, i32, P(u32), P(f)]
+        L.pbo_setup_cells.restype = i32
+        L.pbo_neighbor_table.argtypes = [P(u32), i32, i32, P(u32)]
+        L.pbo_calc_bid.argtypes = [P(f), u32, P(f), P(u32), P(u32)]
+        L.pbo_sort_bins.argtypes = [P(u32), u32, u32, P(u32), P(u32), P(u32)]
+        L.pbo_lj_make_params.argtypes = [P(d), i32, d, i32, C.c_void_p]
+        L.pbo_lj_forces_cells.argtypes = [P(d), P(C.c_uint16), u32, P(d), P(u32), P(d), d,
+                                          C.c_void_p, P(d), P(d), P(d)]
+        L.pbo_lj_forces_cells.restype = i32
+        L.pbo_lj_forces_brute.argtypes = [P(d), P(C.c_uint16), u32, P(d), d, C.c_void_p,
+                                          P(d), P(d), P(d)]
+        L.pbo_vv_step.argtypes = [P(d), P(d), P(d), P(d), P(d), u32, d]
+        L.pbo_wrap.argtypes = [P(d), u32, P(d)]
+        L.pbo_mv2.argtypes = [P(d), P(d), u32]
+        L.pbo_mv2.restype = d
+        L.pbo_lj_run.argtypes = [P(d), P(d), P(d), P(d), P(C.c_uint16), u32, P(d), d, d,
+                                 C.c_void_p, d, u32, P(d), i32]
+        L.pbo_lj_run.restype = i32
+        L.pbo_coord_counts_f32.argtypes = [P(f), u32, P(f), f, f, P(u32)]
+        L.pbo_coord_counts_f32.restype = i32
+        L.pbo_distance2p.argtypes = [f] * 9
+        L.pbo_distance2p.restype = f
+        L.pbo_gr_hist_f32.argtypes = [P(f), u32, P(f), f, u32, P(C.c_uint64)]
+        L.pbo_gr_setup.argtypes = [f, f, P(f), P(u32)]
+        L.pbo_gr_normalize.argtypes = [P(C.c_uint64), u32, f, P(f), u32, P(f)]
+        _extra(L)
+        _lib = L
+    return _lib
+
+
+def _extra(L):
+    """Signatures of the SPH / DEM restatements (pb_oracle_sph.c / _dem.c)."""
+    P = C.POINTER
+    u32, d, i32 = C.c_uint32, C.c_double, C.c_int
+    for name, res, args in [
+        ("pbo_sph_density", i32, [P(d), P(d), P(d), u32, d, P(d), P(d), P(d)]),
+        ("pbo_sph_accel", i32, [P(d), P(d), P(d), P(d), P(d), P(d), P(d), u32, d, d, P(d)]),
+        ("pbo_dem_contacts", i32, [P(d), P(d), u32, P(u32), P(u32), u32, P(u32)]),
+        ("pbo_dem_force", i32, [P(d), P(d), P(d), P(d), P(d), P(C.c_uint8), u32,
+                                P(u32), P(u32), P(d), d, d, d, d, d, d, P(d), P(d)]),
+    ]:
+        if hasattr(L, name):
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class LJParams(C.Structure):
+    _fields_ = [("ntypes", C.c_int), ("s2", C.c_double * 25), ("e4", C.c_double * 25),
+                ("shift", C.c_double * 25)]
+
+
+def setup_cells(cell6, rmax, periodic=True):
+    bn = np.zeros(3, dtype=np.uint32)
+    bl = np.zeros(6, dtype=np.float32)
+    c = f32(cell6)
+    rc = lib().pbo_setup_cells(_p(c, C.c_float), np.float32(rmax), int(periodic),
+                               _p(bn, C.c_uint32), _p(bl, C.c_float))
+    return rc, bn, bl
+
+
+def neighbor_table(bn, w=1, periodic=True):
+    bn = np.ascontiguousarray(bn, dtype=np.uint32)
+    ncell = int(np.prod(bn.astype(np.int64)))
+    t = np.empty(ncell * (2 * w + 1) ** 3, dtype=np.uint32)
+    lib().pbo_neighbor_table(_p(bn, C.c_uint32), w, int(periodic), _p(t, C.c_uint32))
+    return t
+
+
+def calc_bid(r32, bl, bn):
+    r = f32(r32).reshape(-1)
+    n = r.size // 3
+    out = np.empty(n, dtype=np.uint32)
+    lib().pbo_calc_bid(_p(r, C.c_float), n, _p(f32(bl), C.c_float),
+                       _p(np.ascontiguousarray(bn, dtype=np.uint32), C.c_uint32),
+                       _p(out, C.c_uint32))
+    return out
+
+
+def sort_bins(bid, ncell):
+    bid = np.ascontiguousarray(bid, dtype=np.uint32)
+    n = bid.size
+    pid = np.empty(n, dtype=np.uint32)
+    bs = np.empty(n, dtype=np.uint32)
+    bindex = np.empty(ncell + 1, dtype=np.uint32)
+    lib().pbo_sort_bins(_p(bid, C.c_uint32), n, ncell, _p(pid, C.c_uint32), _p(bs, C.c_uint32),
+                        _p(bindex, C.c_uint32))
+    return pid, bs, bindex
+
+
+def lj_params(sigma_eps, ntypes, rc2, shifted=True):
+    p = LJParams()
+    se = f64(sigma_eps)
+    lib().pbo_lj_make_params(_p(se, C.c_double), ntypes, rc2, int(shifted), C.byref(p))
+    return p
+
+
+def lj_forces_cells(r, type_, cell6, rc2, params, rmax_cells=None):
+    r = f64(r).reshape(-1)
+    n = r.size // 3
+    t = np.ascontiguousarray(type_, dtype=np.uint16)
+    cell = f64(cell6)
+    rmax = math.sqrt(rc2) if rmax_cells is None else rmax_cells
+    bn = np.array([int((cell[2 * a + 1] - cell[2 * a]) / rmax) for a in range(3)], dtype=np.uint32)
+    bl = np.array([(cell[2 * a + 1] - cell[2 * a]) / bn[a] for a in range(3)] +
+                  [cell[0], cell[2], cell[4]])
+    F = np.empty(3 * n)
+    pe = C.c_double()
+    w = C.c_double()
+    rc = lib().pbo_lj_forces_cells(_p(r, C.c_double), _p(t, C.c_uint16), n, _p(cell, C.c_double),
+                                   _p(bn, C.c_uint32), _p(bl, C.c_double), rc2, C.byref(params),
+                                   _p(F, C.c_double), C.byref(pe), C.byref(w))
+    if rc:
+        raise ValueError("oracle cells invalid")
+    return F.reshape(n, 3), pe.value, w.value
+
+
+def lj_forces_brute(r, type_, cell6, rc2, params):
+    r = f64(r).reshape(-1)
+    n = r.size // 3
+    t = np.ascontiguousarray(type_, dtype=np.uint16)
+    cell = f64(cell6)
+    F = np.empty(3 * n)
+    pe = C.c_double()
+    w = C.c_double()
+    lib().pbo_lj_forces_brute(_p(r, C.c_double), _p(t, C.c_uint16), n, _p(cell, C.c_double), rc2,
+                              C.byref(params), _p(F, C.c_double), C.byref(pe), C.byref(w))
+    return F.reshape(n, 3), pe.value, w.value
+
+
+def vv_step(r, v, F, Fold, minv, dt):
+    r, v, F, Fold, minv = (f64(x).copy() for x in (r, v, F, Fold, minv))
+    n = minv.size
+    lib().pbo_vv_step(_p(r, C.c_double), _p(v, C.c_double), _p(F, C.c_double),
+                      _p(Fold, C.c_double), _p(minv, C.c_double), n, dt)
+    return r, v, Fold
+
+
+def wrap(r, cell6):
+    r = f64(r).copy()
+    lib().pbo_wrap(_p(r, C.c_double), r.size // 3, _p(f64(cell6), C.c_double))
+    return r
+
+
+def mv2(v, minv):
+    v = f64(v)
+    minv = f64(minv)
+    return lib().pbo_mv2(_p(v, C.c_double), _p(minv, C.c_double), minv.size)
+
+
+def lj_run(r, v, Fold, minv, type_, cell6, rmax_cells, rc2, params, dt, nsteps, nthreads=0):
+    r, v, Fold, minv = (f64(x).copy() for x in (r, v, Fold, minv))
+    t = np.ascontiguousarray(type_, dtype=np.uint16)
+    n = minv.size
+    e = np.zeros(3 * max(nsteps, 1))
+    rc = lib().pbo_lj_run(_p(r, C.c_double), _p(v, C.c_double), _p(Fold, C.c_double),
+                          _p(minv, C.c_double), _p(t, C.c_uint16), n,
+                          _p(f64(cell6), C.c_double), rmax_cells, rc2, C.byref(params), dt,
+                          nsteps, _p(e, C.c_double), int(nthreads))
+    if rc:
+        raise ValueError("oracle run failed (cells)")
+    return r.reshape(n, 3), v.reshape(n, 3), Fold.reshape(n, 3), e.reshape(-1, 3)[:nsteps]
+
+
+def coord_counts_f32(r32, cell6, rmax_cells, rmax2t):
+    r = f32(r32).reshape(-1)
+    n = r.size // 3
+    out = np.empty(n, dtype=np.uint32)
+    rc = lib().pbo_coord_counts_f32(_p(r, C.c_float), n, _p(f32(cell6), C.c_float),
+                                    np.float32(rmax_cells), np.float32(rmax2t),
+                                    _p(out, C.c_uint32))
+    if rc:
+        raise ValueError("oracle cells invalid")
+    return out
+
+
+def distance2p(a, b, c3):
+    return lib().pbo_distance2p(*(float(x) for x in a), *(float(x) for x in b),
+                                *(float(x) for x in c3))
+
+
+def gr_setup(rstep, rmax):
+    ri = C.c_float()
+    rn = C.c_uint32()
+    lib().pbo_gr_setup(np.float32(rstep), np.float32(rmax), C.byref(ri), C.byref(rn))
+    return ri.value, rn.value
+
+
+def gr_hist_f32(r32, cell6, rstepinv, rnum):
+    r = f32(r32).reshape(-1)
+    n = r.size // 3
+    h = np.zeros(rnum, dtype=np.uint64)
+    lib().pbo_gr_hist_f32(_p(r, C.c_float), n, _p(f32(cell6), C.c_float), np.float32(rstepinv),
+                          rnum, _p(h, C.c_uint64))
+    return h
+
+
+def gr_normalize(igr, rstepinv, cell6, n):
+    igr = np.ascontiguousarray(igr, dtype=np.uint64)
+    rnum = igr.size
+    gr = np.zeros(rnum, dtype=np.float32)
+    lib().pbo_gr_normalize(_p(igr, C.c_uint64), rnum, np.float32(rstepinv),
+                           _p(f32(cell6), C.c_float), n, _p(gr, C.c_float))
+    return gr
+
+
+import math  # noqa: E402  (used by lj_forces_cells)

@@ -1,0 +1,232 @@
+// pb_internal.h -- shared types of the MI355X particle engine (host + device).
+//
+// Storage layout (HBM): every per-particle array is indexed by a storage
+// slot s in [0, npad).  Each replica owns a 64-aligned slot range
+// [atom0, atom0 + npad_r) holding its natoms real particles followed by
+// padding slots, so every wavefront belongs to exactly one replica and all
+// per-replica reductions are per-wave partials summed in a fixed order.
+// Positions are double4 (x, y, z, w = type id), velocities double4
+// (x, y, z, w = 1/mass): one 32-byte load brings everything the pair and
+// integration kernels need about a particle.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/pbamd.h"
+
+#define PB_WAVE 64
+#define PB_PAD_TYPE 0xFFFFu
+#define PB_NONE 0xFFFFFFFFu
+#define PB_MAXT 5
+
+struct RepInfo {
+  double lo[3], hi[3], L[3], Linv[3];
+  uint32_t nb[3];      // list-cell grid of this replica (fused path)
+  uint32_t cell0;      // first global list cell; the replica's pad cell is cell0 + nb0*nb1*nb2
+  uint32_t atom0;      // first storage slot (multiple of 64)
+  uint32_t natoms;     // real particles
+  uint32_t npad;       // slots (multiple of 64)
+  uint32_t wave0, nwave;
+};
+
+struct LJTable {
+  int ntypes;
+  double s2[PB_MAXT * PB_MAXT], e24[PB_MAXT * PB_MAXT], e4[PB_MAXT * PB_MAXT],
+      shift[PB_MAXT * PB_MAXT];
+  float s2f[PB_MAXT * PB_MAXT], e24f[PB_MAXT * PB_MAXT], e4f[PB_MAXT * PB_MAXT],
+      shiftf[PB_MAXT * PB_MAXT];
+};
+
+struct pb_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int cu_count = 0;
+  std::string name;
+};
+
+// device buffer helper
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+};
+
+struct pb_sys {
+  pb_ctx *ctx = nullptr;
+  uint32_t nrep = 0, n = 0, npad = 0, nwave = 0;
+  std::vector<uint32_t> counts, offsets;   // per replica (original order)
+  std::vector<RepInfo> reps;               // host copy
+  std::vector<double> cells;               // 6 per replica
+  bool cell_set = false;
+
+  // device state (storage order)
+  double4 *r = nullptr, *v = nullptr, *a = nullptr, *F = nullptr;
+  double4 *w = nullptr, *Lm = nullptr, *T = nullptr;   // rotation (lazy)
+  double *radius = nullptr;                            // DEM (lazy)
+  uint8_t *move = nullptr;
+  uint16_t *type = nullptr;
+  uint32_t *orig = nullptr;       // storage slot -> original index (PB_NONE: pad)
+  uint32_t *waveRep = nullptr;    // wave -> replica
+  RepInfo *d_reps = nullptr;
+  double *partials = nullptr;     // per-wave partial sums, 4 per wave
+  uint32_t *d_counter = nullptr;  // last-block counters / flags (16 words)
+  double *d_out = nullptr;        // small reduction outputs (8 per replica)
+
+  // reference cell grid (pb_setup_cells), single replica
+  bool ref_cells = false;
+  int ref_periodic = 1, ref_single = 0;
+  uint32_t blocknum[3] = {0, 0, 0};
+  float blocklen[6] = {0, 0, 0, 0, 0, 0};
+  uint32_t ref_ncell = 0;
+  uint32_t *ref_table = nullptr;  // ncell * 27 (or 125)
+  uint32_t *ref_bid = nullptr, *ref_pid = nullptr, *ref_bindex = nullptr;
+  bool ref_binned = false;
+
+  // counting-sort workspace (shared by pb_bin and the list rebuild)
+  uint32_t *cellid = nullptr, *rank = nullptr, *perm = nullptr;
+  uint32_t *ccount = nullptr, *cstart = nullptr;  // ncell_cap + 1
+  uint32_t ncell_cap = 0;
+  uint32_t *scan_tmp = nullptr;
+  size_t scan_tmp_n = 0;
+
+  // LJ
+  LJTable lj{};
+  bool lj_set = false;
+  double rc2 = 0.0;
+  int shifted = 0;
+  LJTable *d_lj = nullptr;
+
+  // fused path: Verlet list
+  uint32_t *nl = nullptr, *nn = nullptr;
+  uint32_t maxnb = 0;
+  float4 *rbuild = nullptr;
+  double4 *r2buf = nullptr, *v2buf = nullptr;   // ping-pong partners
+  uint32_t *orig2 = nullptr;
+  uint16_t *type2 = nullptr;
+  uint8_t *move2 = nullptr;
+  double4 *a2buf = nullptr;
+  uint32_t list_ncell = 0;
+  double list_rlist = 0.0;
+  uint32_t last_rebuilds = 0, last_maxnn = 0;
+  int last_skin_violation = 0;
+  double last_step_ms = 0.0, last_rebuild_ms = 0.0;
+  uint32_t last_nstep_timed = 0, last_nrebuild_timed = 0;
+
+  // g(r)
+  uint64_t *gr_hist = nullptr;
+  size_t gr_cap = 0;
+};
+
+// --------------------------------------------------------------- host util
+int pb_fail(pb_ctx *ctx, int code, const std::string &msg);
+int pb_hip_check(pb_ctx *ctx, hipError_t e, const char *what);
+#define PB_HIP(ctx, call)                                              \
+  do {                                                                 \
+    hipError_t _e = (call);                                            \
+    if (_e != hipSuccess) return pb_hip_check((ctx), _e, #call);       \
+  } while (0)
+#define PB_LAUNCH_CHECK(ctx, what)                                     \
+  do {                                                                 \
+    hipError_t _e = hipGetLastError();                                 \
+    if (_e != hipSuccess) return pb_hip_check((ctx), _e, what);        \
+  } while (0)
+
+template <typename T>
+int pb_alloc(pb_ctx *ctx, T **p, size_t n) {
+  if (*p) {
+    (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, sizeof(T) * n);
+  if (e != hipSuccess) return pb_hip_check(ctx, e, "hipMalloc");
+  return PB_OK;
+}
+template <typename T>
+void pb_free(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+static inline uint32_t pb_blocks(size_t n, uint32_t bs) {
+  return (uint32_t)((n + bs - 1) / bs);
+}
+
+// counting sort helpers (pb_sort.hip)
+int pb_ensure_sort_ws(pb_sys *sys, uint32_t ncell);
+// exclusive scan of in[0..n) into out[0..n], out[n] = total
+int pb_exclusive_scan(pb_sys *sys, const uint32_t *in, uint32_t *out, uint32_t n);
+// cellid[s] given (PB_NONE never); builds perm (sorted slot -> storage slot)
+// ordered by (cell, key[s]); counts/starts in sys->ccount / sys->cstart.
+int pb_counting_sort(pb_sys *sys, uint32_t ncell, const uint32_t *key);
+// per-replica reduction of K per-wave partial components into out[rep*8+c]
+int pb_reduce_waves(pb_sys *sys, int K);
+
+// ------------------------------------------------------------ device util
+__device__ __forceinline__ uint32_t pb_lane() { return __lane_id(); }
+
+__device__ __forceinline__ double pb_wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+__device__ __forceinline__ double pb_wave_max(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  return x;
+}
+
+// XCD-aware block index: consecutive logical blocks go to the same XCD so
+// each XCD's L2 holds one contiguous slab of particles.  Bijective for any
+// grid size (MI355X dispatches workgroups round-robin over 8 XCDs).
+__device__ __forceinline__ uint32_t pb_xcd_block() {
+  const uint32_t b = blockIdx.x, nb = gridDim.x;
+  const uint32_t q = nb >> 3, rmd = nb & 7u, x = b & 7u;
+  const uint32_t start = x * q + (x < rmd ? x : rmd);
+  return start + (b >> 3);
+}
+
+__device__ __forceinline__ double pb_load_relaxed(const double *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// sum of K per-wave partial components for every replica, executed by one
+// block (used by the last-block reductions and by k_reduce_waves).
+__device__ __forceinline__ void pb_block_reduce_reps(const double *partials, int K, const RepInfo *reps,
+                                     uint32_t nrep, double *out, bool relaxed) {
+  // Deterministic: every replica's wave partials are summed in a fixed order
+  // that depends only on the launch shape of this block.
+  __shared__ double sm[4][256];
+  const uint32_t t = threadIdx.x, bs = blockDim.x;
+  if (nrep >= 16) {
+    for (uint32_t q = t; q < nrep; q += bs) {
+      const RepInfo R = reps[q];
+      double acc[4] = {0, 0, 0, 0};
+      for (uint32_t w = R.wave0; w < R.wave0 + R.nwave; ++w)
+        for (int c = 0; c < K; ++c)
+          acc[c] += relaxed ? pb_load_relaxed(partials + 4 * w + c) : partials[4 * w + c];
+      for (int c = 0; c < K; ++c) out[8 * q + c] = acc[c];
+    }
+    return;
+  }
+  for (uint32_t q = 0; q < nrep; ++q) {
+    const uint32_t w0 = reps[q].wave0, nw = reps[q].nwave;
+    double acc[4] = {0, 0, 0, 0};
+    for (uint32_t w = t; w < nw; w += bs)
+      for (int c = 0; c < K; ++c)
+        acc[c] += relaxed ? pb_load_relaxed(partials + 4 * (w0 + w) + c) : partials[4 * (w0 + w) + c];
+    for (int c = 0; c < K; ++c) sm[c][t] = acc[c];
+    __syncthreads();
+    for (uint32_t o = bs / 2; o > 0; o >>= 1) {
+      if (t < o)
+        for (int c = 0; c < K; ++c) sm[c][t] += sm[c][t + o];
+      __syncthreads();
+    }
+    if (t == 0)
+      for (int c = 0; c < K; ++c) out[8 * q + c] = sm[c][0];
+    __syncthreads();
+  }
+}
+

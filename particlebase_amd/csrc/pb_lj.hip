@@ -1,0 +1,743 @@
+// pb_lj.hip -- Lennard-Jones path of cudaParticleMD on MI355X.
+//
+// Reference path (cudaParticleMD.cu:24-69, kerneltemplate.hh:229-294,
+// 523-565, kernelinline.hh:47-110, kernelfuncs.cu:333-367, 935-949): per
+// step a cell sort, a (cell x 27) pair kernel writing 81 partial forces per
+// particle to HBM, a 27-way reduction, then VV and the periodic wrap.
+//
+// Here: a Verlet list (cutoff rc + skin) is built from a cell grid whose
+// cells are >= rc + skin; at every rebuild the particles are re-sorted into
+// cell order so list neighbours are close in HBM.  Between rebuilds one
+// kernel per step computes the pair forces from the list, integrates
+// (velocity Verlet in its kick-drift form, FP64) and -- on energy steps --
+// reduces KE, PE and the virial per replica in FP64 with a deterministic
+// last-workgroup reduction.  The 81N partial-force buffer, the reduce27 pass
+// and every host synchronisation of the reference loop are gone.
+#include "pb_internal.h"
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+int pb_ensure_sort_ws(pb_sys *s, uint32_t ncell);
+int pb_counting_sort(pb_sys *s, uint32_t ncell, const uint32_t *key);
+__global__ void k_fill_u32(uint32_t *x, uint32_t n, uint32_t val);
+__global__ void k_wrap(double4 *r, const uint32_t *orig, const uint32_t *waveRep,
+                       const RepInfo *reps, uint32_t n);
+
+enum { CNT_LAST = 0, CNT_SKIN = 1, CNT_OVF = 2, CNT_MAXNN = 3, CNT_REC = 4 };
+
+// ---------------------------------------------------------------- params
+static void fill_table(pb_sys *s) {
+  LJTable &t = s->lj;
+  for (int k = 0; k < PB_MAXT * PB_MAXT; ++k) {
+    t.shift[k] = 0.0;
+    if (s->shifted && s->rc2 > 0 && t.s2[k] > 0) {
+      double sr = t.s2[k] / s->rc2;
+      sr = sr * sr * sr;
+      t.shift[k] = t.e4[k] * (sr * sr - sr);
+    }
+    t.s2f[k] = (float)t.s2[k];
+    t.e24f[k] = (float)t.e24[k];
+    t.e4f[k] = (float)t.e4[k];
+    t.shiftf[k] = (float)t.shift[k];
+  }
+}
+
+static int upload_lj(pb_sys *s) {
+  fill_table(s);
+  PB_HIP(s->ctx, hipMemcpyAsync(s->d_lj, &s->lj, sizeof(LJTable), hipMemcpyHostToDevice,
+                                s->ctx->stream));
+  return PB_OK;
+}
+
+extern "C" int pb_lj_set_params(pb_sys *s, const double *sigma_eps, int ntypes) {
+  if (!s || !sigma_eps || ntypes < 1 || ntypes > PB_MAXT) return PB_ERR_ARG;
+  // _setLJparams, kernelinline.hh:120-134 (sigma^2, 24 eps), kept in FP64
+  std::memset(&s->lj, 0, sizeof(LJTable));
+  s->lj.ntypes = ntypes;
+  for (int i = 0; i < ntypes; ++i)
+    for (int j = 0; j < ntypes; ++j) {
+      const double sg = sigma_eps[(i * ntypes + j) * 2], ep = sigma_eps[(i * ntypes + j) * 2 + 1];
+      const int k = i * PB_MAXT + j;
+      s->lj.s2[k] = sg * sg;
+      s->lj.e24[k] = 24.0 * ep;
+      s->lj.e4[k] = 4.0 * ep;
+    }
+  s->lj_set = true;
+  return upload_lj(s);
+}
+
+extern "C" int pb_lj_set_cutoff(pb_sys *s, double rc2, int shifted) {
+  if (!s || !(rc2 > 0)) return PB_ERR_ARG;
+  s->rc2 = rc2;
+  s->shifted = shifted ? 1 : 0;
+  return upload_lj(s);
+}
+
+// ------------------------------------------------------------ list cells
+__device__ __forceinline__ double wrap_pos(double x, double lo, double L, double Linv) {
+  double t = x - lo;
+  t -= L * floor(t * Linv);
+  if (t >= L) t -= L;
+  if (t < 0.0) t = 0.0;
+  double y = lo + t;
+  if (y >= lo + L) y = lo;
+  return y;
+}
+
+__device__ __forceinline__ uint32_t cell_axis(double x, double lo, uint32_t nb, double Linv) {
+  const double q = (x - lo) * ((double)nb * Linv);
+  uint32_t d = (q > 0.0) ? (uint32_t)q : 0u;
+  return d >= nb ? nb - 1 : d;
+}
+
+__global__ void k_list_cellid(const double4 *r, const uint32_t *orig, const uint32_t *waveRep,
+                              const RepInfo *reps, uint32_t n, uint32_t *cellid) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const RepInfo &R = reps[waveRep[s / PB_WAVE]];
+  const uint32_t ncr = R.nb[0] * R.nb[1] * R.nb[2];
+  if (orig[s] == PB_NONE) {
+    cellid[s] = R.cell0 + ncr;
+    return;
+  }
+  const double4 p = r[s];
+  const double x = wrap_pos(p.x, R.lo[0], R.L[0], R.Linv[0]);
+  const double y = wrap_pos(p.y, R.lo[1], R.L[1], R.Linv[1]);
+  const double z = wrap_pos(p.z, R.lo[2], R.L[2], R.Linv[2]);
+  const uint32_t ci = cell_axis(x, R.lo[0], R.nb[0], R.Linv[0]);
+  const uint32_t cj = cell_axis(y, R.lo[1], R.nb[1], R.Linv[1]);
+  const uint32_t ck = cell_axis(z, R.lo[2], R.nb[2], R.Linv[2]);
+  cellid[s] = R.cell0 + ck + R.nb[2] * (cj + R.nb[1] * ci);
+}
+
+struct ReorderArgs {
+  const uint32_t *perm;
+  const double4 *r, *v, *a;
+  const uint32_t *orig, *cellid;
+  const uint16_t *type;
+  const uint8_t *move;
+  double4 *r2, *v2, *a2;
+  uint32_t *orig2, *cell2;
+  uint16_t *type2;
+  uint8_t *move2;
+  float4 *rbuild;
+  const uint32_t *waveRep;
+  const RepInfo *reps;
+  uint32_t n;
+};
+
+// gather every per-particle array into cell order (positions wrapped)
+__global__ void k_reorder(ReorderArgs A) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= A.n) return;
+  const uint32_t s = A.perm[k];
+  double4 p = A.r[s];
+  const uint32_t o = A.orig[s];
+  if (o != PB_NONE) {
+    const RepInfo &R = A.reps[A.waveRep[k / PB_WAVE]];
+    p.x = wrap_pos(p.x, R.lo[0], R.L[0], R.Linv[0]);
+    p.y = wrap_pos(p.y, R.lo[1], R.L[1], R.Linv[1]);
+    p.z = wrap_pos(p.z, R.lo[2], R.L[2], R.Linv[2]);
+  }
+  A.r2[k] = p;
+  A.v2[k] = A.v[s];
+  A.a2[k] = A.a[s];
+  A.orig2[k] = o;
+  A.cell2[k] = A.cellid[s];
+  A.type2[k] = A.type[s];
+  A.move2[k] = A.move[s];
+  A.rbuild[k] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.f);
+}
+
+// One thread per particle (cell order): scan the (deduplicated) 27
+// neighbouring list cells, keep j with |r_ij| < rc + skin (FP64, minimum
+// image).  The list is stored neighbour-major, nl[k*npad + i], so the pair
+// kernel's loads of the k-th neighbour of 64 consecutive particles are one
+// coalesced 256-byte read.
+__global__ void __launch_bounds__(256) k_build_list(const double4 *r, const uint32_t *cell,
+                                                    const uint32_t *cstart,
+                                                    const uint32_t *waveRep, const RepInfo *reps,
+                                                    uint32_t n, double rlist2, uint32_t maxnb,
+                                                    uint32_t *nl, uint32_t *nn,
+                                                    uint32_t *counter) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double4 pi = r[i];
+  if (pi.w == (double)PB_PAD_TYPE) {
+    nn[i] = 0;
+    return;
+  }
+  const RepInfo &R = reps[waveRep[i / PB_WAVE]];
+  const uint32_t c = cell[i] - R.cell0;
+  const int nb0 = (int)R.nb[0], nb1 = (int)R.nb[1], nb2 = (int)R.nb[2];
+  const int ck = (int)(c % R.nb[2]), cj = (int)((c / R.nb[2]) % R.nb[1]),
+            ci = (int)(c / (R.nb[2] * R.nb[1]));
+  const int lx = nb0 >= 3 ? -1 : 0, hx = nb0 >= 2 ? 1 : 0;
+  const int ly = nb1 >= 3 ? -1 : 0, hy = nb1 >= 2 ? 1 : 0;
+  const int lz = nb2 >= 3 ? -1 : 0, hz = nb2 >= 2 ? 1 : 0;
+  uint32_t cnt = 0;
+  for (int a = lx; a <= hx; ++a) {
+    const int ii = (ci + a + nb0) % nb0;
+    for (int b = ly; b <= hy; ++b) {
+      const int jj = (cj + b + nb1) % nb1;
+      for (int d = lz; d <= hz; ++d) {
+        const int kk = (ck + d + nb2) % nb2;
+        const uint32_t J = R.cell0 + (uint32_t)(kk + nb2 * (jj + nb1 * ii));
+        const uint32_t je = cstart[J + 1];
+        for (uint32_t j = cstart[J]; j < je; ++j) {
+          if (j == i) continue;
+          const double4 pj = r[j];
+          double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+          dx -= R.L[0] * rint(dx * R.Linv[0]);
+          dy -= R.L[1] * rint(dy * R.Linv[1]);
+          dz -= R.L[2] * rint(dz * R.Linv[2]);
+          const double r2 = dx * dx + dy * dy + dz * dz;
+          if (r2 < rlist2) {
+            if (cnt < maxnb) nl[(size_t)cnt * n + i] = j;
+            ++cnt;
+          }
+        }
+      }
+    }
+  }
+  nn[i] = cnt < maxnb ? cnt : maxnb;
+  if (cnt > maxnb) atomicMax(&counter[CNT_OVF], cnt);
+  atomicMax(&counter[CNT_MAXNN], cnt);
+}
+
+// ------------------------------------------------------------- step kernel
+struct StepArgs {
+  const double4 *rin, *vin;
+  double4 *rout, *vout, *Fout;
+  const uint32_t *nl, *nn;
+  const float4 *rbuild;
+  const uint32_t *waveRep;
+  const RepInfo *reps;
+  const LJTable *lj;
+  double *partials;
+  uint32_t *counter;
+  double *energy;       // records of 8*nrep doubles
+  uint32_t npad, nrep;
+  double dt, rc2, half_skin2;
+  float rc2f;
+  // single-type parameters
+  double s2, e24, e4, shift;
+  float s2f, e24f, e4f, shiftf;
+};
+
+__device__ __forceinline__ double rcp64(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(fma(-x, y, 1.0), y, y);
+  y = fma(fma(-x, y, 1.0), y, y);
+  return y;
+}
+
+template <bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
+__global__ void __launch_bounds__(256) k_lj_step(StepArgs A) {
+  const uint32_t i = pb_xcd_block() * blockDim.x + threadIdx.x;
+  const bool inr = i < A.npad;
+  // a wavefront never spans two replicas: make the replica index scalar
+  const uint32_t rep = __builtin_amdgcn_readfirstlane(inr ? A.waveRep[i / PB_WAVE] : 0u);
+  const RepInfo &R = A.reps[rep];
+  double4 pi = inr ? A.rin[i] : make_double4(0, 0, 0, (double)PB_PAD_TYPE);
+  const bool real = inr && pi.w != (double)PB_PAD_TYPE;
+  const uint32_t nni = real ? A.nn[i] : 0u;
+  const uint32_t ti = ONE_TYPE ? 0u : (real ? (uint32_t)pi.w : 0u);
+  const double L0 = R.L[0], L1 = R.L[1], L2 = R.L[2];
+  const double I0 = R.Linv[0], I1 = R.Linv[1], I2 = R.Linv[2];
+
+  double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
+  float ffx = 0.f, ffy = 0.f, ffz = 0.f, fpe = 0.f, fvir = 0.f;
+  const uint32_t *nlp = A.nl + i;
+  for (uint32_t k = 0; k < nni; ++k) {
+    const uint32_t j = nlp[(size_t)k * A.npad];
+    const double4 pj = A.rin[j];
+    double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    dx -= L0 * rint(dx * I0);
+    dy -= L1 * rint(dy * I1);
+    dz -= L2 * rint(dz * I2);
+    uint32_t t = 0;
+    if (!ONE_TYPE) t = ti * PB_MAXT + (uint32_t)pj.w;
+    if (MIXED) {
+      const float ex = (float)dx, ey = (float)dy, ez = (float)dz;
+      const float r2 = ex * ex + ey * ey + ez * ez;
+      if (r2 < A.rc2f) {
+        const float s2 = ONE_TYPE ? A.s2f : A.lj->s2f[t];
+        const float e24 = ONE_TYPE ? A.e24f : A.lj->e24f[t];
+        const float ir2 = __builtin_amdgcn_rcpf(r2);
+        float s6 = s2 * ir2;
+        s6 = s6 * s6 * s6;
+        const float fr = e24 * s6 * (2.0f * s6 - 1.0f) * ir2;   // kernelinline.hh:90-95
+        ffx += fr * ex;
+        ffy += fr * ey;
+        ffz += fr * ez;
+        if (ENERGY) {
+          const float e4 = ONE_TYPE ? A.e4f : A.lj->e4f[t];
+          const float sh = ONE_TYPE ? A.shiftf : A.lj->shiftf[t];
+          fpe += e4 * s6 * (s6 - 1.0f) - sh;
+          fvir += fr * r2;
+        }
+      }
+    } else {
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 < A.rc2) {
+        const double s2 = ONE_TYPE ? A.s2 : A.lj->s2[t];
+        const double e24 = ONE_TYPE ? A.e24 : A.lj->e24[t];
+        const double ir2 = rcp64(r2);
+        double s6 = s2 * ir2;
+        s6 = s6 * s6 * s6;
+        const double fr = e24 * s6 * (2.0 * s6 - 1.0) * ir2;
+        fx += fr * dx;
+        fy += fr * dy;
+        fz += fr * dz;
+        if (ENERGY) {
+          const double e4 = ONE_TYPE ? A.e4 : A.lj->e4[t];
+          const double sh = ONE_TYPE ? A.shift : A.lj->shift[t];
+          pe += e4 * s6 * (s6 - 1.0) - sh;
+          vir += fr * r2;
+        }
+      }
+    }
+  }
+  if (MIXED) {
+    fx = ffx;
+    fy = ffy;
+    fz = ffz;
+    pe = fpe;
+    vir = fvir;
+  }
+
+  double ke = 0.0;
+  if (INTEGRATE) {
+    if (inr) {
+      double4 vi = A.vin[i];
+      if (real) {
+        // kick-drift form of propagateVelocityVerlet_F4 (kernelfuncs.cu:
+        // 347-365): v holds v(t - dt/2) = v_ref(t-dt) + Fold dt/2m
+        const double dtm2 = 0.5 * A.dt * vi.w;
+        const double vx = vi.x + fx * dtm2, vy = vi.y + fy * dtm2, vz = vi.z + fz * dtm2;  // v(t)
+        if (ENERGY) ke = 0.5 * (vx * vx + vy * vy + vz * vz) / vi.w;
+        vi.x = vx + fx * dtm2;
+        vi.y = vy + fy * dtm2;
+        vi.z = vz + fz * dtm2;
+        pi.x += vi.x * A.dt;
+        pi.y += vi.y * A.dt;
+        pi.z += vi.z * A.dt;
+        const float4 rb = A.rbuild[i];
+        const float ddx = (float)(pi.x - (double)rb.x), ddy = (float)(pi.y - (double)rb.y),
+                    ddz = (float)(pi.z - (double)rb.z);
+        if ((double)(ddx * ddx + ddy * ddy + ddz * ddz) > A.half_skin2)
+          atomicOr(&A.counter[CNT_SKIN], 1u);
+      }
+      A.rout[i] = pi;
+      A.vout[i] = vi;
+    }
+  }
+  if (WRITE_F && inr) A.Fout[i] = make_double4(fx, fy, fz, 0.0);
+
+  if (ENERGY) {
+    // per-wave partials (a wave never spans two replicas) ...
+    const double kw = pb_wave_sum(ke), pw = pb_wave_sum(0.5 * pe), ww = pb_wave_sum(0.5 * vir);
+    if (pb_lane() == 0 && inr) {
+      double *P = A.partials + 4 * (size_t)(i / PB_WAVE);
+      P[0] = kw;
+      P[1] = pw;
+      P[2] = ww;
+    }
+    // ... summed per replica by the last workgroup to finish, in a fixed order
+    __shared__ bool last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      const uint32_t done = atomicAdd(&A.counter[CNT_LAST], 1u);
+      last = (done == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last) {
+      __threadfence();
+      const uint32_t rec = __hip_atomic_load(&A.counter[CNT_REC], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+      pb_block_reduce_reps(A.partials, 3, A.reps, A.nrep, A.energy + (size_t)rec * 8 * A.nrep,
+                           true);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        A.counter[CNT_REC] = rec + 1;
+        A.counter[CNT_LAST] = 0;
+        __threadfence();
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------- host side
+static uint32_t expected_maxnb(const pb_sys *s, double rlist) {
+  double rho = 0.0;
+  for (uint32_t q = 0; q < s->nrep; ++q) {
+    const RepInfo &R = s->reps[q];
+    rho = std::max(rho, (double)R.natoms / (R.L[0] * R.L[1] * R.L[2]));
+  }
+  const double mean = 4.0 / 3.0 * M_PI * rlist * rlist * rlist * rho;
+  uint32_t m = (uint32_t)std::ceil(1.5 * mean + 48.0);
+  return (m + 15u) & ~15u;
+}
+
+static int setup_list_cells(pb_sys *s, double rlist) {
+  uint32_t cell0 = 0;
+  for (uint32_t q = 0; q < s->nrep; ++q) {
+    RepInfo &R = s->reps[q];
+    for (int a = 0; a < 3; ++a) {
+      if (!(2.0 * rlist < R.L[a]))
+        return pb_fail(s->ctx, PB_ERR_CELLS, "list cutoff must be below half the box");
+      uint32_t nb = (uint32_t)std::floor(R.L[a] / rlist);
+      if (nb < 1) nb = 1;
+      if (nb > 1024) nb = 1024;
+      R.nb[a] = nb;
+    }
+    R.cell0 = cell0;
+    cell0 += R.nb[0] * R.nb[1] * R.nb[2] + 1;   // + pad cell
+  }
+  s->list_ncell = cell0;
+  s->list_rlist = rlist;
+  PB_HIP(s->ctx, hipMemcpyAsync(s->d_reps, s->reps.data(), sizeof(RepInfo) * s->nrep,
+                                hipMemcpyHostToDevice, s->ctx->stream));
+  return pb_ensure_sort_ws(s, s->list_ncell);
+}
+
+static int ensure_list_buffers(pb_sys *s, uint32_t maxnb) {
+  int rc;
+  if (!s->r2buf) {
+    if ((rc = pb_alloc(s->ctx, &s->r2buf, s->npad)) || (rc = pb_alloc(s->ctx, &s->v2buf, s->npad)) ||
+        (rc = pb_alloc(s->ctx, &s->a2buf, s->npad)) || (rc = pb_alloc(s->ctx, &s->orig2, s->npad)) ||
+        (rc = pb_alloc(s->ctx, &s->type2, s->npad)) || (rc = pb_alloc(s->ctx, &s->move2, s->npad)) ||
+        (rc = pb_alloc(s->ctx, &s->rbuild, s->npad)) || (rc = pb_alloc(s->ctx, &s->nn, s->npad)))
+      return rc;
+  }
+  if (s->maxnb < maxnb) {
+    if ((rc = pb_alloc(s->ctx, &s->nl, (size_t)maxnb * s->npad))) return rc;
+    s->maxnb = maxnb;
+  }
+  return PB_OK;
+}
+
+// rebuild: cell ids -> counting sort -> reorder into cell order -> list.
+// F is not carried: every caller rewrites it before it is read again.
+static int rebuild(pb_sys *s) {
+  hipStream_t st = s->ctx->stream;
+  const uint32_t n = s->npad;
+  hipLaunchKernelGGL(k_list_cellid, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->r, s->orig,
+                     s->waveRep, s->d_reps, n, s->cellid);
+  PB_LAUNCH_CHECK(s->ctx, "k_list_cellid");
+  if (int rc = pb_counting_sort(s, s->list_ncell, s->orig)) return rc;
+  ReorderArgs A;
+  A.perm = s->perm;
+  A.r = s->r; A.v = s->v; A.a = s->a;
+  A.orig = s->orig; A.cellid = s->cellid; A.type = s->type; A.move = s->move;
+  A.r2 = s->r2buf; A.v2 = s->v2buf; A.a2 = s->a2buf;
+  A.orig2 = s->orig2; A.cell2 = s->rank; A.type2 = s->type2; A.move2 = s->move2;
+  A.rbuild = s->rbuild;
+  A.waveRep = s->waveRep; A.reps = s->d_reps; A.n = n;
+  hipLaunchKernelGGL(k_reorder, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A);
+  PB_LAUNCH_CHECK(s->ctx, "k_reorder");
+  std::swap(s->r, s->r2buf);
+  std::swap(s->v, s->v2buf);
+  std::swap(s->a, s->a2buf);
+  std::swap(s->orig, s->orig2);
+  std::swap(s->type, s->type2);
+  std::swap(s->move, s->move2);
+  hipLaunchKernelGGL(k_build_list, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->r, s->rank,
+                     s->cstart, s->waveRep, s->d_reps, n, s->list_rlist * s->list_rlist,
+                     s->maxnb, s->nl, s->nn, s->d_counter);
+  PB_LAUNCH_CHECK(s->ctx, "k_build_list");
+  s->ref_binned = false;
+  return PB_OK;
+}
+
+static StepArgs make_args(pb_sys *s, double dt) {
+  StepArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.nl = s->nl;
+  A.nn = s->nn;
+  A.rbuild = s->rbuild;
+  A.waveRep = s->waveRep;
+  A.reps = s->d_reps;
+  A.lj = s->d_lj;
+  A.partials = s->partials;
+  A.counter = s->d_counter;
+  A.npad = s->npad;
+  A.nrep = s->nrep;
+  A.dt = dt;
+  A.rc2 = s->rc2;
+  A.rc2f = (float)s->rc2;
+  const double hs = 0.5 * (s->list_rlist - std::sqrt(s->rc2));
+  A.half_skin2 = hs * hs;
+  A.s2 = s->lj.s2[0];
+  A.e24 = s->lj.e24[0];
+  A.e4 = s->lj.e4[0];
+  A.shift = s->lj.shift[0];
+  A.s2f = s->lj.s2f[0];
+  A.e24f = s->lj.e24f[0];
+  A.e4f = s->lj.e4f[0];
+  A.shiftf = s->lj.shiftf[0];
+  return A;
+}
+
+typedef void (*StepKernel)(StepArgs);
+
+template <bool MIXED, bool ONE>
+static StepKernel pick3(bool energy, bool wf, bool integ) {
+  if (integ) {
+    if (energy) return wf ? k_lj_step<MIXED, true, true, ONE, true> : k_lj_step<MIXED, true, false, ONE, true>;
+    return wf ? k_lj_step<MIXED, false, true, ONE, true> : k_lj_step<MIXED, false, false, ONE, true>;
+  }
+  return energy ? k_lj_step<MIXED, true, true, ONE, false> : k_lj_step<MIXED, false, true, ONE, false>;
+}
+
+static StepKernel pick_kernel(bool mixed, bool one, bool energy, bool wf, bool integ) {
+  if (mixed) return one ? pick3<true, true>(energy, wf, integ) : pick3<true, false>(energy, wf, integ);
+  return one ? pick3<false, true>(energy, wf, integ) : pick3<false, false>(energy, wf, integ);
+}
+
+static int launch_step(pb_sys *s, const StepArgs &A, bool mixed, bool energy, bool wf,
+                       bool integ) {
+  const bool one = s->lj.ntypes == 1;
+  StepKernel k = pick_kernel(mixed, one, energy, wf, integ);
+  hipLaunchKernelGGL(k, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, s->ctx->stream, A);
+  PB_LAUNCH_CHECK(s->ctx, "k_lj_step");
+  return PB_OK;
+}
+
+static int check_lj_ready(pb_sys *s) {
+  if (!s->lj_set) return pb_fail(s->ctx, PB_ERR_STATE, "LJ parameters not set");
+  if (!(s->rc2 > 0)) return pb_fail(s->ctx, PB_ERR_STATE, "LJ cutoff not set");
+  if (!s->cell_set) return pb_fail(s->ctx, PB_ERR_STATE, "cell not set");
+  if (s->w || s->radius)
+    return pb_fail(s->ctx, PB_ERR_STATE, "LJ path does not carry rotational/radius state");
+  return PB_OK;
+}
+
+static int read_counters(pb_sys *s, uint32_t *c) {
+  PB_HIP(s->ctx, hipMemcpyAsync(c, s->d_counter, 16 * 4, hipMemcpyDeviceToHost, s->ctx->stream));
+  PB_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
+  return PB_OK;
+}
+
+static int reset_counters(pb_sys *s) {
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s->ctx->stream, s->d_counter, 16u, 0u);
+  PB_LAUNCH_CHECK(s->ctx, "reset counters");
+  return PB_OK;
+}
+
+// build (or grow) the list so that no particle overflows it
+static int build_checked(pb_sys *s, double rlist) {
+  uint32_t want = std::max(s->maxnb, expected_maxnb(s, rlist));
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    if (int rc = ensure_list_buffers(s, want)) return rc;
+    if (int rc = reset_counters(s)) return rc;
+    if (int rc = rebuild(s)) return rc;
+    uint32_t c[16];
+    if (int rc = read_counters(s, c)) return rc;
+    if (c[CNT_OVF] == 0) {
+      s->last_maxnn = c[CNT_MAXNN];
+      return PB_OK;
+    }
+    want = (c[CNT_OVF] * 5 / 4 + 16u) & ~15u;
+  }
+  return pb_fail(s->ctx, PB_ERR_NLIST, "neighbour list keeps overflowing");
+}
+
+extern "C" int pb_lj_force(pb_sys *s, int precision, double *pe, double *virial) {
+  if (!s) return PB_ERR_ARG;
+  if (int rc = check_lj_ready(s)) return rc;
+  PB_HIP(s->ctx, hipSetDevice(s->ctx->device));
+  const double rc = std::sqrt(s->rc2);
+  if (int e = setup_list_cells(s, rc * (1.0 + 1e-12))) return e;
+  if (int e = build_checked(s, rc * (1.0 + 1e-12))) return e;
+  StepArgs A = make_args(s, 0.0);
+  A.rin = s->r;
+  A.vin = s->v;
+  A.Fout = s->F;
+  const bool energy = (pe || virial);
+  double *ebuf = nullptr;
+  if (energy) {
+    if (int e = pb_alloc(s->ctx, &ebuf, 8 * (size_t)s->nrep)) return e;
+    A.energy = ebuf;
+  }
+  if (int e = launch_step(s, A, precision == PB_MIXED, energy, true, false)) {
+    pb_free(ebuf);
+    return e;
+  }
+  if (energy) {
+    std::vector<double> h(8 * (size_t)s->nrep);
+    hipError_t e = hipMemcpyAsync(h.data(), ebuf, 8 * sizeof(double) * s->nrep,
+                                  hipMemcpyDeviceToHost, s->ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+    pb_free(ebuf);
+    if (e != hipSuccess) return pb_hip_check(s->ctx, e, "pb_lj_force energies");
+    for (uint32_t q = 0; q < s->nrep; ++q) {
+      if (pe) pe[q] = h[8 * q + 1];
+      if (virial) virial[q] = h[8 * q + 2];
+    }
+  }
+  return PB_OK;
+}
+
+// v(t-dt/2) <-> reference VV state conversions
+__global__ void k_to_half(double4 *v, const double4 *Fold, uint32_t n, double dt) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double4 V = v[s];
+  const double4 f = Fold[s];
+  const double dtm2 = 0.5 * dt * V.w;
+  V.x += f.x * dtm2;
+  V.y += f.y * dtm2;
+  V.z += f.z * dtm2;
+  v[s] = V;
+}
+__global__ void k_from_half(double4 *v, double4 *Fold, const double4 *F, uint32_t n, double dt) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  double4 V = v[s];
+  const double4 f = F[s];
+  const double dtm2 = 0.5 * dt * V.w;
+  V.x -= f.x * dtm2;
+  V.y -= f.y * dtm2;
+  V.z -= f.z * dtm2;
+  v[s] = V;
+  Fold[s] = make_double4(f.x, f.y, f.z, 0.0);
+}
+
+extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
+  if (!s || !o) return PB_ERR_ARG;
+  if (int rc = check_lj_ready(s)) return rc;
+  if (o->nsteps == 0) return PB_OK;
+  if (!(o->dt > 0) || o->skin < 0 || o->rebuild_every == 0)
+    return pb_fail(s->ctx, PB_ERR_ARG, "pb_lj_run: bad options");
+  PB_HIP(s->ctx, hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  const bool mixed = o->precision == PB_MIXED;
+  const double rlist = std::sqrt(s->rc2) + o->skin;
+  if (int rc = setup_list_cells(s, rlist)) return rc;
+  const uint32_t ne = o->energy_every;
+  const uint32_t nrec = ne ? (o->nsteps + ne - 1) / ne : 0;
+  double *ebuf = nullptr;
+  if (nrec) {
+    if (int rc = pb_alloc(s->ctx, &ebuf, (size_t)nrec * 8 * s->nrep)) return rc;
+  }
+  int rc = PB_OK;
+  uint32_t rebuilds = 0;
+  int skin_bad = 0;
+  // optional HIP-event timing on the context's stream
+  std::vector<hipEvent_t> ev_step, ev_reb;
+  auto mk = [&](std::vector<hipEvent_t> &v) -> hipEvent_t {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) == hipSuccess) v.push_back(e);
+    return e;
+  };
+  auto rec = [&](std::vector<hipEvent_t> &v) {
+    if (o->timing) {
+      hipEvent_t e = mk(v);
+      if (e) (void)hipEventRecord(e, st);
+    }
+  };
+  // reference VV state -> kick-drift state
+  hipLaunchKernelGGL(k_to_half, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->v, s->a,
+                     s->npad, o->dt);
+  if ((rc = build_checked(s, rlist))) goto done;
+  ++rebuilds;
+  if ((rc = reset_counters(s))) goto done;
+  {
+    StepArgs A = make_args(s, o->dt);
+    A.energy = ebuf;
+    for (uint32_t step = 0; step < o->nsteps; ++step) {
+      if (step > 0 && step % o->rebuild_every == 0) {
+        // rebuilds after the first use the capacity found by the first one;
+        // an overflow is reported at the end of the run
+        rec(ev_reb);
+        if ((rc = rebuild(s))) goto done;
+        rec(ev_reb);
+        ++rebuilds;
+        A.nl = s->nl;
+        A.nn = s->nn;
+      }
+      const bool energy = ne && (step % ne == 0);
+      const bool last = step + 1 == o->nsteps;
+      A.rin = s->r;
+      A.vin = s->v;
+      A.rout = s->r2buf;
+      A.vout = s->v2buf;
+      A.Fout = s->F;
+      rec(ev_step);
+      if ((rc = launch_step(s, A, mixed, energy, last, true))) goto done;
+      rec(ev_step);
+      std::swap(s->r, s->r2buf);
+      std::swap(s->v, s->v2buf);
+    }
+  }
+  // kick-drift state -> reference VV state: v(t) and Fold = F(t)
+  hipLaunchKernelGGL(k_from_half, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->v, s->a,
+                     s->F, s->npad, o->dt);
+  hipLaunchKernelGGL(k_wrap, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->r, s->orig,
+                     s->waveRep, s->d_reps, s->npad);
+  {
+    uint32_t c[16];
+    if ((rc = read_counters(s, c))) goto done;
+    skin_bad = c[CNT_SKIN] ? 1 : 0;
+    s->last_maxnn = std::max(s->last_maxnn, c[CNT_MAXNN]);
+    if (c[CNT_OVF]) {
+      rc = pb_fail(s->ctx, PB_ERR_NLIST, "neighbour list overflow during the run");
+      goto done;
+    }
+  }
+  if (energies && nrec) {
+    std::vector<double> h((size_t)nrec * 8 * s->nrep);
+    hipError_t e = hipMemcpyAsync(h.data(), ebuf, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      rc = pb_hip_check(s->ctx, e, "pb_lj_run energies");
+      goto done;
+    }
+    for (uint32_t r = 0; r < nrec; ++r)
+      for (uint32_t q = 0; q < s->nrep; ++q)
+        for (int c = 0; c < 3; ++c)
+          energies[((size_t)r * s->nrep + q) * 3 + c] = h[((size_t)r * s->nrep + q) * 8 + c];
+  }
+  s->last_rebuilds = rebuilds;
+  s->last_skin_violation = skin_bad;
+  s->last_step_ms = s->last_rebuild_ms = 0.0;
+  s->last_nstep_timed = (uint32_t)(ev_step.size() / 2);
+  s->last_nrebuild_timed = (uint32_t)(ev_reb.size() / 2);
+  for (size_t k = 0; k + 1 < ev_step.size(); k += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev_step[k], ev_step[k + 1]) == hipSuccess) s->last_step_ms += ms;
+  }
+  for (size_t k = 0; k + 1 < ev_reb.size(); k += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev_reb[k], ev_reb[k + 1]) == hipSuccess) s->last_rebuild_ms += ms;
+  }
+  if (skin_bad) rc = pb_fail(s->ctx, PB_ERR_SKIN, "skin violated: increase skin or rebuild more often");
+done:
+  for (hipEvent_t e : ev_step) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ev_reb) (void)hipEventDestroy(e);
+  pb_free(ebuf);
+  return rc;
+}
+
+extern "C" int pb_lj_last_run_timing(pb_sys *s, double *step_ms, uint32_t *nstep,
+                                     double *rebuild_ms, uint32_t *nrebuild) {
+  if (!s) return PB_ERR_ARG;
+  if (step_ms) *step_ms = s->last_step_ms;
+  if (nstep) *nstep = s->last_nstep_timed;
+  if (rebuild_ms) *rebuild_ms = s->last_rebuild_ms;
+  if (nrebuild) *nrebuild = s->last_nrebuild_timed;
+  return PB_OK;
+}
+
+extern "C" int pb_lj_last_run_stats(pb_sys *s, uint32_t *rebuilds, uint32_t *max_nn,
+                                    int *skin_violation) {
+  if (!s) return PB_ERR_ARG;
+  if (rebuilds) *rebuilds = s->last_rebuilds;
+  if (max_nn) *max_nn = s->last_maxnn;
+  if (skin_violation) *skin_violation = s->last_skin_violation;
+  return PB_OK;
+}

@@ -1,0 +1,211 @@
+// pb_sort.hip -- deterministic counting sort of storage slots by cell.
+//
+// Replaces the reference's calcBlockID sort chain (sortByBID_M1 insertion
+// pre-sort, single-block odd-even sortByBID and makeBindex,
+// kernelfuncs.cu:844-932, cudaCutoffBlock.cu:241-256), whose in-cell order is
+// unstable and timing dependent.  Here:
+//   1. k_count   : wave64 atomics on per-cell counters; equal keys in
+//                  neighbouring lanes are aggregated into one atomic.
+//   2. scan      : exclusive prefix sum, LDS-tiled (reduce / scan-of-sums /
+//                  scan-and-add), any length.
+//   3. k_scatter : slot = start[cell] + rank.
+//   4. k_cell_sort: inside every cell, order by a key (the original particle
+//                  index) -- the in-cell order produced by the atomics is
+//                  discarded, so the permutation is the canonical stable one.
+#include "pb_internal.h"
+
+#define SCAN_BS 256
+#define SCAN_PER 8
+#define SCAN_CH (SCAN_BS * SCAN_PER)
+
+__global__ void k_fill_u32(uint32_t *x, uint32_t n, uint32_t val) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = val;
+}
+
+// inclusive wave scan (64 lanes)
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  const uint32_t lane = pb_lane();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  return x;
+}
+
+// exclusive block scan of one value per thread; returns the exclusive prefix
+// and the block total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t *total) {
+  __shared__ uint32_t wsum[SCAN_BS / 64];
+  const uint32_t lane = pb_lane(), w = threadIdx.x / 64;
+  const uint32_t inc = wave_incl_scan(x);
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (uint32_t k = 0; k < blockDim.x / 64; ++k) {
+    if (k < w) off += wsum[k];
+    tot += wsum[k];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - x;
+}
+
+__global__ void __launch_bounds__(SCAN_BS) k_scan_reduce(const uint32_t *in, uint32_t n,
+                                                         uint32_t *bsum) {
+  const uint32_t base = blockIdx.x * SCAN_CH + threadIdx.x * SCAN_PER;
+  uint32_t x = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; ++k)
+    if (base + k < n) x += in[base + k];
+  uint32_t tot;
+  (void)block_excl_scan(x, &tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// single block: exclusive scan of nb block sums in place; out_total <- sum
+__global__ void __launch_bounds__(SCAN_BS) k_scan_bsums(uint32_t *bsum, uint32_t nb,
+                                                        uint32_t *out_total) {
+  const uint32_t per = (nb + SCAN_BS - 1) / SCAN_BS;
+  const uint32_t b0 = threadIdx.x * per;
+  uint32_t x = 0;
+  for (uint32_t k = 0; k < per; ++k)
+    if (b0 + k < nb) x += bsum[b0 + k];
+  uint32_t tot;
+  uint32_t run = block_excl_scan(x, &tot);
+  for (uint32_t k = 0; k < per; ++k)
+    if (b0 + k < nb) {
+      const uint32_t v = bsum[b0 + k];
+      bsum[b0 + k] = run;
+      run += v;
+    }
+  if (threadIdx.x == 0) *out_total = tot;
+}
+
+__global__ void __launch_bounds__(SCAN_BS) k_scan_final(const uint32_t *in, uint32_t n,
+                                                        const uint32_t *bsum, uint32_t *out) {
+  const uint32_t base = blockIdx.x * SCAN_CH + threadIdx.x * SCAN_PER;
+  uint32_t v[SCAN_PER];
+  uint32_t x = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0u;
+    x += v[k];
+  }
+  uint32_t tot;
+  uint32_t run = block_excl_scan(x, &tot) + bsum[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; ++k)
+    if (base + k < n) {
+      out[base + k] = run;
+      run += v[k];
+    }
+}
+
+int pb_exclusive_scan(pb_sys *s, const uint32_t *in, uint32_t *out, uint32_t n) {
+  const uint32_t nb = (n + SCAN_CH - 1) / SCAN_CH;
+  const size_t need = (size_t)nb + 1;
+  if (s->scan_tmp_n < need) {
+    if (int rc = pb_alloc(s->ctx, &s->scan_tmp, need)) return rc;
+    s->scan_tmp_n = need;
+  }
+  hipStream_t st = s->ctx->stream;
+  if (n == 0) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, out, 1u, 0u);
+    PB_LAUNCH_CHECK(s->ctx, "scan(empty)");
+    return PB_OK;
+  }
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BS), 0, st, in, n, s->scan_tmp);
+  hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(SCAN_BS), 0, st, s->scan_tmp, nb, out + n);
+  hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(SCAN_BS), 0, st, in, n, s->scan_tmp, out);
+  PB_LAUNCH_CHECK(s->ctx, "pb_exclusive_scan");
+  return PB_OK;
+}
+
+// Wave-aggregated counting: lanes holding the same cell as their left
+// neighbour form a run; the run head does one atomicAdd for the whole run and
+// broadcasts the base.  Correct for any input; one atomic per run when the
+// input is already nearly cell-ordered (the steady state after a re-sort).
+__global__ void k_count(const uint32_t *cellid, uint32_t n, uint32_t *counts, uint32_t *rank) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lane = pb_lane();
+  const bool valid = s < n;
+  const uint32_t c = valid ? cellid[s] : PB_NONE;
+  const uint32_t left = __shfl_up(c, 1, 64);
+  const bool head = valid && (lane == 0 || left != c);
+  const uint64_t heads = __ballot(head);
+  const uint64_t validm = __ballot(valid);
+  // my run head: highest head lane <= lane
+  const uint64_t le = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
+  const uint32_t hl = le ? (63u - (uint32_t)__clzll((long long)le)) : 0u;
+  // run length (for heads): distance to the next head or the end of valid lanes
+  uint32_t base = 0;
+  if (head) {
+    const uint64_t gt = heads & ~((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
+    const uint32_t nextl = gt ? (uint32_t)__ffsll((long long)gt) - 1u : (uint32_t)__popcll(validm);
+    base = atomicAdd(&counts[c], nextl - lane);
+  }
+  base = __shfl(base, (int)hl, 64);
+  if (valid) rank[s] = base + (lane - hl);
+}
+
+__global__ void k_scatter(const uint32_t *cellid, const uint32_t *rank, const uint32_t *start,
+                          uint32_t n, uint32_t *perm) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) perm[start[cellid[s]] + rank[s]] = s;
+}
+
+// one thread per cell: insertion sort of the cell's slots by key
+__global__ void k_cell_sort(const uint32_t *start, uint32_t ncell, const uint32_t *key,
+                            uint32_t *perm) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const uint32_t b = start[c], e = start[c + 1];
+  for (uint32_t i = b + 1; i < e; ++i) {
+    const uint32_t p = perm[i];
+    const uint32_t kp = key[p];
+    uint32_t j = i;
+    while (j > b) {
+      const uint32_t q = perm[j - 1];
+      if (key[q] <= kp) break;
+      perm[j] = q;
+      --j;
+    }
+    perm[j] = p;
+  }
+}
+
+int pb_ensure_sort_ws(pb_sys *s, uint32_t ncell) {
+  int rc;
+  if (!s->cellid) {
+    if ((rc = pb_alloc(s->ctx, &s->cellid, s->npad)) || (rc = pb_alloc(s->ctx, &s->rank, s->npad)) ||
+        (rc = pb_alloc(s->ctx, &s->perm, s->npad)))
+      return rc;
+  }
+  if (s->ncell_cap < ncell) {
+    if ((rc = pb_alloc(s->ctx, &s->ccount, (size_t)ncell + 1)) ||
+        (rc = pb_alloc(s->ctx, &s->cstart, (size_t)ncell + 1)))
+      return rc;
+    s->ncell_cap = ncell;
+  }
+  return PB_OK;
+}
+
+int pb_counting_sort(pb_sys *s, uint32_t ncell, const uint32_t *key) {
+  if (int rc = pb_ensure_sort_ws(s, ncell)) return rc;
+  hipStream_t st = s->ctx->stream;
+  const uint32_t n = s->npad;
+  hipLaunchKernelGGL(k_fill_u32, dim3(pb_blocks(ncell + 1, 256)), dim3(256), 0, st, s->ccount,
+                     ncell + 1, 0u);
+  hipLaunchKernelGGL(k_count, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->cellid, n, s->ccount,
+                     s->rank);
+  PB_LAUNCH_CHECK(s->ctx, "k_count");
+  if (int rc = pb_exclusive_scan(s, s->ccount, s->cstart, ncell)) return rc;
+  hipLaunchKernelGGL(k_scatter, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->cellid, s->rank,
+                     s->cstart, n, s->perm);
+  hipLaunchKernelGGL(k_cell_sort, dim3(pb_blocks(ncell, 256)), dim3(256), 0, st, s->cstart, ncell,
+                     key, s->perm);
+  PB_LAUNCH_CHECK(s->ctx, "pb_counting_sort");
+  return PB_OK;
+}

@@ -1,0 +1,49 @@
+"""The C-ABI library loads and exports every symbol include/pbamd.h declares;
+the oracle exports what pb_oracle.h declares.  No GPU calls."""
+import os
+import re
+import ctypes
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    txt = open(header).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(pbo?_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_symbols_exported():
+    import particlebase_amd as pb
+    lib = pb.load()
+    names = declared(os.path.join(ROOT, "include", "pbamd.h"))
+    assert len(names) >= 35
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(pb.EXPORTS) <= set(names)
+
+
+def test_oracle_symbols_exported(orc):
+    names = declared(os.path.join(ROOT, "oracle", "pb_oracle.h"))
+    lib = orc.lib()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_product_does_not_reference_oracle():
+    # the product path never loads, links or imports the oracle
+    for dirpath, _, files in os.walk(os.path.join(ROOT, "particlebase_amd")):
+        for f in files:
+            if f.endswith((".py", ".hip", ".h", ".hh", ".cpp", ".cc")) or f == "Makefile":
+                txt = open(os.path.join(dirpath, f), errors="ignore").read()
+                assert "pb_oracle" not in txt and "liboracle" not in txt, f
+
+
+def test_context_fails_loudly_without_device():
+    import particlebase_amd as pb
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU driver is present")
+    with pytest.raises(pb.PBError):
+        pb.Context(0)

@@ -1,0 +1,178 @@
+"""GPU parity through the C ABI: binning / sort (bit-exact), neighbour table,
+neighbour counts (bit-exact), g(r) histograms (bit-exact) and the
+element-wise integrators, each against the oracle on the same inputs."""
+import numpy as np
+import pytest
+
+import particlebase_amd as pb
+from particlebase_amd import workloads
+
+pytestmark = pytest.mark.gpu
+NONE = 0xFFFFFFFF
+
+
+def say(msg):
+    print(f"\n[case] {msg}", flush=True)
+
+
+def make_sys(ctx, r, cell, m=None, v=None, counts=None):
+    n = r.shape[0]
+    s = pb.System(ctx, [n] if counts is None else counts)
+    s.set_cell(cell)
+    s.import_particles(r, np.ones(n) if m is None else m, v=v)
+    return s
+
+
+def bins_case(ctx, orc, r32, cell32, rmax):
+    s = make_sys(ctx, r32.astype(np.float64), cell32.astype(np.float64))
+    s.setup_cells(rmax)
+    s.bin()
+    pid, bid, bindex = s.get_bins()
+    rc, bn, bl = orc.setup_cells(cell32, rmax, True)
+    gbn, gbl = s.cell_info()
+    assert np.array_equal(bn, gbn) and np.array_equal(bl, gbl)
+    obid = orc.calc_bid(r32, bl, bn)
+    opid, obs, obindex = orc.sort_bins(obid, int(np.prod(bn.astype(np.int64))))
+    assert np.array_equal(pid, opid)
+    assert np.array_equal(bid, obs)
+    assert np.array_equal(bindex, obindex)
+    return s
+
+
+def test_bin_fcc_lattice_boundaries(ctx, orc):
+    # a perfect lattice puts whole planes of particles exactly on cell faces
+    say("bin fcc 10^3x4 (config 1 start)")
+    w = workloads.lj_config1()
+    bins_case(ctx, orc, w.r.astype(np.float32), w.cell.astype(np.float32), 2.5)
+
+
+def test_bin_random_and_adversarial(ctx, orc, rng):
+    say("bin adversarial n=5003")
+    L = 16.795962
+    cell = np.array([0, L, 0, L, 0, L], dtype=np.float32)
+    r = rng.uniform(0, L, size=(5003, 3)).astype(np.float32)
+    r[:40] = np.float32(L)                              # on the upper face
+    r[40:60] = np.nextafter(np.float32(L), np.float32(0))
+    r[60:80] = -np.float32(0.3)                          # below the box
+    r[80:90] = np.float32(3 * L)                         # far above (d > nb)
+    r[100:3000, 0] = np.maximum(r[100:3000, 0], 3.0)     # cell column x=0 sparse
+    bins_case(ctx, orc, r, cell, 2.5)
+
+
+def test_bin_empty_cell_zero(ctx, orc, rng):
+    say("bin with empty cell 0")
+    cell = np.array([0, 20.0, 0, 20.0, 0, 20.0], dtype=np.float32)
+    r = rng.uniform(5.0, 20.0, size=(3000, 3)).astype(np.float32)
+    s = bins_case(ctx, orc, r, cell, 3.0)
+    _, _, bindex = s.get_bins()
+    assert bindex[0] == NONE
+
+
+def test_neighbor_table(ctx, orc):
+    say("neighbour table")
+    cell = np.array([0, 13.0, 0, 16.0, 0, 19.0])
+    s = make_sys(ctx, np.full((10, 3), 1.0), cell)
+    s.setup_cells(3.0)
+    t = s.neighbor_table()
+    rc, bn, _ = orc.setup_cells(cell.astype(np.float32), 3.0, True)
+    assert np.array_equal(t, orc.neighbor_table(bn, 1, True))
+
+
+def test_coord_counts_bitexact(ctx, orc):
+    say("coord counts config1 jittered")
+    w = workloads.lj_config1()
+    rng = np.random.default_rng(3)
+    r = ((w.r + rng.normal(scale=0.1, size=w.r.shape)) % w.L).astype(np.float32)
+    cell = w.cell.astype(np.float32)
+    s = make_sys(ctx, r.astype(np.float64), cell.astype(np.float64))
+    s.setup_cells(2.8)
+    rmax2t = np.float32((np.sqrt(np.float32(6.25)) + np.float32(0.3)) ** 2)
+    c = s.coord_counts(rmax2t)
+    oc = orc.coord_counts_f32(r, cell, 2.8, rmax2t)
+    assert np.array_equal(c, oc)
+    assert c.mean() > 60
+
+
+def test_gr_bitexact_single_and_batched(ctx, orc):
+    say("g(r) 3 replicas")
+    reps = workloads.lj_replicas(3, 500, 3000, seed=21)
+    rng = np.random.default_rng(4)
+    rs, cells, rsteps = [], [], []
+    for w in reps.systems:
+        w.r = ((w.r + rng.normal(scale=0.07, size=w.r.shape)) % w.L).astype(np.float32)
+        rs.append(w.r)
+        cells.append(w.cell)
+        rsteps.append(np.float32(w.L / 400.0))
+    r = np.concatenate(rs)
+    s = make_sys(ctx, r.astype(np.float64), np.concatenate(cells), counts=reps.counts)
+    h = s.gr_hist(np.array(rsteps), 200)
+    for k, w in enumerate(reps.systems):
+        ri = np.float32(1.0 / np.float64(rsteps[k]))
+        oh = orc.gr_hist_f32(w.r, w.cell.astype(np.float32), ri, 200)
+        assert np.array_equal(h[k], oh), k
+        assert int(h[k].sum()) == w.n * (w.n - 1) // 2
+
+
+def test_vv_wrap_mv2(ctx, orc, rng):
+    say("vv + wrap + mv2")
+    n = 3001
+    L = 9.0
+    cell = np.array([0, L, -1.0, L - 1, 2.0, L + 2])
+    r = rng.uniform(0, L, size=(n, 3)) + np.array([0, -1, 2])
+    v = rng.normal(size=(n, 3))
+    F = rng.normal(size=(n, 3))
+    Fo = rng.normal(size=(n, 3))
+    m = rng.uniform(0.5, 2.0, n)
+    s = make_sys(ctx, r, cell, m=m, v=v)
+    s.set("F", F)
+    s.set("a", Fo)
+    s.vv_step(0.01)
+    s.treat_periodic()
+    r1, v1, fo1 = orc.vv_step(r, v, F, Fo, 1.0 / m, 0.01)
+    r1 = orc.wrap(r1, cell).reshape(n, 3)
+    assert np.allclose(s.get("r"), r1, rtol=0, atol=1e-13)
+    assert np.allclose(s.get("v"), v1.reshape(n, 3), rtol=1e-15, atol=1e-15)
+    assert np.array_equal(s.get("a"), F)
+    assert s.mv2()[0] == pytest.approx(orc.mv2(v1, 1.0 / m), rel=1e-13)
+
+
+def test_lf_and_rollback(ctx, rng):
+    say("leap-frog step + rollback")
+    n = 2000
+    r = rng.uniform(0, 5, size=(n, 3))
+    v = rng.normal(size=(n, 3))
+    a = rng.normal(size=(n, 3))
+    fixed = (rng.uniform(size=n) < 0.1).astype(np.uint8)
+    s = pb.System(ctx, [n])
+    s.set_cell([0, 5, 0, 5, 0, 5])
+    s.import_particles(r, np.ones(n), v=v, a=a, is_fixed=fixed)
+    dt = 0.003
+    s.lf_step(dt)
+    s.lf_step(dt)
+    mv = (1 - fixed)[:, None]
+    v1 = v + a * dt * mv
+    r1 = r + v1 * dt * mv
+    v2 = v1 + a * dt * mv
+    r2 = r1 + v2 * dt * mv
+    assert np.allclose(s.get("r"), r2, atol=1e-14) and np.allclose(s.get("v"), v2, atol=1e-14)
+    # rollbackLeapFrog_F4: r goes back two steps, v one half-step pair
+    s.lf_rollback(dt)
+    assert np.allclose(s.get("r"), r, atol=1e-12) and np.allclose(s.get("v"), v1, atol=1e-12)
+
+
+def test_inspect_velocity(ctx, rng):
+    say("inspect velocity")
+    n = 1000
+    v = rng.normal(scale=0.1, size=(n, 3))
+    s = pb.System(ctx, [n])
+    s.set_cell([0, 5, 0, 5, 0, 5])
+    s.import_particles(rng.uniform(0, 5, (n, 3)), np.ones(n), v=v)
+    vmax = np.sqrt((v ** 2).sum(axis=1)).max()
+    # vlim 2.7 truncates to 2 (uint32 argument of inspectV_F4)
+    res, ratio, imax = s.inspect_velocity(2.7, 0.9, 0.5)
+    assert ratio == pytest.approx(vmax / 2.0, rel=1e-14)
+    assert imax == int(np.argmax((v ** 2).sum(axis=1)))
+    assert res == (1 if ratio >= np.float32(0.9) else 0 if ratio >= np.float32(0.5) else -1)
+    v[7] = [3.0, 0, 0]
+    s.set("v", v)
+    assert s.inspect_velocity(2.7, 0.9, 0.5)[0] == 1

@@ -1,0 +1,171 @@
+"""Pins of the oracle's LJ force / energy / integrator restatement
+(MDpairForce kerneltemplate.hh:530-564, potentialLJ kernelinline.hh:86-109,
+propagateVelocityVerlet_F4 kernelfuncs.cu:333-367, applyPeriodicCondition_F4
+kernelfuncs.cu:87-139).  Pinned by the two-body closed form, FCC lattice
+sums, Newton's third law, -grad PE, brute force and energy conservation."""
+import math
+
+import numpy as np
+import pytest
+
+from particlebase_amd import workloads
+
+RC = 2.5
+RC2 = RC * RC
+
+
+def lj_closed(r, sig=1.0, eps=1.0, rc=RC, shifted=True):
+    """Textbook LJ: phi = 4 eps ((s/r)^12 - (s/r)^6) - phi(rc); |F| = -dphi/dr."""
+    def phi(x):
+        return 4 * eps * ((sig / x) ** 12 - (sig / x) ** 6)
+    f = 24 * eps / r * (2 * (sig / r) ** 12 - (sig / r) ** 6)
+    return f, phi(r) - (phi(rc) if shifted else 0.0)
+
+
+@pytest.mark.parametrize("dist", [0.95, 1.12246, 1.5, 2.2, 2.49])
+def test_two_body_closed_form(orc, dist, rng):
+    cell = np.array([0, 20.0, 0, 20.0, 0, 20.0])
+    u = rng.normal(size=3)
+    u /= np.linalg.norm(u)
+    r = np.array([[10.0, 10.0, 10.0], 10.0 + dist * u])
+    p = orc.lj_params([1.0, 1.0], 1, RC2, True)
+    F, pe, w = orc.lj_forces_brute(r, np.zeros(2), cell, RC2, p)
+    fmag, phi = lj_closed(dist)
+    assert np.allclose(F[0], -fmag * u, rtol=1e-13, atol=1e-13)
+    assert np.allclose(F[1], fmag * u, rtol=1e-13, atol=1e-13)
+    assert pe == pytest.approx(phi, rel=1e-13, abs=1e-15)
+    assert w == pytest.approx(fmag * dist, rel=1e-13)
+    # the reference's phi(rc) at rc = 2.5 (SURVEY config 1)
+    assert lj_closed(RC)[1] == 0.0
+    assert 4 * (RC ** -12 - RC ** -6) == pytest.approx(-0.016316891136, rel=1e-9)
+
+
+def test_two_types_table(orc):
+    # asymmetric table entries must be looked up as (type_i, type_j)
+    se = [1.0, 1.0, 1.2, 0.5, 1.2, 0.5, 0.9, 2.0]
+    p = orc.lj_params(se, 2, 9.0, shifted=False)
+    cell = np.array([0, 30.0] * 3)
+    r = np.array([[5.0, 5.0, 5.0], [6.3, 5.0, 5.0]])
+    for ti, tj, (sg, ep) in [(0, 1, (1.2, 0.5)), (1, 1, (0.9, 2.0)), (0, 0, (1.0, 1.0))]:
+        F, pe, _ = orc.lj_forces_brute(r, np.array([ti, tj]), cell, 9.0, p)
+        f, phi = lj_closed(1.3, sg, ep, rc=3.0, shifted=False)
+        assert F[1, 0] == pytest.approx(f, rel=1e-13)
+        assert pe == pytest.approx(phi, rel=1e-13)
+
+
+def fcc_lattice_sum(a, rc):
+    """Per-atom PE of a perfect FCC lattice by direct enumeration of sites."""
+    k = int(math.ceil(rc / a)) + 1
+    pts = workloads.fcc_sites(2 * k + 1, a) - k * a
+    d = np.linalg.norm(pts, axis=1)
+    d = d[(d > 1e-9) & (d < rc)]
+    return 0.5 * sum(lj_closed(x)[1] for x in d), d
+
+
+def test_fcc_lattice_forces_and_energy(orc):
+    w = workloads.lj_fcc(8, rho=0.8442)
+    p = orc.lj_params([1.0, 1.0], 1, RC2, True)
+    F, pe, vir = orc.lj_forces_cells(w.r, w.type, w.cell, RC2, p)
+    assert np.abs(F).max() < 1e-11
+    a = w.L / 8
+    per_atom, d = fcc_lattice_sum(a, RC)
+    assert pe / w.n == pytest.approx(per_atom, rel=1e-12)
+    assert len(d) == 12 + 6 + 24 + 12          # shells a/sqrt2, a, a*sqrt1.5, a*sqrt2 < rc
+
+
+def liquid(n_cells=6, seed=3, jitter=0.12):
+    w = workloads.lj_fcc(n_cells, rho=0.8442, seed=seed)
+    rng = np.random.default_rng(seed)
+    w.r = (w.r + rng.normal(scale=jitter, size=w.r.shape)) % w.L
+    return w
+
+
+def test_cells_equal_brute_and_newton3(orc):
+    w = liquid()
+    p = orc.lj_params([1.0, 1.0], 1, RC2, True)
+    F1, pe1, w1 = orc.lj_forces_cells(w.r, w.type, w.cell, RC2, p)
+    F2, pe2, w2 = orc.lj_forces_brute(w.r, w.type, w.cell, RC2, p)
+    scale = np.sqrt((F2 ** 2).sum(axis=1).mean())
+    assert np.abs(F1 - F2).max() < 1e-12 * scale
+    assert pe1 == pytest.approx(pe2, rel=1e-12) and w1 == pytest.approx(w2, rel=1e-12)
+    assert np.abs(F1.sum(axis=0)).max() < 1e-9 * scale
+
+
+def test_translation_invariance(orc):
+    w = liquid(seed=4)
+    p = orc.lj_params([1.0, 1.0], 1, RC2, True)
+    F1, pe1, _ = orc.lj_forces_cells(w.r, w.type, w.cell, RC2, p)
+    shift = np.array([0.731, -3.2, 5.55])
+    F2, pe2, _ = orc.lj_forces_cells((w.r + shift) % w.L, w.type, w.cell, RC2, p)
+    assert np.abs(F1 - F2).max() < 1e-9 and pe1 == pytest.approx(pe2, rel=1e-11)
+
+
+def test_force_is_minus_gradient(orc):
+    w = liquid(seed=5)
+    p = orc.lj_params([1.0, 1.0], 1, RC2, True)
+    F, _, _ = orc.lj_forces_brute(w.r, w.type, w.cell, RC2, p)
+    h = 1e-6
+    for i in (0, 17, 300):
+        for a in range(3):
+            rp = w.r.copy()
+            rm = w.r.copy()
+            rp[i, a] += h
+            rm[i, a] -= h
+            ep = orc.lj_forces_brute(rp, w.type, w.cell, RC2, p)[1]
+            em = orc.lj_forces_brute(rm, w.type, w.cell, RC2, p)[1]
+            assert -(ep - em) / (2 * h) == pytest.approx(F[i, a], rel=1e-5, abs=1e-5)
+
+
+def test_vv_step_constant_force(orc, rng):
+    n = 50
+    r = rng.normal(size=(n, 3))
+    v = rng.normal(size=(n, 3))
+    F = rng.normal(size=(n, 3))
+    minv = rng.uniform(0.5, 2.0, size=n)
+    dt = 0.01
+    # Fold == F: a uniformly accelerated particle
+    r1, v1, fo = orc.vv_step(r, v, F, F, minv, dt)
+    a = F * minv[:, None]
+    assert np.allclose(v1.reshape(n, 3), v + a * dt, rtol=1e-14, atol=1e-15)
+    assert np.allclose(r1.reshape(n, 3), r + (v + a * dt) * dt + 0.5 * a * dt * dt,
+                       rtol=1e-14, atol=1e-15)
+    assert np.array_equal(fo.reshape(n, 3), F)
+    # Fold == 0: the reference's first-step half kick
+    r2, v2, _ = orc.vv_step(r, v, F, np.zeros_like(F), minv, dt)
+    assert np.allclose(v2.reshape(n, 3), v + 0.5 * a * dt, rtol=1e-14, atol=1e-15)
+
+
+def test_wrap(orc, rng):
+    cell = np.array([-1.0, 7.0, 2.0, 5.0, 0.0, 10.0])
+    x = np.stack([rng.uniform(-17, 23, 3000), rng.uniform(-7, 12, 3000),
+                  rng.uniform(-25, 30, 3000)], axis=1)
+    y = orc.wrap(x, cell).reshape(-1, 3)
+    lo = cell[0::2]
+    L = cell[1::2] - lo
+    exp = lo + np.mod(x - lo, L)
+    assert np.allclose(y, exp, atol=1e-12)
+    assert np.all(y >= lo) and np.all(y < cell[1::2])
+    # the FP edge: -tiny wraps onto the upper face -> put on the lower face
+    c0 = np.array([0.0, 8.0, 0.0, 8.0, 0.0, 8.0])
+    z = orc.wrap(np.array([[-1e-300, 2.0, 0.0]]), c0).reshape(3)
+    assert z[0] == 0.0 and z[1] == 2.0
+
+
+def test_mv2(orc, rng):
+    v = rng.normal(size=(100, 3))
+    minv = rng.uniform(0.5, 2, 100)
+    assert orc.mv2(v, minv) == pytest.approx(((v ** 2).sum(axis=1) / minv).sum(), rel=1e-13)
+
+
+def test_run_energy_conservation(orc):
+    w = workloads.lj_fcc(5, rho=0.8442, T=0.72, seed=6)   # N = 500, L = 8.4 -> 3 cells
+    p = orc.lj_params([1.0, 1.0], 1, RC2, True)
+    minv = 1.0 / w.m
+    r, v, fo, e = orc.lj_run(w.r, w.v, np.zeros_like(w.r), minv, w.type, w.cell, RC, RC2, p,
+                             0.002, 400)
+    etot = e[1:, 0] + e[1:, 1]
+    assert np.ptp(etot) < 1e-3 * abs(etot.mean())
+    # the lattice melts: PE rises from the lattice sum while E stays put
+    assert e[-1, 1] > e[0, 1] + 0.1 * w.n
+    # KE reported is 1/2 m v^2 of v(t)
+    assert e[-1, 0] > 0
