@@ -7,8 +7,11 @@ no fallback of any kind -- a missing library raises at import of a system.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -149,6 +152,19 @@ def _p(arr, ctype):
     return arr.ctypes.data_as(C.POINTER(ctype))
 
 
+_LIVE_CONTEXTS = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    # release device objects before the HIP runtime tears itself down
+    for c in list(_LIVE_CONTEXTS):
+        try:
+            c.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One HIP device + one stream (pb_ctx)."""
 
@@ -159,6 +175,8 @@ class Context:
         if rc != PB_OK:
             raise PBError(rc, f"pb_ctx_create(device={device}) failed (is a GPU visible?)")
         self.h = h
+        self.systems = weakref.WeakSet()
+        _LIVE_CONTEXTS.add(self)
 
     def check(self, rc):
         if rc != PB_OK:
@@ -176,10 +194,14 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
+            for s in list(self.systems):
+                s.close()
             self.lib.pb_ctx_destroy(self.h)
             self.h = None
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -199,14 +221,18 @@ class System:
         h = C.c_void_p()
         ctx.check(self.lib.pb_sys_create(ctx.h, self.nrep, _p(counts, C.c_uint32), C.byref(h)))
         self.h = h
+        ctx.systems.add(self)
 
     # --- lifecycle
     def close(self):
         if getattr(self, "h", None):
-            self.lib.pb_sys_destroy(self.h)
+            if self.ctx.h:
+                self.lib.pb_sys_destroy(self.h)
             self.h = None
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -1,0 +1,128 @@
+// cudaParticleMD.hh -- LJ molecular dynamics (reference cudaParticleMD.hh/.cu).
+// calcForce computes F from the current positions with the engine's
+// Verlet-list pair kernel (FP64 by default, or FP32 pair math with FP64
+// accumulation); calcPotentialE returns the correct 1/2 sum phi in FP64
+// (the reference's per-cell assignment bug, kerneltemplate.hh:582, is not
+// reproduced).  runFused() is the engine's fused loop (no reference
+// counterpart as one call).
+#pragma once
+#include <cmath>
+#include <numeric>
+#include <vector>
+#include "cudaCutoffBlock.hh"
+#include "cudaParticleVV.hh"
+#include "cudaSortedPositions.hh"
+
+class cudaParticleMD : public cudaParticleVV,
+                       virtual public cudaCutoffBlock,
+                       virtual public cudaSortedPositions {
+protected:
+  real m0 = 0;
+  real mv2 = 0;
+  double lastPE = 0.0, lastVirial = 0.0;
+  bool paramsSet = false;
+  void syncCutoff() {
+    if (rmax2 != cutoffSent || shifted != shiftSent) {
+      pbamd::check(pb_lj_set_cutoff(sys, rmax2, shifted ? 1 : 0), "rmax2");
+      cutoffSent = rmax2;
+      shiftSent = shifted;
+    }
+  }
+  double cutoffSent = -1;
+  bool shiftSent = false;
+
+public:
+  real kB = 1.0;
+  real rmax2 = 0.0;
+  real thickness = 0.0;
+  bool shifted = false;          //!< shift energies so phi(rc) = 0 (forces unchanged)
+  int precision = PB_FP64;       //!< PB_FP64 or PB_MIXED pair math
+
+  void setup(int n) { cudaParticleVV::setup(n); }
+
+  // calcBlockID "steals" both parents (cudaParticleMD.hh:64-67)
+  void calcBlockID(void) {
+    cudaCutoffBlock::calcBlockID();
+    cudaSortedPositions::calcBlockID();
+  }
+  void setupCutoffBlock(real rmax, bool periodic = true) {
+    cudaCutoffBlock::setupCutoffBlock(rmax, periodic);
+    cudaSortedPositions::setupSortedPositions();
+  }
+
+  void calcForce(void) override {
+    syncCutoff();
+    pbamd::check(pb_lj_force(sys, precision, &lastPE, &lastVirial), "cudaParticleMD::calcForce");
+  }
+
+  void setM(void) {}
+
+  // sum m v^2 in FP64 (cudaParticleMD.cu:124-133)
+  real calcMV2(void) {
+    double out = 0;
+    pbamd::check(pb_mv2(sys, &out), "calcMV2");
+    return (real)out;
+  }
+  real calcTemp(void) {
+    mv2 = calcMV2();
+    return mv2 / (N * 3 * kB);
+  }
+  real calcKineticE(void) const { return mv2 / 2.0; }
+  real calcPotentialE(void) {
+    syncCutoff();
+    double pe = 0;
+    pbamd::check(pb_lj_force(sys, precision, &pe, &lastVirial), "calcPotentialE");
+    lastPE = pe;
+    return (real)pe;
+  }
+  double lastPotentialE(void) const { return lastPE; }
+  double lastVirialW(void) const { return lastVirial; }
+
+  // scaleTemp, cudaParticleMD.cu:168-175
+  real scaleTemp(real Temp) {
+    const real T0 = calcTemp();
+    const real s = std::sqrt(static_cast<real>(3 * N - 1) / static_cast<real>(3 * N) * Temp / T0);
+    pbamd::check(pb_scale_velocities(sys, s), "scaleTemp");
+    return s;
+  }
+
+  // setLJparams, cudaParticleMD.cu:200-202 (sigma, epsilon per type pair)
+  void setLJparams(const std::vector<real> &p, uint32_t elemnum) {
+    std::vector<double> d(p.begin(), p.end());
+    pbamd::check(pb_lj_set_params(sys, d.data(), (int)elemnum), "setLJparams");
+    paramsSet = true;
+  }
+
+  void import(const std::vector<ParticleBase> &P) {
+    cudaParticleBase::import(P);
+    const double s = std::accumulate(P.begin(), P.end(), 0.0,
+                                     [](double acc, const ParticleBase &c) { return acc + c.m; });
+    m0 = (real)(s / P.size());
+  }
+
+  // neighbour counts of makeNList step 1 (cudaParticleMD.cu:343-366)
+  std::vector<uint32_t> coordinationNumbers(void) {
+    const real rmax2t = (std::sqrt(rmax2) + thickness) * (std::sqrt(rmax2) + thickness);
+    std::vector<uint32_t> c(N);
+    pbamd::check(pb_coord_counts(sys, rmax2t, c.data()), "coordinationNumbers");
+    return c;
+  }
+
+  // fused loop: nsteps of [calcForce, TimeEvolution, treatPeriodicCondition]
+  std::vector<double> runFused(uint32_t nsteps, real dt, int prec = PB_MIXED, double skin = 0.3,
+                               uint32_t rebuild = 20, uint32_t energy_every = 0) {
+    syncCutoff();
+    pb_run_opts o{};
+    o.dt = dt;
+    o.nsteps = nsteps;
+    o.precision = prec;
+    o.skin = skin;
+    o.rebuild_every = rebuild;
+    o.energy_every = energy_every;
+    const uint32_t nrec = energy_every ? (nsteps + energy_every - 1) / energy_every : 0;
+    std::vector<double> e(3 * (size_t)std::max<uint32_t>(nrec, 1));
+    pbamd::check(pb_lj_run(sys, &o, e.data()), "runFused");
+    e.resize(3 * (size_t)nrec);
+    return e;
+  }
+};

@@ -113,6 +113,9 @@ struct pb_sys {
   int last_skin_violation = 0;
   double last_step_ms = 0.0, last_rebuild_ms = 0.0;
   uint32_t last_nstep_timed = 0, last_nrebuild_timed = 0;
+  // host-mapped ring through which step kernels publish the previous step's
+  // max displacement / speed (drives the rebuild schedule without a sync)
+  uint32_t *hmon = nullptr, *dmon = nullptr;
 
   // g(r)
   uint64_t *gr_hist = nullptr;

@@ -15,8 +15,11 @@
 // and every host synchronisation of the reference loop are gone.
 #include "pb_internal.h"
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
+#include <thread>
 
 int pb_ensure_sort_ws(pb_sys *s, uint32_t ncell);
 int pb_counting_sort(pb_sys *s, uint32_t ncell, const uint32_t *key);
@@ -24,7 +27,8 @@ __global__ void k_fill_u32(uint32_t *x, uint32_t n, uint32_t val);
 __global__ void k_wrap(double4 *r, const uint32_t *orig, const uint32_t *waveRep,
                        const RepInfo *reps, uint32_t n);
 
-enum { CNT_LAST = 0, CNT_SKIN = 1, CNT_OVF = 2, CNT_MAXNN = 3, CNT_REC = 4 };
+enum { CNT_LAST = 0, CNT_SKIN = 1, CNT_OVF = 2, CNT_MAXNN = 3, CNT_REC = 4, CNT_DISP = 5,
+       CNT_VMAX = 6 };
 
 // ---------------------------------------------------------------- params
 static void fill_table(pb_sys *s) {
@@ -218,6 +222,8 @@ struct StepArgs {
   double *partials;
   uint32_t *counter;
   double *energy;       // records of 8*nrep doubles
+  uint32_t *mon;        // host-mapped ring (8 slots x 4 words) or null
+  uint32_t step;        // index of this step inside the run
   uint32_t npad, nrep;
   double dt, rc2, half_skin2;
   float rc2f;
@@ -233,8 +239,24 @@ __device__ __forceinline__ double rcp64(double x) {
   return y;
 }
 
+// Publish the previous step's displacement / speed maxima to the host ring.
+// Runs in block 0 when this kernel starts, i.e. after the previous step's
+// kernel has finished (stream order): the host reads it without a sync.
+__device__ __forceinline__ void publish_prev(const StepArgs &A) {
+  if (!A.mon || A.step == 0 || blockIdx.x != 0 || threadIdx.x != 0) return;
+  const uint32_t slot = ((A.step - 1) & 7u) * 4u;
+  const uint32_t d = __hip_atomic_load(&A.counter[CNT_DISP], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t v = __hip_atomic_load(&A.counter[CNT_VMAX], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t k = __hip_atomic_load(&A.counter[CNT_SKIN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&A.mon[slot + 0], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&A.mon[slot + 1], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&A.mon[slot + 2], k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&A.mon[slot + 3], A.step, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 template <bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
 __global__ void __launch_bounds__(256) k_lj_step(StepArgs A) {
+  publish_prev(A);
   const uint32_t i = pb_xcd_block() * blockDim.x + threadIdx.x;
   const bool inr = i < A.npad;
   // a wavefront never spans two replicas: make the replica index scalar
@@ -247,8 +269,9 @@ __global__ void __launch_bounds__(256) k_lj_step(StepArgs A) {
   const double L0 = R.L[0], L1 = R.L[1], L2 = R.L[2];
   const double I0 = R.Linv[0], I1 = R.Linv[1], I2 = R.Linv[2];
 
+  // forces and energies are accumulated in FP64 in both modes; MIXED
+  // evaluates each pair in FP32 on the FP64 displacement
   double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
-  float ffx = 0.f, ffy = 0.f, ffz = 0.f, fpe = 0.f, fvir = 0.f;
   const uint32_t *nlp = A.nl + i;
   for (uint32_t k = 0; k < nni; ++k) {
     const uint32_t j = nlp[(size_t)k * A.npad];
@@ -265,18 +288,19 @@ __global__ void __launch_bounds__(256) k_lj_step(StepArgs A) {
       if (r2 < A.rc2f) {
         const float s2 = ONE_TYPE ? A.s2f : A.lj->s2f[t];
         const float e24 = ONE_TYPE ? A.e24f : A.lj->e24f[t];
-        const float ir2 = __builtin_amdgcn_rcpf(r2);
+        float ir2 = __builtin_amdgcn_rcpf(r2);
+        ir2 = ir2 * (2.0f - r2 * ir2);                 // one Newton step: ~0.5 ulp
         float s6 = s2 * ir2;
         s6 = s6 * s6 * s6;
         const float fr = e24 * s6 * (2.0f * s6 - 1.0f) * ir2;   // kernelinline.hh:90-95
-        ffx += fr * ex;
-        ffy += fr * ey;
-        ffz += fr * ez;
+        fx += (double)(fr * ex);
+        fy += (double)(fr * ey);
+        fz += (double)(fr * ez);
         if (ENERGY) {
           const float e4 = ONE_TYPE ? A.e4f : A.lj->e4f[t];
           const float sh = ONE_TYPE ? A.shiftf : A.lj->shiftf[t];
-          fpe += e4 * s6 * (s6 - 1.0f) - sh;
-          fvir += fr * r2;
+          pe += (double)(e4 * s6 * (s6 - 1.0f) - sh);
+          vir += (double)(fr * r2);
         }
       }
     } else {
@@ -300,16 +324,10 @@ __global__ void __launch_bounds__(256) k_lj_step(StepArgs A) {
       }
     }
   }
-  if (MIXED) {
-    fx = ffx;
-    fy = ffy;
-    fz = ffz;
-    pe = fpe;
-    vir = fvir;
-  }
 
   double ke = 0.0;
   if (INTEGRATE) {
+    float d2 = 0.f, v2 = 0.f;
     if (inr) {
       double4 vi = A.vin[i];
       if (real) {
@@ -327,11 +345,37 @@ __global__ void __launch_bounds__(256) k_lj_step(StepArgs A) {
         const float4 rb = A.rbuild[i];
         const float ddx = (float)(pi.x - (double)rb.x), ddy = (float)(pi.y - (double)rb.y),
                     ddz = (float)(pi.z - (double)rb.z);
-        if ((double)(ddx * ddx + ddy * ddy + ddz * ddz) > A.half_skin2)
-          atomicOr(&A.counter[CNT_SKIN], 1u);
+        d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        v2 = (float)(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z);
+        if ((double)d2 > A.half_skin2) atomicOr(&A.counter[CNT_SKIN], 1u);
       }
       A.rout[i] = pi;
       A.vout[i] = vi;
+    }
+    // block maxima of displacement^2 and speed^2; an atomic only when the
+    // block beats the running maximum (non-negative floats order as uints)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      d2 = fmaxf(d2, __shfl_xor(d2, o, 64));
+      v2 = fmaxf(v2, __shfl_xor(v2, o, 64));
+    }
+    __shared__ float smx[2][8];
+    if (pb_lane() == 0) {
+      smx[0][threadIdx.x / 64] = d2;
+      smx[1][threadIdx.x / 64] = v2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float bd = 0.f, bv = 0.f;
+      for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
+        bd = fmaxf(bd, smx[0][w]);
+        bv = fmaxf(bv, smx[1][w]);
+      }
+      const uint32_t ud = __float_as_uint(bd), uv = __float_as_uint(bv);
+      if (ud > __hip_atomic_load(&A.counter[CNT_DISP], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(&A.counter[CNT_DISP], ud);
+      if (uv > __hip_atomic_load(&A.counter[CNT_VMAX], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(&A.counter[CNT_VMAX], uv);
     }
   }
   if (WRITE_F && inr) A.Fout[i] = make_double4(fx, fy, fz, 0.0);
@@ -640,6 +684,16 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
       if (e) (void)hipEventRecord(e, st);
     }
   };
+  // host-mapped ring for the lagged displacement reports
+  if (!s->hmon) {
+    if (hipHostMalloc((void **)&s->hmon, 32 * sizeof(uint32_t),
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&s->dmon, s->hmon, 0) != hipSuccess) {
+      rc = pb_fail(s->ctx, PB_ERR_HIP, "pb_lj_run: host-mapped monitor allocation failed");
+      goto done;
+    }
+  }
+  for (int k = 0; k < 32; ++k) ((volatile uint32_t *)s->hmon)[k] = 0u;
   // reference VV state -> kick-drift state
   hipLaunchKernelGGL(k_to_half, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->v, s->a,
                      s->npad, o->dt);
@@ -649,14 +703,47 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
   {
     StepArgs A = make_args(s, o->dt);
     A.energy = ebuf;
+    A.mon = s->dmon;
+    const double half_skin = 0.5 * o->skin;
+    uint32_t last_build = 0;
     for (uint32_t step = 0; step < o->nsteps; ++step) {
-      if (step > 0 && step % o->rebuild_every == 0) {
+      // Rebuild schedule: at most every rebuild_every steps, and earlier when
+      // the particles may reach skin/2.  Kernel `step-1` published the
+      // maxima after step `step-2` when it started, so reading them keeps one
+      // kernel in flight; the one step of motion in between is bounded by
+      // 1.5 * vmax * dt.
+      bool reb = false;
+      if (step > 0) {
+        if (step - last_build >= o->rebuild_every) {
+          reb = true;
+        } else if (step - last_build >= 2) {
+          const uint32_t slot = ((step - 2) & 7u) * 4u;
+          volatile uint32_t *mon = (volatile uint32_t *)s->hmon;
+          const auto t0 = std::chrono::steady_clock::now();
+          while (mon[slot + 3] != step - 1) {
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(30)) {
+              rc = pb_fail(s->ctx, PB_ERR_HIP, "pb_lj_run: step report timed out");
+              goto done;
+            }
+            std::this_thread::yield();
+          }
+          std::atomic_thread_fence(std::memory_order_acquire);
+          const uint32_t ud = mon[slot + 0], uv = mon[slot + 1];
+          float d2, v2;
+          std::memcpy(&d2, &ud, 4);
+          std::memcpy(&v2, &uv, 4);
+          if (std::sqrt((double)d2) + 1.5 * std::sqrt((double)v2) * o->dt >= half_skin) reb = true;
+        }
+      }
+      if (reb) {
         // rebuilds after the first use the capacity found by the first one;
         // an overflow is reported at the end of the run
         rec(ev_reb);
         if ((rc = rebuild(s))) goto done;
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s->d_counter + CNT_DISP, 2u, 0u);
         rec(ev_reb);
         ++rebuilds;
+        last_build = step;
         A.nl = s->nl;
         A.nn = s->nn;
       }
@@ -667,6 +754,7 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
       A.rout = s->r2buf;
       A.vout = s->v2buf;
       A.Fout = s->F;
+      A.step = step;
       rec(ev_step);
       if ((rc = launch_step(s, A, mixed, energy, last, true))) goto done;
       rec(ev_step);
