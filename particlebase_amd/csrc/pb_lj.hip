@@ -331,6 +331,14 @@ __global__ void __launch_bounds__(256) k_lj_step(StepArgs A) {
     if (inr) {
       double4 vi = A.vin[i];
       if (real) {
+        // the list is valid for this step iff the positions the pair loop
+        // used are within skin/2 of their positions at the last rebuild
+        const float4 rb = A.rbuild[i];
+        {
+          const float ex = (float)(pi.x - (double)rb.x), ey = (float)(pi.y - (double)rb.y),
+                      ez = (float)(pi.z - (double)rb.z);
+          if ((double)(ex * ex + ey * ey + ez * ez) > A.half_skin2) atomicOr(&A.counter[CNT_SKIN], 1u);
+        }
         // kick-drift form of propagateVelocityVerlet_F4 (kernelfuncs.cu:
         // 347-365): v holds v(t - dt/2) = v_ref(t-dt) + Fold dt/2m
         const double dtm2 = 0.5 * A.dt * vi.w;
@@ -342,12 +350,11 @@ __global__ void __launch_bounds__(256) k_lj_step(StepArgs A) {
         pi.x += vi.x * A.dt;
         pi.y += vi.y * A.dt;
         pi.z += vi.z * A.dt;
-        const float4 rb = A.rbuild[i];
+        // displacement of the new positions feeds the rebuild schedule
         const float ddx = (float)(pi.x - (double)rb.x), ddy = (float)(pi.y - (double)rb.y),
                     ddz = (float)(pi.z - (double)rb.z);
         d2 = ddx * ddx + ddy * ddy + ddz * ddz;
         v2 = (float)(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z);
-        if ((double)d2 > A.half_skin2) atomicOr(&A.counter[CNT_SKIN], 1u);
       }
       A.rout[i] = pi;
       A.vout[i] = vi;
