@@ -167,6 +167,7 @@ extern "C" void pb_sys_destroy(pb_sys *s) {
   pb_free(s->nl); pb_free(s->nn); pb_free(s->rbuild); pb_free(s->r2buf); pb_free(s->v2buf);
   pb_free(s->orig2); pb_free(s->type2); pb_free(s->move2); pb_free(s->a2buf);
   pb_free(s->gr_hist);
+  pb_free(s->list_segs);
   if (s->hmon) (void)hipHostFree(s->hmon);
   delete s;
 }

@@ -109,6 +109,9 @@ struct pb_sys {
   double4 *a2buf = nullptr;
   uint32_t list_ncell = 0;
   double list_rlist = 0.0;
+  uint32_t *list_segs = nullptr;   // list-build work units (5 words each)
+  uint32_t list_nsegs = 0;
+  float list_rb2 = 0.f;            // FP32 build radius^2 (rlist + rounding margin)
   uint32_t last_rebuilds = 0, last_maxnn = 0;
   int last_skin_violation = 0;
   double last_step_ms = 0.0, last_rebuild_ms = 0.0;
@@ -200,7 +203,10 @@ __device__ __forceinline__ double pb_load_relaxed(const double *p) {
 __device__ __forceinline__ void pb_block_reduce_reps(const double *partials, int K, const RepInfo *reps,
                                      uint32_t nrep, double *out, bool relaxed) {
   // Deterministic: every replica's wave partials are summed in a fixed order
-  // that depends only on the launch shape of this block.
+  // that depends only on the launch shape of this block.  Callers that read
+  // partials written by other workgroups issue a device-scope fence first
+  // (it invalidates this CU's vector L1), so plain loads are coherent.
+  (void)relaxed;
   __shared__ double sm[4][256];
   const uint32_t t = threadIdx.x, bs = blockDim.x;
   if (nrep >= 16) {
@@ -209,7 +215,7 @@ __device__ __forceinline__ void pb_block_reduce_reps(const double *partials, int
       double acc[4] = {0, 0, 0, 0};
       for (uint32_t w = R.wave0; w < R.wave0 + R.nwave; ++w)
         for (int c = 0; c < K; ++c)
-          acc[c] += relaxed ? pb_load_relaxed(partials + 4 * w + c) : partials[4 * w + c];
+          acc[c] += partials[4 * w + c];
       for (int c = 0; c < K; ++c) out[8 * q + c] = acc[c];
     }
     return;
@@ -219,7 +225,7 @@ __device__ __forceinline__ void pb_block_reduce_reps(const double *partials, int
     double acc[4] = {0, 0, 0, 0};
     for (uint32_t w = t; w < nw; w += bs)
       for (int c = 0; c < K; ++c)
-        acc[c] += relaxed ? pb_load_relaxed(partials + 4 * (w0 + w) + c) : partials[4 * (w0 + w) + c];
+        acc[c] += partials[4 * (w0 + w) + c];
     for (int c = 0; c < K; ++c) sm[c][t] = acc[c];
     __syncthreads();
     for (uint32_t o = bs / 2; o > 0; o >>= 1) {

@@ -126,6 +126,7 @@ struct ReorderArgs {
   uint16_t *type2;
   uint8_t *move2;
   float4 *rbuild;
+  uint32_t *nn;
   const uint32_t *waveRep;
   const RepInfo *reps;
   uint32_t n;
@@ -152,62 +153,153 @@ __global__ void k_reorder(ReorderArgs A) {
   A.type2[k] = A.type[s];
   A.move2[k] = A.move[s];
   A.rbuild[k] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.f);
+  if (o == PB_NONE) A.nn[k] = 0;   // padding never gets a list
 }
 
-// One thread per particle (cell order): scan the (deduplicated) 27
-// neighbouring list cells, keep j with |r_ij| < rc + skin (FP64, minimum
-// image).  The list is stored neighbour-major, nl[k*npad + i], so the pair
-// kernel's loads of the k-th neighbour of 64 consecutive particles are one
-// coalesced 256-byte read.
-__global__ void __launch_bounds__(256) k_build_list(const double4 *r, const uint32_t *cell,
-                                                    const uint32_t *cstart,
-                                                    const uint32_t *waveRep, const RepInfo *reps,
-                                                    uint32_t n, double rlist2, uint32_t maxnb,
-                                                    uint32_t *nl, uint32_t *nn,
-                                                    uint32_t *counter) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double4 pi = r[i];
-  if (pi.w == (double)PB_PAD_TYPE) {
-    nn[i] = 0;
-    return;
+// A list-build work unit: cells (ci, cj, ck0 .. ck0+nz-1) of one replica,
+// i.e. one z-run of a cell column, whose particles are contiguous in the
+// cell-ordered storage.
+struct ListSeg {
+  uint32_t rep, ci, cj, ck0, nz;
+};
+#define LB_CZ 6         // cells per segment along z
+#define LB_BS 128       // threads per workgroup
+#define LB_CAP 2048     // candidate particles staged in LDS per workgroup
+
+// scan cell J (global memory) for neighbours of particle i: fallback path
+__device__ __forceinline__ void scan_cell_global(const double4 *r, uint32_t i, const double4 &pi,
+                                                 uint32_t jb, uint32_t je, const RepInfo &R,
+                                                 double rlist2, uint32_t maxnb, uint32_t n,
+                                                 uint32_t *nl, uint32_t &cnt) {
+  for (uint32_t j = jb; j < je; ++j) {
+    if (j == i) continue;
+    const double4 pj = r[j];
+    double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    dx -= R.L[0] * rint(dx * R.Linv[0]);
+    dy -= R.L[1] * rint(dy * R.Linv[1]);
+    dz -= R.L[2] * rint(dz * R.Linv[2]);
+    if (dx * dx + dy * dy + dz * dz < rlist2) {
+      if (cnt < maxnb) nl[(size_t)cnt * n + i] = j;
+      ++cnt;
+    }
   }
-  const RepInfo &R = reps[waveRep[i / PB_WAVE]];
-  const uint32_t c = cell[i] - R.cell0;
+}
+
+// Verlet-list build on LDS-staged cell tiles.  A workgroup takes one
+// segment, stages the particles of the (deduplicated) 3x3 neighbouring
+// columns over the segment's z-range +-1 cell in LDS (positions in FP32 plus
+// their storage indices), then every particle of the segment scans its 27
+// neighbour cells from LDS and keeps j with |r_ij| < rb (FP32 minimum image;
+// rb = rc + skin + a margin covering FP32 rounding, so the list is a superset
+// of the FP64 pairs within rc + skin).  The list is stored neighbour-major,
+// nl[k*npad + i]: the pair kernel's loads of the k-th neighbour of 64
+// consecutive particles are one coalesced 256-byte read.  A segment whose
+// candidates exceed LB_CAP scans global memory instead.
+__global__ void __launch_bounds__(LB_BS) k_build_list(const double4 *r, const uint32_t *cell,
+                                                      const uint32_t *cstart,
+                                                      const ListSeg *segs, const RepInfo *reps,
+                                                      uint32_t n, float rb2, double rlist2,
+                                                      uint32_t maxnb, uint32_t *nl, uint32_t *nn,
+                                                      uint32_t *counter) {
+  __shared__ float4 sp[LB_CAP];
+  __shared__ uint32_t sg[LB_CAP];
+  __shared__ uint32_t toff[9][LB_CZ + 2], tcnt[9][LB_CZ + 2];
+  __shared__ uint32_t s_total;
+  const ListSeg G = segs[blockIdx.x];
+  const RepInfo &R = reps[G.rep];
   const int nb0 = (int)R.nb[0], nb1 = (int)R.nb[1], nb2 = (int)R.nb[2];
-  const int ck = (int)(c % R.nb[2]), cj = (int)((c / R.nb[2]) % R.nb[1]),
-            ci = (int)(c / (R.nb[2] * R.nb[1]));
   const int lx = nb0 >= 3 ? -1 : 0, hx = nb0 >= 2 ? 1 : 0;
   const int ly = nb1 >= 3 ? -1 : 0, hy = nb1 >= 2 ? 1 : 0;
   const int lz = nb2 >= 3 ? -1 : 0, hz = nb2 >= 2 ? 1 : 0;
-  uint32_t cnt = 0;
-  for (int a = lx; a <= hx; ++a) {
-    const int ii = (ci + a + nb0) % nb0;
-    for (int b = ly; b <= hy; ++b) {
-      const int jj = (cj + b + nb1) % nb1;
-      for (int d = lz; d <= hz; ++d) {
-        const int kk = (ck + d + nb2) % nb2;
-        const uint32_t J = R.cell0 + (uint32_t)(kk + nb2 * (jj + nb1 * ii));
-        const uint32_t je = cstart[J + 1];
-        for (uint32_t j = cstart[J]; j < je; ++j) {
-          if (j == i) continue;
-          const double4 pj = r[j];
-          double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-          dx -= R.L[0] * rint(dx * R.Linv[0]);
-          dy -= R.L[1] * rint(dy * R.Linv[1]);
-          dz -= R.L[2] * rint(dz * R.Linv[2]);
-          const double r2 = dx * dx + dy * dy + dz * dz;
-          if (r2 < rlist2) {
-            if (cnt < maxnb) nl[(size_t)cnt * n + i] = j;
-            ++cnt;
-          }
+  const int ncx = hx - lx + 1, ncy = hy - ly + 1;
+  // z window of the staged cells: the whole column when it is short
+  const bool whole = nb2 <= (int)G.nz + 2;
+  const int zslots = whole ? nb2 : (int)G.nz + 2;
+  const int zfirst = whole ? 0 : ((int)G.ck0 - 1 + nb2) % nb2;
+  auto colcell = [&](int col, int t) -> uint32_t {
+    const int ii = ((int)G.ci + lx + col / ncy + nb0) % nb0;
+    const int jj = ((int)G.cj + ly + col % ncy + nb1) % nb1;
+    const int kk = (zfirst + t) % nb2;
+    return R.cell0 + (uint32_t)(kk + nb2 * (jj + nb1 * ii));
+  };
+  if (threadIdx.x == 0) {
+    uint32_t off = 0;
+    for (int col = 0; col < ncx * ncy; ++col)
+      for (int t = 0; t < zslots; ++t) {
+        const uint32_t J = colcell(col, t);
+        const uint32_t c = cstart[J + 1] - cstart[J];
+        toff[col][t] = off;
+        tcnt[col][t] = c;
+        off += c;
+      }
+    s_total = off;
+  }
+  __syncthreads();
+  const bool staged = s_total <= LB_CAP;
+  if (staged) {
+    // copy every staged cell (consecutive z cells of a column are
+    // consecutive in storage; the copy loops are short and uniform)
+    for (int col = 0; col < ncx * ncy; ++col)
+      for (int t = 0; t < zslots; ++t) {
+        const uint32_t J = colcell(col, t);
+        const uint32_t g0 = cstart[J], c = tcnt[col][t], o = toff[col][t];
+        for (uint32_t q = threadIdx.x; q < c; q += LB_BS) {
+          const double4 p = r[g0 + q];
+          sp[o + q] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.f);
+          sg[o + q] = g0 + q;
         }
       }
-    }
   }
-  nn[i] = cnt < maxnb ? cnt : maxnb;
-  if (cnt > maxnb) atomicMax(&counter[CNT_OVF], cnt);
-  atomicMax(&counter[CNT_MAXNN], cnt);
+  __syncthreads();
+  const uint32_t Jfirst = R.cell0 + G.ck0 + (uint32_t)nb2 * (G.cj + (uint32_t)nb1 * G.ci);
+  const uint32_t i0 = cstart[Jfirst], i1 = cstart[Jfirst + G.nz];
+  const float Lx = (float)R.L[0], Ly = (float)R.L[1], Lz = (float)R.L[2];
+  const float Ix = (float)R.Linv[0], Iy = (float)R.Linv[1], Iz = (float)R.Linv[2];
+  uint32_t mymax = 0;
+  for (uint32_t i = i0 + threadIdx.x; i < i1; i += LB_BS) {
+    const double4 pi = r[i];
+    const int kz = (int)((cell[i] - R.cell0) % R.nb[2]);
+    uint32_t cnt = 0;
+    if (staged) {
+      const float xi = (float)pi.x, yi = (float)pi.y, zi = (float)pi.z;
+      for (int col = 0; col < ncx * ncy; ++col)
+        for (int d = lz; d <= hz; ++d) {
+          const int kk = (kz + d + nb2) % nb2;
+          const int t = whole ? kk : (kk - zfirst + nb2) % nb2;
+          const uint32_t o = toff[col][t], c = tcnt[col][t];
+          for (uint32_t q = 0; q < c; ++q) {
+            const float4 pj = sp[o + q];
+            float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
+            dx -= Lx * rintf(dx * Ix);
+            dy -= Ly * rintf(dy * Iy);
+            dz -= Lz * rintf(dz * Iz);
+            if (dx * dx + dy * dy + dz * dz < rb2) {
+              const uint32_t j = sg[o + q];
+              if (j != i) {
+                if (cnt < maxnb) nl[(size_t)cnt * n + i] = j;
+                ++cnt;
+              }
+            }
+          }
+        }
+    } else {
+      for (int col = 0; col < ncx * ncy; ++col)
+        for (int d = lz; d <= hz; ++d) {
+          const int ii = ((int)G.ci + lx + col / ncy + nb0) % nb0;
+          const int jj = ((int)G.cj + ly + col % ncy + nb1) % nb1;
+          const int kk = (kz + d + nb2) % nb2;
+          const uint32_t J = R.cell0 + (uint32_t)(kk + nb2 * (jj + nb1 * ii));
+          scan_cell_global(r, i, pi, cstart[J], cstart[J + 1], R, rlist2, maxnb, n, nl, cnt);
+        }
+    }
+    nn[i] = cnt < maxnb ? cnt : maxnb;
+    if (cnt > maxnb) atomicMax(&counter[CNT_OVF], cnt);
+    mymax = max(mymax, cnt);
+  }
+  // one atomic per wave for the list-length statistic
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mymax = max(mymax, (uint32_t)__shfl_xor((int)mymax, o, 64));
+  if (pb_lane() == 0 && mymax) atomicMax(&counter[CNT_MAXNN], mymax);
 }
 
 // ------------------------------------------------------------- step kernel
@@ -270,22 +362,20 @@ __global__ void __launch_bounds__(256) k_lj_step(StepArgs A) {
   const double I0 = R.Linv[0], I1 = R.Linv[1], I2 = R.Linv[2];
 
   // forces and energies are accumulated in FP64 in both modes; MIXED
-  // evaluates each pair in FP32 on the FP64 displacement
+  // evaluates each pair in FP32 on the FP64 displacement, after the FP64
+  // cutoff test (so both modes keep exactly the same pairs)
   double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
-  const uint32_t *nlp = A.nl + i;
-  for (uint32_t k = 0; k < nni; ++k) {
-    const uint32_t j = nlp[(size_t)k * A.npad];
-    const double4 pj = A.rin[j];
+  auto pair = [&](const double4 &pj) {
     double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
     dx -= L0 * rint(dx * I0);
     dy -= L1 * rint(dy * I1);
     dz -= L2 * rint(dz * I2);
     uint32_t t = 0;
     if (!ONE_TYPE) t = ti * PB_MAXT + (uint32_t)pj.w;
-    if (MIXED) {
-      const float ex = (float)dx, ey = (float)dy, ez = (float)dz;
-      const float r2 = ex * ex + ey * ey + ez * ez;
-      if (r2 < A.rc2f) {
+    const double r2d = dx * dx + dy * dy + dz * dz;
+    if (r2d < A.rc2) {
+      if (MIXED) {
+        const float ex = (float)dx, ey = (float)dy, ez = (float)dz, r2 = (float)r2d;
         const float s2 = ONE_TYPE ? A.s2f : A.lj->s2f[t];
         const float e24 = ONE_TYPE ? A.e24f : A.lj->e24f[t];
         float ir2 = __builtin_amdgcn_rcpf(r2);
@@ -302,13 +392,10 @@ __global__ void __launch_bounds__(256) k_lj_step(StepArgs A) {
           pe += (double)(e4 * s6 * (s6 - 1.0f) - sh);
           vir += (double)(fr * r2);
         }
-      }
-    } else {
-      const double r2 = dx * dx + dy * dy + dz * dz;
-      if (r2 < A.rc2) {
+      } else {
         const double s2 = ONE_TYPE ? A.s2 : A.lj->s2[t];
         const double e24 = ONE_TYPE ? A.e24 : A.lj->e24[t];
-        const double ir2 = rcp64(r2);
+        const double ir2 = rcp64(r2d);
         double s6 = s2 * ir2;
         s6 = s6 * s6 * s6;
         const double fr = e24 * s6 * (2.0 * s6 - 1.0) * ir2;
@@ -319,11 +406,26 @@ __global__ void __launch_bounds__(256) k_lj_step(StepArgs A) {
           const double e4 = ONE_TYPE ? A.e4 : A.lj->e4[t];
           const double sh = ONE_TYPE ? A.shift : A.lj->shift[t];
           pe += e4 * s6 * (s6 - 1.0) - sh;
-          vir += fr * r2;
+          vir += fr * r2d;
         }
       }
     }
+  };
+  // four list entries and their four gathers are issued before any pair is
+  // evaluated, so each wave keeps several memory requests in flight
+  const uint32_t *nlp = A.nl + i;
+  const size_t ld = A.npad;
+  uint32_t k = 0;
+  for (; k + 4 <= nni; k += 4) {
+    const uint32_t j0 = nlp[(size_t)k * ld], j1 = nlp[(size_t)(k + 1) * ld],
+                   j2 = nlp[(size_t)(k + 2) * ld], j3 = nlp[(size_t)(k + 3) * ld];
+    const double4 p0 = A.rin[j0], p1 = A.rin[j1], p2 = A.rin[j2], p3 = A.rin[j3];
+    pair(p0);
+    pair(p1);
+    pair(p2);
+    pair(p3);
   }
+  for (; k < nni; ++k) pair(A.rin[nlp[(size_t)k * ld]]);
 
   double ke = 0.0;
   if (INTEGRATE) {
@@ -450,8 +552,27 @@ static int setup_list_cells(pb_sys *s, double rlist) {
   }
   s->list_ncell = cell0;
   s->list_rlist = rlist;
+  // list-build segments: z-runs of LB_CZ cells of every column
+  std::vector<ListSeg> segs;
+  double lmax = 0.0;
+  for (uint32_t q = 0; q < s->nrep; ++q) {
+    const RepInfo &R = s->reps[q];
+    for (int a = 0; a < 3; ++a) lmax = std::max(lmax, std::fabs(R.lo[a]) + R.L[a]);
+    for (uint32_t ci = 0; ci < R.nb[0]; ++ci)
+      for (uint32_t cj = 0; cj < R.nb[1]; ++cj)
+        for (uint32_t ck = 0; ck < R.nb[2]; ck += LB_CZ)
+          segs.push_back(ListSeg{q, ci, cj, ck, std::min<uint32_t>(LB_CZ, R.nb[2] - ck)});
+  }
+  s->list_nsegs = (uint32_t)segs.size();
+  if (int rc = pb_alloc(s->ctx, &s->list_segs, 5 * segs.size())) return rc;
+  PB_HIP(s->ctx, hipMemcpyAsync(s->list_segs, segs.data(), sizeof(ListSeg) * segs.size(),
+                                hipMemcpyHostToDevice, s->ctx->stream));
+  // FP32 build radius: covers the rounding of FP32 coordinates up to |x| = lmax
+  const double rb = rlist + 1e-4 + 8.0 * lmax * 0x1p-24;
+  s->list_rb2 = (float)(rb * rb);
   PB_HIP(s->ctx, hipMemcpyAsync(s->d_reps, s->reps.data(), sizeof(RepInfo) * s->nrep,
                                 hipMemcpyHostToDevice, s->ctx->stream));
+  PB_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));   // segs is a host temporary
   return pb_ensure_sort_ws(s, s->list_ncell);
 }
 
@@ -487,6 +608,7 @@ static int rebuild(pb_sys *s) {
   A.r2 = s->r2buf; A.v2 = s->v2buf; A.a2 = s->a2buf;
   A.orig2 = s->orig2; A.cell2 = s->rank; A.type2 = s->type2; A.move2 = s->move2;
   A.rbuild = s->rbuild;
+  A.nn = s->nn;
   A.waveRep = s->waveRep; A.reps = s->d_reps; A.n = n;
   hipLaunchKernelGGL(k_reorder, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A);
   PB_LAUNCH_CHECK(s->ctx, "k_reorder");
@@ -496,9 +618,9 @@ static int rebuild(pb_sys *s) {
   std::swap(s->orig, s->orig2);
   std::swap(s->type, s->type2);
   std::swap(s->move, s->move2);
-  hipLaunchKernelGGL(k_build_list, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->r, s->rank,
-                     s->cstart, s->waveRep, s->d_reps, n, s->list_rlist * s->list_rlist,
-                     s->maxnb, s->nl, s->nn, s->d_counter);
+  hipLaunchKernelGGL(k_build_list, dim3(s->list_nsegs), dim3(LB_BS), 0, st, s->r, s->rank,
+                     s->cstart, (const ListSeg *)s->list_segs, s->d_reps, n, s->list_rb2,
+                     s->list_rlist * s->list_rlist, s->maxnb, s->nl, s->nn, s->d_counter);
   PB_LAUNCH_CHECK(s->ctx, "k_build_list");
   s->ref_binned = false;
   return PB_OK;

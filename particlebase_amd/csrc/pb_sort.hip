@@ -156,11 +156,46 @@ __global__ void k_scatter(const uint32_t *cellid, const uint32_t *rank, const ui
   if (s < n) perm[start[cellid[s]] + rank[s]] = s;
 }
 
-// one thread per cell: insertion sort of the cell's slots by key
-__global__ void k_cell_sort(const uint32_t *start, uint32_t ncell, const uint32_t *key,
-                            uint32_t *perm) {
-  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
+// 256 consecutive cells per workgroup own one contiguous range of perm.
+// The range and its keys are staged in LDS, every thread insertion-sorts
+// its cell there, and the range is written back.  A range larger than the
+// LDS stage (a pathological pile-up) is sorted in global memory instead.
+#define CS_CELLS 256
+#define CS_CAP 6144
+__global__ void __launch_bounds__(CS_CELLS) k_cell_sort(const uint32_t *start, uint32_t ncell,
+                                                        const uint32_t *key, uint32_t *perm) {
+  __shared__ uint32_t sv[CS_CAP], sk[CS_CAP];
+  const uint32_t c0 = blockIdx.x * CS_CELLS;
+  const uint32_t c1 = min(c0 + CS_CELLS, ncell);
+  const uint32_t base = start[c0], end = start[c1];
+  const uint32_t len = end - base;
+  const uint32_t c = c0 + threadIdx.x;
+  if (len <= CS_CAP) {
+    for (uint32_t q = threadIdx.x; q < len; q += blockDim.x) {
+      const uint32_t p = perm[base + q];
+      sv[q] = p;
+      sk[q] = key[p];
+    }
+    __syncthreads();
+    if (c < c1) {
+      const uint32_t b = start[c] - base, e = start[c + 1] - base;
+      for (uint32_t i = b + 1; i < e; ++i) {
+        const uint32_t p = sv[i], kp = sk[i];
+        uint32_t j = i;
+        while (j > b && sk[j - 1] > kp) {
+          sv[j] = sv[j - 1];
+          sk[j] = sk[j - 1];
+          --j;
+        }
+        sv[j] = p;
+        sk[j] = kp;
+      }
+    }
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < len; q += blockDim.x) perm[base + q] = sv[q];
+    return;
+  }
+  if (c >= c1) return;
   const uint32_t b = start[c], e = start[c + 1];
   for (uint32_t i = b + 1; i < e; ++i) {
     const uint32_t p = perm[i];
@@ -204,8 +239,8 @@ int pb_counting_sort(pb_sys *s, uint32_t ncell, const uint32_t *key) {
   if (int rc = pb_exclusive_scan(s, s->ccount, s->cstart, ncell)) return rc;
   hipLaunchKernelGGL(k_scatter, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->cellid, s->rank,
                      s->cstart, n, s->perm);
-  hipLaunchKernelGGL(k_cell_sort, dim3(pb_blocks(ncell, 256)), dim3(256), 0, st, s->cstart, ncell,
-                     key, s->perm);
+  hipLaunchKernelGGL(k_cell_sort, dim3(pb_blocks(ncell, CS_CELLS)), dim3(CS_CELLS), 0, st, s->cstart,
+                     ncell, key, s->perm);
   PB_LAUNCH_CHECK(s->ctx, "pb_counting_sort");
   return PB_OK;
 }
