@@ -28,6 +28,7 @@ struct RepInfo {
   uint32_t natoms;     // real particles
   uint32_t npad;       // slots (multiple of 64)
   uint32_t wave0, nwave;
+  uint32_t seg0, nseg;   // pair-kernel work units (segments) of this replica
 };
 
 struct LJTable {
@@ -201,19 +202,20 @@ __device__ __forceinline__ double pb_load_relaxed(const double *p) {
 // sum of K per-wave partial components for every replica, executed by one
 // block (used by the last-block reductions and by k_reduce_waves).
 __device__ __forceinline__ void pb_block_reduce_reps(const double *partials, int K, const RepInfo *reps,
-                                     uint32_t nrep, double *out, bool relaxed) {
-  // Deterministic: every replica's wave partials are summed in a fixed order
-  // that depends only on the launch shape of this block.  Callers that read
+                                     uint32_t nrep, double *out, bool by_segment) {
+  // Deterministic: every replica's partials (one per wave, or one per
+  // pair-kernel segment when by_segment) are summed in a fixed order that
+  // depends only on the launch shape of this block.  Callers that read
   // partials written by other workgroups issue a device-scope fence first
   // (it invalidates this CU's vector L1), so plain loads are coherent.
-  (void)relaxed;
   __shared__ double sm[4][256];
   const uint32_t t = threadIdx.x, bs = blockDim.x;
   if (nrep >= 16) {
     for (uint32_t q = t; q < nrep; q += bs) {
       const RepInfo R = reps[q];
+      const uint32_t w0 = by_segment ? R.seg0 : R.wave0, nw = by_segment ? R.nseg : R.nwave;
       double acc[4] = {0, 0, 0, 0};
-      for (uint32_t w = R.wave0; w < R.wave0 + R.nwave; ++w)
+      for (uint32_t w = w0; w < w0 + nw; ++w)
         for (int c = 0; c < K; ++c)
           acc[c] += partials[4 * w + c];
       for (int c = 0; c < K; ++c) out[8 * q + c] = acc[c];
@@ -221,7 +223,8 @@ __device__ __forceinline__ void pb_block_reduce_reps(const double *partials, int
     return;
   }
   for (uint32_t q = 0; q < nrep; ++q) {
-    const uint32_t w0 = reps[q].wave0, nw = reps[q].nwave;
+    const uint32_t w0 = by_segment ? reps[q].seg0 : reps[q].wave0;
+    const uint32_t nw = by_segment ? reps[q].nseg : reps[q].nwave;
     double acc[4] = {0, 0, 0, 0};
     for (uint32_t w = t; w < nw; w += bs)
       for (int c = 0; c < K; ++c)

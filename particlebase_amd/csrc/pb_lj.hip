@@ -156,147 +156,230 @@ __global__ void k_reorder(ReorderArgs A) {
   if (o == PB_NONE) A.nn[k] = 0;   // padding never gets a list
 }
 
-// A list-build work unit: cells (ci, cj, ck0 .. ck0+nz-1) of one replica,
-// i.e. one z-run of a cell column, whose particles are contiguous in the
-// cell-ordered storage.
+// ---------------------------------------------------------------- tiles
+// Work unit of the list build and of the pair kernel ("segment"): a tile of
+// tx x ty cell columns x nz cells along z of one replica.  Its window -- the
+// tile grown by one list cell on every side -- is staged in LDS with the
+// periodic image shift of every window cell already applied, so a pair is
+// x_i - x_j with no minimum-image arithmetic.  The Verlet list stores
+// window-local indices.  A replica with fewer than 3 list cells along an
+// axis is one "whole" segment: every particle is staged once and pairs use
+// the minimum image.
 struct ListSeg {
-  uint32_t rep, ci, cj, ck0, nz;
+  uint32_t rep, ci, cj, ck0, nz, tx, ty, whole;
 };
-#define LB_CZ 6         // cells per segment along z
-#define LB_BS 128       // threads per workgroup
-#define LB_CAP 2048     // candidate particles staged in LDS per workgroup
+#define TB 384            // threads per workgroup (6 waves)
+#define WCAP 2816         // staged window particles per workgroup (LDS)
+#define MAXCOL 16         // (tx+2)*(ty+2) window columns, tx, ty <= 2
+#define MAXZ 16           // nz <= MAXZ
+#define MAXE (MAXCOL * (MAXZ + 2))
 
-// scan cell J (global memory) for neighbours of particle i: fallback path
-__device__ __forceinline__ void scan_cell_global(const double4 *r, uint32_t i, const double4 &pi,
-                                                 uint32_t jb, uint32_t je, const RepInfo &R,
-                                                 double rlist2, uint32_t maxnb, uint32_t n,
-                                                 uint32_t *nl, uint32_t &cnt) {
-  for (uint32_t j = jb; j < je; ++j) {
-    if (j == i) continue;
-    const double4 pj = r[j];
-    double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-    dx -= R.L[0] * rint(dx * R.Linv[0]);
-    dy -= R.L[1] * rint(dy * R.Linv[1]);
-    dz -= R.L[2] * rint(dz * R.Linv[2]);
-    if (dx * dx + dy * dy + dz * dz < rlist2) {
-      if (cnt < maxnb) nl[(size_t)cnt * n + i] = j;
-      ++cnt;
-    }
+struct Stage {
+  double x[WCAP], y[WCAP], z[WCAP];
+  uint8_t t[WCAP];
+  uint32_t eoff[MAXE + 1], eg0[MAXE], ec[MAXE], esh[MAXE];
+  uint32_t nent, total, ncol, zs;
+  uint32_t nrun, rloc[4], rpref[5], rg0[4], rcol[4];
+  uint32_t wsum[TB / 64];
+};
+
+// exclusive block scan (blockDim multiple of 64, <= TB)
+__device__ __forceinline__ uint32_t tile_scan(uint32_t x, uint32_t *wsum, uint32_t *total) {
+  const uint32_t lane = pb_lane(), w = threadIdx.x / 64, nw = blockDim.x / 64;
+  uint32_t inc = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(inc, o, 64);
+    if (lane >= (uint32_t)o) inc += y;
   }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (uint32_t k = 0; k < nw; ++k) {
+    if (k < w) off += wsum[k];
+    tot += wsum[k];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + inc - x;
 }
 
-// Verlet-list build on LDS-staged cell tiles.  A workgroup takes one
-// segment, stages the particles of the (deduplicated) 3x3 neighbouring
-// columns over the segment's z-range +-1 cell in LDS (positions in FP32 plus
-// their storage indices), then every particle of the segment scans its 27
-// neighbour cells from LDS and keeps j with |r_ij| < rb (FP32 minimum image;
-// rb = rc + skin + a margin covering FP32 rounding, so the list is a superset
-// of the FP64 pairs within rc + skin).  The list is stored neighbour-major,
-// nl[k*npad + i]: the pair kernel's loads of the k-th neighbour of 64
-// consecutive particles are one coalesced 256-byte read.  A segment whose
-// candidates exceed LB_CAP scans global memory instead.
-__global__ void __launch_bounds__(LB_BS) k_build_list(const double4 *r, const uint32_t *cell,
-                                                      const uint32_t *cstart,
-                                                      const ListSeg *segs, const RepInfo *reps,
-                                                      uint32_t n, float rb2, double rlist2,
-                                                      uint32_t maxnb, uint32_t *nl, uint32_t *nn,
-                                                      uint32_t *counter) {
-  __shared__ float4 sp[LB_CAP];
-  __shared__ uint32_t sg[LB_CAP];
-  __shared__ uint32_t toff[9][LB_CZ + 2], tcnt[9][LB_CZ + 2];
-  __shared__ uint32_t s_total;
-  const ListSeg G = segs[blockIdx.x];
-  const RepInfo &R = reps[G.rep];
+// window entry e -> (cell, image shift); shift packed as 3 x 2 bits (+1 / -1)
+__device__ __forceinline__ void entry_cell(const ListSeg &G, const RepInfo &R, uint32_t e,
+                                           uint32_t zs, uint32_t &J, uint32_t &sh) {
   const int nb0 = (int)R.nb[0], nb1 = (int)R.nb[1], nb2 = (int)R.nb[2];
-  const int lx = nb0 >= 3 ? -1 : 0, hx = nb0 >= 2 ? 1 : 0;
-  const int ly = nb1 >= 3 ? -1 : 0, hy = nb1 >= 2 ? 1 : 0;
-  const int lz = nb2 >= 3 ? -1 : 0, hz = nb2 >= 2 ? 1 : 0;
-  const int ncx = hx - lx + 1, ncy = hy - ly + 1;
-  // z window of the staged cells: the whole column when it is short
-  const bool whole = nb2 <= (int)G.nz + 2;
-  const int zslots = whole ? nb2 : (int)G.nz + 2;
-  const int zfirst = whole ? 0 : ((int)G.ck0 - 1 + nb2) % nb2;
-  auto colcell = [&](int col, int t) -> uint32_t {
-    const int ii = ((int)G.ci + lx + col / ncy + nb0) % nb0;
-    const int jj = ((int)G.cj + ly + col % ncy + nb1) % nb1;
-    const int kk = (zfirst + t) % nb2;
-    return R.cell0 + (uint32_t)(kk + nb2 * (jj + nb1 * ii));
-  };
-  if (threadIdx.x == 0) {
-    uint32_t off = 0;
-    for (int col = 0; col < ncx * ncy; ++col)
-      for (int t = 0; t < zslots; ++t) {
-        const uint32_t J = colcell(col, t);
-        const uint32_t c = cstart[J + 1] - cstart[J];
-        toff[col][t] = off;
-        tcnt[col][t] = c;
-        off += c;
-      }
-    s_total = off;
+  const uint32_t col = e / zs, t = e % zs;
+  const int wy = (int)G.ty + 2;
+  const int ii = (int)G.ci - 1 + (int)(col / wy), jj = (int)G.cj - 1 + (int)(col % wy);
+  const int kk = (int)G.ck0 - 1 + (int)t;
+  const int sx = ii < 0 ? -1 : (ii >= nb0 ? 1 : 0);
+  const int sy = jj < 0 ? -1 : (jj >= nb1 ? 1 : 0);
+  const int sz = kk < 0 ? -1 : (kk >= nb2 ? 1 : 0);
+  J = R.cell0 + (uint32_t)((kk - sz * nb2) + nb2 * ((jj - sy * nb1) + nb1 * (ii - sx * nb0)));
+  sh = (uint32_t)(sx + 1) | ((uint32_t)(sy + 1) << 2) | ((uint32_t)(sz + 1) << 4);
+}
+
+// Build the entry table of the window and (if it fits) stage its positions.
+// Returns through S; all threads of the block must call it.
+__device__ void stage_window(Stage &S, const ListSeg &G, const RepInfo &R, const uint32_t *cstart,
+                             const double4 *r) {
+  const uint32_t tid = threadIdx.x;
+  if (tid == 0) {
+    S.ncol = G.whole ? 1u : (G.tx + 2) * (G.ty + 2);
+    S.zs = G.whole ? 1u : G.nz + 2;
+    S.nent = S.ncol * S.zs;
   }
   __syncthreads();
-  const bool staged = s_total <= LB_CAP;
-  if (staged) {
-    // copy every staged cell (consecutive z cells of a column are
-    // consecutive in storage; the copy loops are short and uniform)
-    for (int col = 0; col < ncx * ncy; ++col)
-      for (int t = 0; t < zslots; ++t) {
-        const uint32_t J = colcell(col, t);
-        const uint32_t g0 = cstart[J], c = tcnt[col][t], o = toff[col][t];
-        for (uint32_t q = threadIdx.x; q < c; q += LB_BS) {
-          const double4 p = r[g0 + q];
-          sp[o + q] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.f);
-          sg[o + q] = g0 + q;
-        }
-      }
-  }
-  __syncthreads();
-  const uint32_t Jfirst = R.cell0 + G.ck0 + (uint32_t)nb2 * (G.cj + (uint32_t)nb1 * G.ci);
-  const uint32_t i0 = cstart[Jfirst], i1 = cstart[Jfirst + G.nz];
-  const float Lx = (float)R.L[0], Ly = (float)R.L[1], Lz = (float)R.L[2];
-  const float Ix = (float)R.Linv[0], Iy = (float)R.Linv[1], Iz = (float)R.Linv[2];
-  uint32_t mymax = 0;
-  for (uint32_t i = i0 + threadIdx.x; i < i1; i += LB_BS) {
-    const double4 pi = r[i];
-    const int kz = (int)((cell[i] - R.cell0) % R.nb[2]);
-    uint32_t cnt = 0;
-    if (staged) {
-      const float xi = (float)pi.x, yi = (float)pi.y, zi = (float)pi.z;
-      for (int col = 0; col < ncx * ncy; ++col)
-        for (int d = lz; d <= hz; ++d) {
-          const int kk = (kz + d + nb2) % nb2;
-          const int t = whole ? kk : (kk - zfirst + nb2) % nb2;
-          const uint32_t o = toff[col][t], c = tcnt[col][t];
-          for (uint32_t q = 0; q < c; ++q) {
-            const float4 pj = sp[o + q];
-            float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
-            dx -= Lx * rintf(dx * Ix);
-            dy -= Ly * rintf(dy * Iy);
-            dz -= Lz * rintf(dz * Iz);
-            if (dx * dx + dy * dy + dz * dz < rb2) {
-              const uint32_t j = sg[o + q];
-              if (j != i) {
-                if (cnt < maxnb) nl[(size_t)cnt * n + i] = j;
-                ++cnt;
-              }
-            }
-          }
-        }
+  uint32_t c = 0;
+  if (tid < S.nent) {
+    uint32_t g0, sh = 0x15;   // no shift
+    if (G.whole) {
+      g0 = R.atom0;
+      c = R.natoms;
     } else {
-      for (int col = 0; col < ncx * ncy; ++col)
-        for (int d = lz; d <= hz; ++d) {
-          const int ii = ((int)G.ci + lx + col / ncy + nb0) % nb0;
-          const int jj = ((int)G.cj + ly + col % ncy + nb1) % nb1;
-          const int kk = (kz + d + nb2) % nb2;
-          const uint32_t J = R.cell0 + (uint32_t)(kk + nb2 * (jj + nb1 * ii));
-          scan_cell_global(r, i, pi, cstart[J], cstart[J + 1], R, rlist2, maxnb, n, nl, cnt);
+      uint32_t J;
+      entry_cell(G, R, tid, S.zs, J, sh);
+      g0 = cstart[J];
+      c = cstart[J + 1] - g0;
+    }
+    S.eg0[tid] = g0;
+    S.ec[tid] = c;
+    S.esh[tid] = sh;
+  }
+  uint32_t tot;
+  const uint32_t off = tile_scan(c, S.wsum, &tot);
+  if (tid < S.nent) S.eoff[tid] = off;
+  if (tid == 0) {
+    S.eoff[S.nent] = tot;
+    S.total = tot;
+    // runs of the tile's own particles: inner columns, inner z slots
+    if (G.whole) {
+      S.nrun = 1;
+      S.rloc[0] = 0;
+      S.rg0[0] = R.atom0;
+      S.rcol[0] = 0;
+      S.rpref[0] = 0;
+      S.rpref[1] = tot;
+    } else {
+      uint32_t n = 0, pref = 0;
+      for (uint32_t a = 0; a < G.tx; ++a)
+        for (uint32_t b = 0; b < G.ty; ++b) {
+          const uint32_t col = (a + 1) * (G.ty + 2) + (b + 1);
+          const uint32_t e0 = col * S.zs + 1;
+          S.rloc[n] = S.eoff[e0];
+          S.rg0[n] = S.eg0[e0];
+          S.rcol[n] = col;
+          S.rpref[n] = pref;
+          pref += S.eoff[e0 + G.nz] - S.eoff[e0];
+          ++n;
+        }
+      S.nrun = n;
+      S.rpref[n] = pref;
+    }
+  }
+  __syncthreads();
+  if (S.total <= WCAP) {
+    const double Lx = R.L[0], Ly = R.L[1], Lz = R.L[2];
+    for (uint32_t q = tid; q < S.total; q += blockDim.x) {
+      uint32_t lo = 0, hi = S.nent;                 // largest e with eoff[e] <= q
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (S.eoff[mid] <= q) lo = mid;
+        else hi = mid;
+      }
+      const uint32_t sh = S.esh[lo];
+      const double4 p = r[S.eg0[lo] + (q - S.eoff[lo])];
+      S.x[q] = p.x + (double)((int)(sh & 3u) - 1) * Lx;
+      S.y[q] = p.y + (double)((int)((sh >> 2) & 3u) - 1) * Ly;
+      S.z[q] = p.z + (double)((int)((sh >> 4) & 3u) - 1) * Lz;
+      S.t[q] = (uint8_t)p.w;
+    }
+  }
+  __syncthreads();
+}
+
+// position of window-local particle q (from LDS, or from global memory when
+// the window did not fit)
+__device__ __forceinline__ double4 window_pos(const Stage &S, const RepInfo &R, const double4 *r,
+                                              bool staged, uint32_t q) {
+  if (staged) return make_double4(S.x[q], S.y[q], S.z[q], (double)S.t[q]);
+  uint32_t lo = 0, hi = S.nent;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (S.eoff[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t sh = S.esh[lo];
+  double4 p = r[S.eg0[lo] + (q - S.eoff[lo])];
+  p.x += (double)((int)(sh & 3u) - 1) * R.L[0];
+  p.y += (double)((int)((sh >> 2) & 3u) - 1) * R.L[1];
+  p.z += (double)((int)((sh >> 4) & 3u) - 1) * R.L[2];
+  return p;
+}
+
+// tile particle u (0 <= u < rpref[nrun]) -> window-local index, storage slot, run
+__device__ __forceinline__ void tile_particle(const Stage &S, uint32_t u, uint32_t &li,
+                                              uint32_t &gi, uint32_t &run) {
+  uint32_t k = 0;
+  while (k + 1 < S.nrun && S.rpref[k + 1] <= u) ++k;
+  run = k;
+  li = S.rloc[k] + (u - S.rpref[k]);
+  gi = S.rg0[k] + (u - S.rpref[k]);
+}
+
+// Verlet-list build: every tile particle scans its 27 neighbour cells of the
+// window (all staged particles in whole mode) and keeps window-local q with
+// |r_iq| < rc + skin (FP64).
+__global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint32_t *cstart,
+                                                   const ListSeg *segs, const RepInfo *reps,
+                                                   uint32_t npad, double rlist2, uint32_t maxnb,
+                                                   uint32_t *nl, uint32_t *nn, uint32_t *counter) {
+  __shared__ Stage S;
+  const ListSeg G = segs[pb_xcd_block()];
+  const RepInfo &R = reps[G.rep];
+  stage_window(S, G, R, cstart, r);
+  const bool staged = S.total <= WCAP;
+  const uint32_t ntile = S.rpref[S.nrun];
+  uint32_t mymax = 0;
+  for (uint32_t u = threadIdx.x; u < ntile; u += blockDim.x) {
+    uint32_t li, gi, run;
+    tile_particle(S, u, li, gi, run);
+    const double4 pi = window_pos(S, R, r, staged, li);
+    uint32_t cnt = 0;
+    auto consider = [&](uint32_t q, bool mi) {
+      if (q == li) return;
+      const double4 pj = window_pos(S, R, r, staged, q);
+      double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      if (mi) {
+        dx -= R.L[0] * rint(dx * R.Linv[0]);
+        dy -= R.L[1] * rint(dy * R.Linv[1]);
+        dz -= R.L[2] * rint(dz * R.Linv[2]);
+      }
+      if (dx * dx + dy * dy + dz * dz < rlist2) {
+        if (cnt < maxnb) nl[(size_t)cnt * npad + gi] = q;
+        ++cnt;
+      }
+    };
+    if (G.whole) {
+      for (uint32_t q = 0; q < S.total; ++q) consider(q, true);
+    } else {
+      // z slot of this particle: the inner entry of its column holding li
+      const uint32_t col = S.rcol[run];
+      uint32_t ti = 1;
+      while (ti < G.nz && S.eoff[col * S.zs + ti + 1] <= li) ++ti;
+      const uint32_t wy = G.ty + 2;
+      const uint32_t ca = col / wy, cb = col % wy;
+      for (uint32_t a = ca - 1; a <= ca + 1; ++a)
+        for (uint32_t b = cb - 1; b <= cb + 1; ++b) {
+          const uint32_t e0 = (a * wy + b) * S.zs + ti - 1;
+          const uint32_t q0 = S.eoff[e0], q1 = S.eoff[e0 + 3];   // three z slots
+          for (uint32_t q = q0; q < q1; ++q) consider(q, false);
         }
     }
-    nn[i] = cnt < maxnb ? cnt : maxnb;
+    nn[gi] = cnt < maxnb ? cnt : maxnb;
     if (cnt > maxnb) atomicMax(&counter[CNT_OVF], cnt);
     mymax = max(mymax, cnt);
   }
-  // one atomic per wave for the list-length statistic
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mymax = max(mymax, (uint32_t)__shfl_xor((int)mymax, o, 64));
   if (pb_lane() == 0 && mymax) atomicMax(&counter[CNT_MAXNN], mymax);
@@ -308,7 +391,8 @@ struct StepArgs {
   double4 *rout, *vout, *Fout;
   const uint32_t *nl, *nn;
   const float4 *rbuild;
-  const uint32_t *waveRep;
+  const uint32_t *cstart;
+  const ListSeg *segs;
   const RepInfo *reps;
   const LJTable *lj;
   double *partials;
@@ -318,7 +402,6 @@ struct StepArgs {
   uint32_t step;        // index of this step inside the run
   uint32_t npad, nrep;
   double dt, rc2, half_skin2;
-  float rc2f;
   // single-type parameters
   double s2, e24, e4, shift;
   float s2f, e24f, e4f, shiftf;
@@ -332,8 +415,9 @@ __device__ __forceinline__ double rcp64(double x) {
 }
 
 // Publish the previous step's displacement / speed maxima to the host ring.
-// Runs in block 0 when this kernel starts, i.e. after the previous step's
-// kernel has finished (stream order): the host reads it without a sync.
+// Runs in workgroup 0 when this kernel starts, i.e. after the previous
+// step's kernel has finished (stream order): the host reads it without a
+// synchronisation.
 __device__ __forceinline__ void publish_prev(const StepArgs &A) {
   if (!A.mon || A.step == 0 || blockIdx.x != 0 || threadIdx.x != 0) return;
   const uint32_t slot = ((A.step - 1) & 7u) * 4u;
@@ -347,136 +431,157 @@ __device__ __forceinline__ void publish_prev(const StepArgs &A) {
 }
 
 template <bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
-__global__ void __launch_bounds__(256) k_lj_step(StepArgs A) {
+__global__ void __launch_bounds__(TB) k_lj_step(StepArgs A) {
+  __shared__ Stage S;
   publish_prev(A);
-  const uint32_t i = pb_xcd_block() * blockDim.x + threadIdx.x;
-  const bool inr = i < A.npad;
-  // a wavefront never spans two replicas: make the replica index scalar
-  const uint32_t rep = __builtin_amdgcn_readfirstlane(inr ? A.waveRep[i / PB_WAVE] : 0u);
-  const RepInfo &R = A.reps[rep];
-  double4 pi = inr ? A.rin[i] : make_double4(0, 0, 0, (double)PB_PAD_TYPE);
-  const bool real = inr && pi.w != (double)PB_PAD_TYPE;
-  const uint32_t nni = real ? A.nn[i] : 0u;
-  const uint32_t ti = ONE_TYPE ? 0u : (real ? (uint32_t)pi.w : 0u);
+  const uint32_t seg = pb_xcd_block();
+  const ListSeg G = A.segs[seg];
+  const RepInfo &R = A.reps[G.rep];
+  stage_window(S, G, R, A.cstart, A.rin);
+  const bool staged = S.total <= WCAP;
+  const bool mi = G.whole != 0;
+  const uint32_t ntile = S.rpref[S.nrun];
   const double L0 = R.L[0], L1 = R.L[1], L2 = R.L[2];
   const double I0 = R.Linv[0], I1 = R.Linv[1], I2 = R.Linv[2];
 
-  // forces and energies are accumulated in FP64 in both modes; MIXED
-  // evaluates each pair in FP32 on the FP64 displacement, after the FP64
-  // cutoff test (so both modes keep exactly the same pairs)
-  double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
-  auto pair = [&](const double4 &pj) {
-    double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-    dx -= L0 * rint(dx * I0);
-    dy -= L1 * rint(dy * I1);
-    dz -= L2 * rint(dz * I2);
-    uint32_t t = 0;
-    if (!ONE_TYPE) t = ti * PB_MAXT + (uint32_t)pj.w;
-    const double r2d = dx * dx + dy * dy + dz * dz;
-    if (r2d < A.rc2) {
-      if (MIXED) {
-        const float ex = (float)dx, ey = (float)dy, ez = (float)dz, r2 = (float)r2d;
-        const float s2 = ONE_TYPE ? A.s2f : A.lj->s2f[t];
-        const float e24 = ONE_TYPE ? A.e24f : A.lj->e24f[t];
-        float ir2 = __builtin_amdgcn_rcpf(r2);
-        ir2 = ir2 * (2.0f - r2 * ir2);                 // one Newton step: ~0.5 ulp
-        float s6 = s2 * ir2;
-        s6 = s6 * s6 * s6;
-        const float fr = e24 * s6 * (2.0f * s6 - 1.0f) * ir2;   // kernelinline.hh:90-95
-        fx += (double)(fr * ex);
-        fy += (double)(fr * ey);
-        fz += (double)(fr * ez);
-        if (ENERGY) {
-          const float e4 = ONE_TYPE ? A.e4f : A.lj->e4f[t];
-          const float sh = ONE_TYPE ? A.shiftf : A.lj->shiftf[t];
-          pe += (double)(e4 * s6 * (s6 - 1.0f) - sh);
-          vir += (double)(fr * r2);
-        }
-      } else {
-        const double s2 = ONE_TYPE ? A.s2 : A.lj->s2[t];
-        const double e24 = ONE_TYPE ? A.e24 : A.lj->e24[t];
-        const double ir2 = rcp64(r2d);
-        double s6 = s2 * ir2;
-        s6 = s6 * s6 * s6;
-        const double fr = e24 * s6 * (2.0 * s6 - 1.0) * ir2;
-        fx += fr * dx;
-        fy += fr * dy;
-        fz += fr * dz;
-        if (ENERGY) {
-          const double e4 = ONE_TYPE ? A.e4 : A.lj->e4[t];
-          const double sh = ONE_TYPE ? A.shift : A.lj->shift[t];
-          pe += e4 * s6 * (s6 - 1.0) - sh;
-          vir += fr * r2d;
+  double ke = 0.0, pe_b = 0.0, vir_b = 0.0;
+  float d2m = 0.f, v2m = 0.f;
+  for (uint32_t u = threadIdx.x; u < ntile; u += blockDim.x) {
+    uint32_t li, gi, run;
+    tile_particle(S, u, li, gi, run);
+    const double4 pi = window_pos(S, R, A.rin, staged, li);
+    const uint32_t ti = ONE_TYPE ? 0u : (uint32_t)pi.w;
+    const uint32_t nni = A.nn[gi];
+    // forces and energies accumulate in FP64 in both modes; MIXED evaluates
+    // each pair in FP32 on the FP64 displacement after the FP64 cutoff test
+    double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
+    auto pair = [&](const double4 &pj) {
+      double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      if (mi) {
+        dx -= L0 * rint(dx * I0);
+        dy -= L1 * rint(dy * I1);
+        dz -= L2 * rint(dz * I2);
+      }
+      uint32_t t = 0;
+      if (!ONE_TYPE) t = ti * PB_MAXT + (uint32_t)pj.w;
+      const double r2d = dx * dx + dy * dy + dz * dz;
+      if (r2d < A.rc2) {
+        if (MIXED) {
+          const float ex = (float)dx, ey = (float)dy, ez = (float)dz, r2 = (float)r2d;
+          const float s2 = ONE_TYPE ? A.s2f : A.lj->s2f[t];
+          const float e24 = ONE_TYPE ? A.e24f : A.lj->e24f[t];
+          float ir2 = __builtin_amdgcn_rcpf(r2);
+          ir2 = ir2 * (2.0f - r2 * ir2);               // one Newton step: ~0.5 ulp
+          float s6 = s2 * ir2;
+          s6 = s6 * s6 * s6;
+          const float fr = e24 * s6 * (2.0f * s6 - 1.0f) * ir2;   // kernelinline.hh:90-95
+          fx += (double)(fr * ex);
+          fy += (double)(fr * ey);
+          fz += (double)(fr * ez);
+          if (ENERGY) {
+            const float e4 = ONE_TYPE ? A.e4f : A.lj->e4f[t];
+            const float sh = ONE_TYPE ? A.shiftf : A.lj->shiftf[t];
+            pe += (double)(e4 * s6 * (s6 - 1.0f) - sh);
+            vir += (double)(fr * r2);
+          }
+        } else {
+          const double s2 = ONE_TYPE ? A.s2 : A.lj->s2[t];
+          const double e24 = ONE_TYPE ? A.e24 : A.lj->e24[t];
+          const double ir2 = rcp64(r2d);
+          double s6 = s2 * ir2;
+          s6 = s6 * s6 * s6;
+          const double fr = e24 * s6 * (2.0 * s6 - 1.0) * ir2;
+          fx += fr * dx;
+          fy += fr * dy;
+          fz += fr * dz;
+          if (ENERGY) {
+            const double e4 = ONE_TYPE ? A.e4 : A.lj->e4[t];
+            const double sh = ONE_TYPE ? A.shift : A.lj->shift[t];
+            pe += e4 * s6 * (s6 - 1.0) - sh;
+            vir += fr * r2d;
+          }
         }
       }
+    };
+    const uint32_t *nlp = A.nl + gi;
+    const size_t ld = A.npad;
+    uint32_t k = 0;
+    for (; k + 4 <= nni; k += 4) {
+      const uint32_t q0 = nlp[(size_t)k * ld], q1 = nlp[(size_t)(k + 1) * ld],
+                     q2 = nlp[(size_t)(k + 2) * ld], q3 = nlp[(size_t)(k + 3) * ld];
+      pair(window_pos(S, R, A.rin, staged, q0));
+      pair(window_pos(S, R, A.rin, staged, q1));
+      pair(window_pos(S, R, A.rin, staged, q2));
+      pair(window_pos(S, R, A.rin, staged, q3));
     }
-  };
-  // four list entries and their four gathers are issued before any pair is
-  // evaluated, so each wave keeps several memory requests in flight
-  const uint32_t *nlp = A.nl + i;
-  const size_t ld = A.npad;
-  uint32_t k = 0;
-  for (; k + 4 <= nni; k += 4) {
-    const uint32_t j0 = nlp[(size_t)k * ld], j1 = nlp[(size_t)(k + 1) * ld],
-                   j2 = nlp[(size_t)(k + 2) * ld], j3 = nlp[(size_t)(k + 3) * ld];
-    const double4 p0 = A.rin[j0], p1 = A.rin[j1], p2 = A.rin[j2], p3 = A.rin[j3];
-    pair(p0);
-    pair(p1);
-    pair(p2);
-    pair(p3);
-  }
-  for (; k < nni; ++k) pair(A.rin[nlp[(size_t)k * ld]]);
+    for (; k < nni; ++k) pair(window_pos(S, R, A.rin, staged, nlp[(size_t)k * ld]));
 
-  double ke = 0.0;
-  if (INTEGRATE) {
-    float d2 = 0.f, v2 = 0.f;
-    if (inr) {
-      double4 vi = A.vin[i];
-      if (real) {
-        // the list is valid for this step iff the positions the pair loop
-        // used are within skin/2 of their positions at the last rebuild
-        const float4 rb = A.rbuild[i];
-        {
-          const float ex = (float)(pi.x - (double)rb.x), ey = (float)(pi.y - (double)rb.y),
-                      ez = (float)(pi.z - (double)rb.z);
-          if ((double)(ex * ex + ey * ey + ez * ez) > A.half_skin2) atomicOr(&A.counter[CNT_SKIN], 1u);
-        }
-        // kick-drift form of propagateVelocityVerlet_F4 (kernelfuncs.cu:
-        // 347-365): v holds v(t - dt/2) = v_ref(t-dt) + Fold dt/2m
-        const double dtm2 = 0.5 * A.dt * vi.w;
-        const double vx = vi.x + fx * dtm2, vy = vi.y + fy * dtm2, vz = vi.z + fz * dtm2;  // v(t)
-        if (ENERGY) ke = 0.5 * (vx * vx + vy * vy + vz * vz) / vi.w;
-        vi.x = vx + fx * dtm2;
-        vi.y = vy + fy * dtm2;
-        vi.z = vz + fz * dtm2;
-        pi.x += vi.x * A.dt;
-        pi.y += vi.y * A.dt;
-        pi.z += vi.z * A.dt;
-        // displacement of the new positions feeds the rebuild schedule
-        const float ddx = (float)(pi.x - (double)rb.x), ddy = (float)(pi.y - (double)rb.y),
-                    ddz = (float)(pi.z - (double)rb.z);
-        d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-        v2 = (float)(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z);
+    if (INTEGRATE) {
+      double4 p = A.rin[gi];       // unshifted FP64 position
+      double4 vi = A.vin[gi];
+      // the list is valid for this step iff the positions the pair loop used
+      // are within skin/2 of their positions at the last rebuild
+      const float4 rb = A.rbuild[gi];
+      {
+        const float ex = (float)(p.x - (double)rb.x), ey = (float)(p.y - (double)rb.y),
+                    ez = (float)(p.z - (double)rb.z);
+        if ((double)(ex * ex + ey * ey + ez * ez) > A.half_skin2) atomicOr(&A.counter[CNT_SKIN], 1u);
       }
-      A.rout[i] = pi;
-      A.vout[i] = vi;
+      // kick-drift form of propagateVelocityVerlet_F4 (kernelfuncs.cu:
+      // 347-365): v holds v(t - dt/2) = v_ref(t-dt) + Fold dt/2m
+      const double dtm2 = 0.5 * A.dt * vi.w;
+      const double vx = vi.x + fx * dtm2, vy = vi.y + fy * dtm2, vz = vi.z + fz * dtm2;  // v(t)
+      if (ENERGY) ke += 0.5 * (vx * vx + vy * vy + vz * vz) / vi.w;
+      vi.x = vx + fx * dtm2;
+      vi.y = vy + fy * dtm2;
+      vi.z = vz + fz * dtm2;
+      p.x += vi.x * A.dt;
+      p.y += vi.y * A.dt;
+      p.z += vi.z * A.dt;
+      A.rout[gi] = p;
+      A.vout[gi] = vi;
+      // displacement of the new positions feeds the rebuild schedule
+      const float ddx = (float)(p.x - (double)rb.x), ddy = (float)(p.y - (double)rb.y),
+                  ddz = (float)(p.z - (double)rb.z);
+      d2m = fmaxf(d2m, ddx * ddx + ddy * ddy + ddz * ddz);
+      v2m = fmaxf(v2m, (float)(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z));
     }
+    if (WRITE_F) A.Fout[gi] = make_double4(fx, fy, fz, 0.0);
+    if (ENERGY) {
+      pe_b += 0.5 * pe;
+      vir_b += 0.5 * vir;
+    }
+  }
+
+  __shared__ float smx[2][TB / 64];
+  __shared__ double sen[3][TB / 64];
+  if (INTEGRATE) {
     // block maxima of displacement^2 and speed^2; an atomic only when the
     // block beats the running maximum (non-negative floats order as uints)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-      d2 = fmaxf(d2, __shfl_xor(d2, o, 64));
-      v2 = fmaxf(v2, __shfl_xor(v2, o, 64));
+      d2m = fmaxf(d2m, __shfl_xor(d2m, o, 64));
+      v2m = fmaxf(v2m, __shfl_xor(v2m, o, 64));
     }
-    __shared__ float smx[2][8];
     if (pb_lane() == 0) {
-      smx[0][threadIdx.x / 64] = d2;
-      smx[1][threadIdx.x / 64] = v2;
+      smx[0][threadIdx.x / 64] = d2m;
+      smx[1][threadIdx.x / 64] = v2m;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
+  }
+  if (ENERGY) {
+    const double kw = pb_wave_sum(ke), pw = pb_wave_sum(pe_b), ww = pb_wave_sum(vir_b);
+    if (pb_lane() == 0) {
+      sen[0][threadIdx.x / 64] = kw;
+      sen[1][threadIdx.x / 64] = pw;
+      sen[2][threadIdx.x / 64] = ww;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t nw = blockDim.x / 64;
+    if (INTEGRATE) {
       float bd = 0.f, bv = 0.f;
-      for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
+      for (uint32_t w = 0; w < nw; ++w) {
         bd = fmaxf(bd, smx[0][w]);
         bv = fmaxf(bv, smx[1][w]);
       }
@@ -486,19 +591,19 @@ __global__ void __launch_bounds__(256) k_lj_step(StepArgs A) {
       if (uv > __hip_atomic_load(&A.counter[CNT_VMAX], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
         atomicMax(&A.counter[CNT_VMAX], uv);
     }
-  }
-  if (WRITE_F && inr) A.Fout[i] = make_double4(fx, fy, fz, 0.0);
-
-  if (ENERGY) {
-    // per-wave partials (a wave never spans two replicas) ...
-    const double kw = pb_wave_sum(ke), pw = pb_wave_sum(0.5 * pe), ww = pb_wave_sum(0.5 * vir);
-    if (pb_lane() == 0 && inr) {
-      double *P = A.partials + 4 * (size_t)(i / PB_WAVE);
-      P[0] = kw;
-      P[1] = pw;
-      P[2] = ww;
+    if (ENERGY) {
+      // per-segment partials (a segment never spans two replicas), summed
+      // per replica in a fixed order by the last workgroup to finish
+      double acc[3] = {0, 0, 0};
+      for (uint32_t w = 0; w < nw; ++w)
+        for (int c = 0; c < 3; ++c) acc[c] += sen[c][w];
+      double *P = A.partials + 4 * (size_t)seg;
+      P[0] = acc[0];
+      P[1] = acc[1];
+      P[2] = acc[2];
     }
-    // ... summed per replica by the last workgroup to finish, in a fixed order
+  }
+  if (ENERGY) {
     __shared__ bool last;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -552,24 +657,41 @@ static int setup_list_cells(pb_sys *s, double rlist) {
   }
   s->list_ncell = cell0;
   s->list_rlist = rlist;
-  // list-build segments: z-runs of LB_CZ cells of every column
+  // Tiles (segments) of tx x ty columns x nz cells.  nz is chosen so the
+  // expected window (tx+2)(ty+2)(nz+2) cells stays near 3/4 of the LDS stage;
+  // nz <= nb2 - 2 so a window never holds one cell twice.  A replica with
+  // fewer than 3 list cells along an axis is one whole segment.
   std::vector<ListSeg> segs;
-  double lmax = 0.0;
   for (uint32_t q = 0; q < s->nrep; ++q) {
-    const RepInfo &R = s->reps[q];
-    for (int a = 0; a < 3; ++a) lmax = std::max(lmax, std::fabs(R.lo[a]) + R.L[a]);
-    for (uint32_t ci = 0; ci < R.nb[0]; ++ci)
-      for (uint32_t cj = 0; cj < R.nb[1]; ++cj)
-        for (uint32_t ck = 0; ck < R.nb[2]; ck += LB_CZ)
-          segs.push_back(ListSeg{q, ci, cj, ck, std::min<uint32_t>(LB_CZ, R.nb[2] - ck)});
+    RepInfo &R = s->reps[q];
+    R.seg0 = (uint32_t)segs.size();
+    const uint32_t nc = R.nb[0] * R.nb[1] * R.nb[2];
+    if (R.nb[0] < 3 || R.nb[1] < 3 || R.nb[2] < 3) {
+      if (R.natoms > WCAP)
+        return pb_fail(s->ctx, PB_ERR_CELLS,
+                       "a replica thinner than 3 list cells must hold at most 2816 particles");
+      segs.push_back(ListSeg{q, 0, 0, 0, 1, 1, 1, 1});
+    } else {
+      const uint32_t tx = R.nb[0] >= 4 ? 2 : 1, ty = R.nb[1] >= 4 ? 2 : 1;
+      const double per_cell = (double)R.natoms / nc;
+      int nz = (int)std::floor(0.72 * WCAP / ((tx + 2) * (ty + 2) * std::max(per_cell, 1e-3))) - 2;
+      nz = std::max(1, std::min<int>({nz, MAXZ, (int)R.nb[2] - 2}));
+      for (uint32_t ci = 0; ci < R.nb[0]; ci += tx)
+        for (uint32_t cj = 0; cj < R.nb[1]; cj += ty)
+          for (uint32_t ck = 0; ck < R.nb[2]; ck += (uint32_t)nz)
+            segs.push_back(ListSeg{q, ci, cj, ck, std::min<uint32_t>((uint32_t)nz, R.nb[2] - ck),
+                                   std::min(tx, R.nb[0] - ci), std::min(ty, R.nb[1] - cj), 0});
+    }
+    R.nseg = (uint32_t)segs.size() - R.seg0;
   }
   s->list_nsegs = (uint32_t)segs.size();
-  if (int rc = pb_alloc(s->ctx, &s->list_segs, 5 * segs.size())) return rc;
+  if (int rc = pb_alloc(s->ctx, &s->list_segs, 8 * segs.size())) return rc;
   PB_HIP(s->ctx, hipMemcpyAsync(s->list_segs, segs.data(), sizeof(ListSeg) * segs.size(),
                                 hipMemcpyHostToDevice, s->ctx->stream));
-  // FP32 build radius: covers the rounding of FP32 coordinates up to |x| = lmax
-  const double rb = rlist + 1e-4 + 8.0 * lmax * 0x1p-24;
-  s->list_rb2 = (float)(rb * rb);
+  // per-segment energy partials
+  if (s->list_nsegs > s->nwave) {
+    if (int rc = pb_alloc(s->ctx, &s->partials, 4 * (size_t)s->list_nsegs)) return rc;
+  }
   PB_HIP(s->ctx, hipMemcpyAsync(s->d_reps, s->reps.data(), sizeof(RepInfo) * s->nrep,
                                 hipMemcpyHostToDevice, s->ctx->stream));
   PB_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));   // segs is a host temporary
@@ -618,9 +740,9 @@ static int rebuild(pb_sys *s) {
   std::swap(s->orig, s->orig2);
   std::swap(s->type, s->type2);
   std::swap(s->move, s->move2);
-  hipLaunchKernelGGL(k_build_list, dim3(s->list_nsegs), dim3(LB_BS), 0, st, s->r, s->rank,
-                     s->cstart, (const ListSeg *)s->list_segs, s->d_reps, n, s->list_rb2,
-                     s->list_rlist * s->list_rlist, s->maxnb, s->nl, s->nn, s->d_counter);
+  hipLaunchKernelGGL(k_build_list, dim3(s->list_nsegs), dim3(TB), 0, st, s->r, s->cstart,
+                     (const ListSeg *)s->list_segs, s->d_reps, n, s->list_rlist * s->list_rlist,
+                     s->maxnb, s->nl, s->nn, s->d_counter);
   PB_LAUNCH_CHECK(s->ctx, "k_build_list");
   s->ref_binned = false;
   return PB_OK;
@@ -632,7 +754,8 @@ static StepArgs make_args(pb_sys *s, double dt) {
   A.nl = s->nl;
   A.nn = s->nn;
   A.rbuild = s->rbuild;
-  A.waveRep = s->waveRep;
+  A.cstart = s->cstart;
+  A.segs = (const ListSeg *)s->list_segs;
   A.reps = s->d_reps;
   A.lj = s->d_lj;
   A.partials = s->partials;
@@ -641,7 +764,6 @@ static StepArgs make_args(pb_sys *s, double dt) {
   A.nrep = s->nrep;
   A.dt = dt;
   A.rc2 = s->rc2;
-  A.rc2f = (float)s->rc2;
   const double hs = 0.5 * (s->list_rlist - std::sqrt(s->rc2));
   A.half_skin2 = hs * hs;
   A.s2 = s->lj.s2[0];
@@ -675,7 +797,7 @@ static int launch_step(pb_sys *s, const StepArgs &A, bool mixed, bool energy, bo
                        bool integ) {
   const bool one = s->lj.ntypes == 1;
   StepKernel k = pick_kernel(mixed, one, energy, wf, integ);
-  hipLaunchKernelGGL(k, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, s->ctx->stream, A);
+  hipLaunchKernelGGL(k, dim3(s->list_nsegs), dim3(TB), 0, s->ctx->stream, A);
   PB_LAUNCH_CHECK(s->ctx, "k_lj_step");
   return PB_OK;
 }
