@@ -222,16 +222,19 @@ __device__ __forceinline__ void pb_block_reduce_reps(const double *partials, int
     }
     return;
   }
+  // the first 256 threads reduce (any block size >= 256 may call this)
+  const uint32_t T = 256;
   for (uint32_t q = 0; q < nrep; ++q) {
     const uint32_t w0 = by_segment ? reps[q].seg0 : reps[q].wave0;
     const uint32_t nw = by_segment ? reps[q].nseg : reps[q].nwave;
     double acc[4] = {0, 0, 0, 0};
-    for (uint32_t w = t; w < nw; w += bs)
-      for (int c = 0; c < K; ++c)
-        acc[c] += partials[4 * (w0 + w) + c];
-    for (int c = 0; c < K; ++c) sm[c][t] = acc[c];
+    if (t < T)
+      for (uint32_t w = t; w < nw; w += T)
+        for (int c = 0; c < K; ++c) acc[c] += partials[4 * (w0 + w) + c];
+    if (t < T)
+      for (int c = 0; c < K; ++c) sm[c][t] = acc[c];
     __syncthreads();
-    for (uint32_t o = bs / 2; o > 0; o >>= 1) {
+    for (uint32_t o = T / 2; o > 0; o >>= 1) {
       if (t < o)
         for (int c = 0; c < K; ++c) sm[c][t] += sm[c][t + o];
       __syncthreads();
@@ -240,5 +243,6 @@ __device__ __forceinline__ void pb_block_reduce_reps(const double *partials, int
       for (int c = 0; c < K; ++c) out[8 * q + c] = sm[c][0];
     __syncthreads();
   }
+}
 }
 
