@@ -244,5 +244,3 @@ __device__ __forceinline__ void pb_block_reduce_reps(const double *partials, int
     __syncthreads();
   }
 }
-}
-
