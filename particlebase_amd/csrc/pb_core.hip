@@ -662,7 +662,8 @@ __global__ void k_inspect_final(const double *partials, uint32_t nwave, double *
 __global__ void __launch_bounds__(256) k_reduce_waves(const double *partials, int K,
                                                       const RepInfo *reps, uint32_t nrep,
                                                       double *out) {
-  pb_block_reduce_reps(partials, K, reps, nrep, out, false);
+  __shared__ double sm[4 * 256];
+  pb_block_reduce_reps(partials, K, reps, nrep, out, false, sm);
 }
 
 int pb_reduce_waves(pb_sys *s, int K) {

@@ -202,13 +202,15 @@ __device__ __forceinline__ double pb_load_relaxed(const double *p) {
 // sum of K per-wave partial components for every replica, executed by one
 // block (used by the last-block reductions and by k_reduce_waves).
 __device__ __forceinline__ void pb_block_reduce_reps(const double *partials, int K, const RepInfo *reps,
-                                     uint32_t nrep, double *out, bool by_segment) {
+                                     uint32_t nrep, double *out, bool by_segment,
+                                     double *scratch) {
   // Deterministic: every replica's partials (one per wave, or one per
   // pair-kernel segment when by_segment) are summed in a fixed order that
   // depends only on the launch shape of this block.  Callers that read
   // partials written by other workgroups issue a device-scope fence first
   // (it invalidates this CU's vector L1), so plain loads are coherent.
-  __shared__ double sm[4][256];
+  // scratch: 4 x 256 doubles of the caller's LDS
+  double (*sm)[256] = reinterpret_cast<double (*)[256]>(scratch);
   const uint32_t t = threadIdx.x, bs = blockDim.x;
   if (nrep >= 16) {
     for (uint32_t q = t; q < nrep; q += bs) {

@@ -174,13 +174,23 @@ struct ListSeg {
 #define MAXZ 16           // nz <= MAXZ
 #define MAXE (MAXCOL * (MAXZ + 2))
 
-struct Stage {
-  double x[WCAP], y[WCAP], z[WCAP];
-  uint8_t t[WCAP];
+// entry table of a window: entry e = (window column, z slot) -> LDS offset,
+// storage start, count, packed image shift; plus the runs of tile particles
+struct WinTable {
   uint32_t eoff[MAXE + 1], eg0[MAXE], ec[MAXE], esh[MAXE];
   uint32_t nent, total, ncol, zs;
   uint32_t nrun, rloc[4], rpref[5], rg0[4], rcol[4];
   uint32_t wsum[TB / 64];
+};
+struct Stage {              // pair kernel: FP64 positions with shifts applied
+  double x[WCAP], y[WCAP], z[WCAP];
+  uint8_t t[WCAP];
+  WinTable W;
+};
+#define WCAPB 4096          // build kernel stages FP32 (16 B) particles
+struct StageB {             // list build: FP32 window-relative positions
+  float4 p[WCAPB];
+  WinTable W;
 };
 
 // exclusive block scan (blockDim multiple of 64, <= TB)
@@ -219,79 +229,99 @@ __device__ __forceinline__ void entry_cell(const ListSeg &G, const RepInfo &R, u
   sh = (uint32_t)(sx + 1) | ((uint32_t)(sy + 1) << 2) | ((uint32_t)(sz + 1) << 4);
 }
 
-// Build the entry table of the window and (if it fits) stage its positions.
-// Returns through S; all threads of the block must call it.
-__device__ void stage_window(Stage &S, const ListSeg &G, const RepInfo &R, const uint32_t *cstart,
-                             const double4 *r) {
+// Entry table of the segment's window (all threads of the block call it).
+__device__ void window_table(WinTable &W, const ListSeg &G, const RepInfo &R,
+                             const uint32_t *cstart) {
   const uint32_t tid = threadIdx.x;
   if (tid == 0) {
-    S.ncol = G.whole ? 1u : (G.tx + 2) * (G.ty + 2);
-    S.zs = G.whole ? 1u : G.nz + 2;
-    S.nent = S.ncol * S.zs;
+    W.ncol = G.whole ? 1u : (G.tx + 2) * (G.ty + 2);
+    W.zs = G.whole ? 1u : G.nz + 2;
+    W.nent = W.ncol * W.zs;
   }
   __syncthreads();
   uint32_t c = 0;
-  if (tid < S.nent) {
+  if (tid < W.nent) {
     uint32_t g0, sh = 0x15;   // no shift
     if (G.whole) {
       g0 = R.atom0;
       c = R.natoms;
     } else {
       uint32_t J;
-      entry_cell(G, R, tid, S.zs, J, sh);
+      entry_cell(G, R, tid, W.zs, J, sh);
       g0 = cstart[J];
       c = cstart[J + 1] - g0;
     }
-    S.eg0[tid] = g0;
-    S.ec[tid] = c;
-    S.esh[tid] = sh;
+    W.eg0[tid] = g0;
+    W.ec[tid] = c;
+    W.esh[tid] = sh;
   }
   uint32_t tot;
-  const uint32_t off = tile_scan(c, S.wsum, &tot);
-  if (tid < S.nent) S.eoff[tid] = off;
+  const uint32_t off = tile_scan(c, W.wsum, &tot);
+  if (tid < W.nent) W.eoff[tid] = off;
   if (tid == 0) {
-    S.eoff[S.nent] = tot;
-    S.total = tot;
+    W.eoff[W.nent] = tot;
+    W.total = tot;
     // runs of the tile's own particles: inner columns, inner z slots
     if (G.whole) {
-      S.nrun = 1;
-      S.rloc[0] = 0;
-      S.rg0[0] = R.atom0;
-      S.rcol[0] = 0;
-      S.rpref[0] = 0;
-      S.rpref[1] = tot;
+      W.nrun = 1;
+      W.rloc[0] = 0;
+      W.rg0[0] = R.atom0;
+      W.rcol[0] = 0;
+      W.rpref[0] = 0;
+      W.rpref[1] = tot;
     } else {
       uint32_t n = 0, pref = 0;
       for (uint32_t a = 0; a < G.tx; ++a)
         for (uint32_t b = 0; b < G.ty; ++b) {
           const uint32_t col = (a + 1) * (G.ty + 2) + (b + 1);
-          const uint32_t e0 = col * S.zs + 1;
-          S.rloc[n] = S.eoff[e0];
-          S.rg0[n] = S.eg0[e0];
-          S.rcol[n] = col;
-          S.rpref[n] = pref;
-          pref += S.eoff[e0 + G.nz] - S.eoff[e0];
+          const uint32_t e0 = col * W.zs + 1;
+          W.rloc[n] = W.eoff[e0];
+          W.rg0[n] = W.eg0[e0];
+          W.rcol[n] = col;
+          W.rpref[n] = pref;
+          pref += W.eoff[e0 + G.nz] - W.eoff[e0];
           ++n;
         }
-      S.nrun = n;
-      S.rpref[n] = pref;
+      W.nrun = n;
+      W.rpref[n] = pref;
     }
   }
   __syncthreads();
-  if (S.total <= WCAP) {
-    const double Lx = R.L[0], Ly = R.L[1], Lz = R.L[2];
-    for (uint32_t q = tid; q < S.total; q += blockDim.x) {
-      uint32_t lo = 0, hi = S.nent;                 // largest e with eoff[e] <= q
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (S.eoff[mid] <= q) lo = mid;
-        else hi = mid;
-      }
-      const uint32_t sh = S.esh[lo];
-      const double4 p = r[S.eg0[lo] + (q - S.eoff[lo])];
-      S.x[q] = p.x + (double)((int)(sh & 3u) - 1) * Lx;
-      S.y[q] = p.y + (double)((int)((sh >> 2) & 3u) - 1) * Ly;
-      S.z[q] = p.z + (double)((int)((sh >> 4) & 3u) - 1) * Lz;
+}
+
+// entry holding window-local particle q (largest e with eoff[e] <= q)
+__device__ __forceinline__ uint32_t window_entry(const WinTable &W, uint32_t q) {
+  uint32_t lo = 0, hi = W.nent;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (W.eoff[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// image-shifted FP64 position of window particle q read from global memory
+__device__ __forceinline__ double4 window_global(const WinTable &W, const RepInfo &R,
+                                                 const double4 *r, uint32_t q) {
+  const uint32_t e = window_entry(W, q);
+  const uint32_t sh = W.esh[e];
+  double4 p = r[W.eg0[e] + (q - W.eoff[e])];
+  p.x += (double)((int)(sh & 3u) - 1) * R.L[0];
+  p.y += (double)((int)((sh >> 2) & 3u) - 1) * R.L[1];
+  p.z += (double)((int)((sh >> 4) & 3u) - 1) * R.L[2];
+  return p;
+}
+
+// Stage the window in FP64 (pair kernel) when it fits.
+__device__ void stage_window(Stage &S, const ListSeg &G, const RepInfo &R, const uint32_t *cstart,
+                             const double4 *r) {
+  window_table(S.W, G, R, cstart);
+  if (S.W.total <= WCAP) {
+    for (uint32_t q = threadIdx.x; q < S.W.total; q += blockDim.x) {
+      const double4 p = window_global(S.W, R, r, q);
+      S.x[q] = p.x;
+      S.y[q] = p.y;
+      S.z[q] = p.z;
       S.t[q] = (uint8_t)p.w;
     }
   }
@@ -303,77 +333,100 @@ __device__ void stage_window(Stage &S, const ListSeg &G, const RepInfo &R, const
 __device__ __forceinline__ double4 window_pos(const Stage &S, const RepInfo &R, const double4 *r,
                                               bool staged, uint32_t q) {
   if (staged) return make_double4(S.x[q], S.y[q], S.z[q], (double)S.t[q]);
-  uint32_t lo = 0, hi = S.nent;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (S.eoff[mid] <= q) lo = mid;
-    else hi = mid;
-  }
-  const uint32_t sh = S.esh[lo];
-  double4 p = r[S.eg0[lo] + (q - S.eoff[lo])];
-  p.x += (double)((int)(sh & 3u) - 1) * R.L[0];
-  p.y += (double)((int)((sh >> 2) & 3u) - 1) * R.L[1];
-  p.z += (double)((int)((sh >> 4) & 3u) - 1) * R.L[2];
-  return p;
+  return window_global(S.W, R, r, q);
 }
 
 // tile particle u (0 <= u < rpref[nrun]) -> window-local index, storage slot, run
-__device__ __forceinline__ void tile_particle(const Stage &S, uint32_t u, uint32_t &li,
+__device__ __forceinline__ void tile_particle(const WinTable &W, uint32_t u, uint32_t &li,
                                               uint32_t &gi, uint32_t &run) {
   uint32_t k = 0;
-  while (k + 1 < S.nrun && S.rpref[k + 1] <= u) ++k;
+  while (k + 1 < W.nrun && W.rpref[k + 1] <= u) ++k;
   run = k;
-  li = S.rloc[k] + (u - S.rpref[k]);
-  gi = S.rg0[k] + (u - S.rpref[k]);
+  li = W.rloc[k] + (u - W.rpref[k]);
+  gi = W.rg0[k] + (u - W.rpref[k]);
 }
 
-// Verlet-list build: every tile particle scans its 27 neighbour cells of the
-// window (all staged particles in whole mode) and keeps window-local q with
-// |r_iq| < rc + skin (FP64).
+// Verlet-list build: the window is staged in LDS as FP32 coordinates
+// relative to the window's lower corner (image shifts applied; |x| < 4 list
+// cells, so FP32 keeps ~1e-6 absolute); every tile particle scans its 27
+// neighbour cells (all staged particles in whole mode) and keeps
+// window-local q with |r_iq| < rb, where rb = rc + skin + a margin covering
+// the FP32 rounding: the list is a superset of the FP64 pairs within
+// rc + skin.  A window larger than the stage reads FP64 from global memory.
 __global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint32_t *cstart,
                                                    const ListSeg *segs, const RepInfo *reps,
-                                                   uint32_t npad, double rlist2, uint32_t maxnb,
-                                                   uint32_t *nl, uint32_t *nn, uint32_t *counter) {
-  __shared__ Stage S;
+                                                   uint32_t npad, double rlist2, float rb2,
+                                                   uint32_t maxnb, uint32_t *nl, uint32_t *nn,
+                                                   uint32_t *counter) {
+  __shared__ StageB S;
   const ListSeg G = segs[pb_xcd_block()];
   const RepInfo &R = reps[G.rep];
-  stage_window(S, G, R, cstart, r);
-  const bool staged = S.total <= WCAP;
-  const uint32_t ntile = S.rpref[S.nrun];
+  WinTable &W = S.W;
+  window_table(W, G, R, cstart);
+  const bool staged = W.total <= WCAPB;
+  // window origin (FP64): lower corner of the first window cell
+  const double ox = R.lo[0] + ((double)G.ci - 1.0) * R.L[0] / R.nb[0];
+  const double oy = R.lo[1] + ((double)G.cj - 1.0) * R.L[1] / R.nb[1];
+  const double oz = R.lo[2] + ((double)G.ck0 - 1.0) * R.L[2] / R.nb[2];
+  if (staged)
+    for (uint32_t q = threadIdx.x; q < W.total; q += blockDim.x) {
+      const double4 p = window_global(W, R, r, q);
+      S.p[q] = make_float4((float)(p.x - ox), (float)(p.y - oy), (float)(p.z - oz), 0.f);
+    }
+  __syncthreads();
+  const uint32_t ntile = W.rpref[W.nrun];
+  const float Lx = (float)R.L[0], Ly = (float)R.L[1], Lz = (float)R.L[2];
+  const float Ix = (float)R.Linv[0], Iy = (float)R.Linv[1], Iz = (float)R.Linv[2];
   uint32_t mymax = 0;
   for (uint32_t u = threadIdx.x; u < ntile; u += blockDim.x) {
     uint32_t li, gi, run;
-    tile_particle(S, u, li, gi, run);
-    const double4 pi = window_pos(S, R, r, staged, li);
+    tile_particle(W, u, li, gi, run);
     uint32_t cnt = 0;
-    auto consider = [&](uint32_t q, bool mi) {
-      if (q == li) return;
-      const double4 pj = window_pos(S, R, r, staged, q);
-      double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-      if (mi) {
-        dx -= R.L[0] * rint(dx * R.Linv[0]);
-        dy -= R.L[1] * rint(dy * R.Linv[1]);
-        dz -= R.L[2] * rint(dz * R.Linv[2]);
-      }
-      if (dx * dx + dy * dy + dz * dz < rlist2) {
-        if (cnt < maxnb) nl[(size_t)cnt * npad + gi] = q;
-        ++cnt;
+    auto keep = [&](uint32_t q) {
+      if (cnt < maxnb) nl[(size_t)cnt * npad + gi] = q;
+      ++cnt;
+    };
+    auto scan = [&](uint32_t q0, uint32_t q1, bool mi) {
+      if (staged) {
+        const float4 pi = S.p[li];
+        for (uint32_t q = q0; q < q1; ++q) {
+          const float4 pj = S.p[q];
+          float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+          if (mi) {
+            dx -= Lx * rintf(dx * Ix);
+            dy -= Ly * rintf(dy * Iy);
+            dz -= Lz * rintf(dz * Iz);
+          }
+          if (dx * dx + dy * dy + dz * dz < rb2 && q != li) keep(q);
+        }
+      } else {
+        const double4 pi = window_global(W, R, r, li);
+        for (uint32_t q = q0; q < q1; ++q) {
+          if (q == li) continue;
+          const double4 pj = window_global(W, R, r, q);
+          double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+          if (mi) {
+            dx -= R.L[0] * rint(dx * R.Linv[0]);
+            dy -= R.L[1] * rint(dy * R.Linv[1]);
+            dz -= R.L[2] * rint(dz * R.Linv[2]);
+          }
+          if (dx * dx + dy * dy + dz * dz < rlist2) keep(q);
+        }
       }
     };
     if (G.whole) {
-      for (uint32_t q = 0; q < S.total; ++q) consider(q, true);
+      scan(0, W.total, true);
     } else {
       // z slot of this particle: the inner entry of its column holding li
-      const uint32_t col = S.rcol[run];
+      const uint32_t col = W.rcol[run];
       uint32_t ti = 1;
-      while (ti < G.nz && S.eoff[col * S.zs + ti + 1] <= li) ++ti;
+      while (ti < G.nz && W.eoff[col * W.zs + ti + 1] <= li) ++ti;
       const uint32_t wy = G.ty + 2;
       const uint32_t ca = col / wy, cb = col % wy;
       for (uint32_t a = ca - 1; a <= ca + 1; ++a)
         for (uint32_t b = cb - 1; b <= cb + 1; ++b) {
-          const uint32_t e0 = (a * wy + b) * S.zs + ti - 1;
-          const uint32_t q0 = S.eoff[e0], q1 = S.eoff[e0 + 3];   // three z slots
-          for (uint32_t q = q0; q < q1; ++q) consider(q, false);
+          const uint32_t e0 = (a * wy + b) * W.zs + ti - 1;
+          scan(W.eoff[e0], W.eoff[e0 + 3], false);   // three z slots
         }
     }
     nn[gi] = cnt < maxnb ? cnt : maxnb;
@@ -438,9 +491,9 @@ __global__ void __launch_bounds__(TB) k_lj_step(StepArgs A) {
   const ListSeg G = A.segs[seg];
   const RepInfo &R = A.reps[G.rep];
   stage_window(S, G, R, A.cstart, A.rin);
-  const bool staged = S.total <= WCAP;
+  const bool staged = S.W.total <= WCAP;
   const bool mi = G.whole != 0;
-  const uint32_t ntile = S.rpref[S.nrun];
+  const uint32_t ntile = S.W.rpref[S.W.nrun];
   const double L0 = R.L[0], L1 = R.L[1], L2 = R.L[2];
   const double I0 = R.Linv[0], I1 = R.Linv[1], I2 = R.Linv[2];
 
@@ -448,10 +501,18 @@ __global__ void __launch_bounds__(TB) k_lj_step(StepArgs A) {
   float d2m = 0.f, v2m = 0.f;
   for (uint32_t u = threadIdx.x; u < ntile; u += blockDim.x) {
     uint32_t li, gi, run;
-    tile_particle(S, u, li, gi, run);
+    tile_particle(S.W, u, li, gi, run);
     const double4 pi = window_pos(S, R, A.rin, staged, li);
     const uint32_t ti = ONE_TYPE ? 0u : (uint32_t)pi.w;
     const uint32_t nni = A.nn[gi];
+    // per-particle state for the integration, requested before the pair loop
+    double4 p_own = make_double4(0, 0, 0, 0), v_own = p_own;
+    float4 rb_own = make_float4(0, 0, 0, 0);
+    if (INTEGRATE) {
+      p_own = A.rin[gi];       // unshifted FP64 position
+      v_own = A.vin[gi];
+      rb_own = A.rbuild[gi];
+    }
     // forces and energies accumulate in FP64 in both modes; MIXED evaluates
     // each pair in FP32 on the FP64 displacement after the FP64 cutoff test
     double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
@@ -503,25 +564,30 @@ __global__ void __launch_bounds__(TB) k_lj_step(StepArgs A) {
         }
       }
     };
+    // The list is streamed from HBM: the next 8 entries are requested while
+    // the current 8 pairs are evaluated, so each wave keeps its list loads
+    // in flight behind the arithmetic.
     const uint32_t *nlp = A.nl + gi;
     const size_t ld = A.npad;
-    uint32_t k = 0;
-    for (; k + 4 <= nni; k += 4) {
-      const uint32_t q0 = nlp[(size_t)k * ld], q1 = nlp[(size_t)(k + 1) * ld],
-                     q2 = nlp[(size_t)(k + 2) * ld], q3 = nlp[(size_t)(k + 3) * ld];
-      pair(window_pos(S, R, A.rin, staged, q0));
-      pair(window_pos(S, R, A.rin, staged, q1));
-      pair(window_pos(S, R, A.rin, staged, q2));
-      pair(window_pos(S, R, A.rin, staged, q3));
+    uint32_t qa[8], qb[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) qa[j] = (uint32_t)j < nni ? nlp[(size_t)j * ld] : 0u;
+    for (uint32_t k = 0; k < nni; k += 8) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) qb[j] = k + 8 + j < nni ? nlp[(size_t)(k + 8 + j) * ld] : 0u;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (k + j < nni) pair(window_pos(S, R, A.rin, staged, qa[j]));
+#pragma unroll
+      for (int j = 0; j < 8; ++j) qa[j] = qb[j];
     }
-    for (; k < nni; ++k) pair(window_pos(S, R, A.rin, staged, nlp[(size_t)k * ld]));
 
     if (INTEGRATE) {
-      double4 p = A.rin[gi];       // unshifted FP64 position
-      double4 vi = A.vin[gi];
+      double4 p = p_own;
+      double4 vi = v_own;
+      const float4 rb = rb_own;
       // the list is valid for this step iff the positions the pair loop used
       // are within skin/2 of their positions at the last rebuild
-      const float4 rb = A.rbuild[gi];
       {
         const float ex = (float)(p.x - (double)rb.x), ey = (float)(p.y - (double)rb.y),
                     ez = (float)(p.z - (double)rb.z);
@@ -617,7 +683,7 @@ __global__ void __launch_bounds__(TB) k_lj_step(StepArgs A) {
       const uint32_t rec = __hip_atomic_load(&A.counter[CNT_REC], __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_AGENT);
       pb_block_reduce_reps(A.partials, 3, A.reps, A.nrep, A.energy + (size_t)rec * 8 * A.nrep,
-                           true);
+                           true, S.x);   // the stage is no longer needed
       __syncthreads();
       if (threadIdx.x == 0) {
         A.counter[CNT_REC] = rec + 1;
@@ -688,6 +754,9 @@ static int setup_list_cells(pb_sys *s, double rlist) {
   if (int rc = pb_alloc(s->ctx, &s->list_segs, 8 * segs.size())) return rc;
   PB_HIP(s->ctx, hipMemcpyAsync(s->list_segs, segs.data(), sizeof(ListSeg) * segs.size(),
                                 hipMemcpyHostToDevice, s->ctx->stream));
+  // FP32 build radius: window-relative coordinates stay below ~4 list cells
+  // plus the drift since the last wrap; 1e-4 covers their FP32 rounding
+  s->list_rb2 = (float)((rlist + 1e-4) * (rlist + 1e-4));
   // per-segment energy partials
   if (s->list_nsegs > s->nwave) {
     if (int rc = pb_alloc(s->ctx, &s->partials, 4 * (size_t)s->list_nsegs)) return rc;
@@ -742,7 +811,7 @@ static int rebuild(pb_sys *s) {
   std::swap(s->move, s->move2);
   hipLaunchKernelGGL(k_build_list, dim3(s->list_nsegs), dim3(TB), 0, st, s->r, s->cstart,
                      (const ListSeg *)s->list_segs, s->d_reps, n, s->list_rlist * s->list_rlist,
-                     s->maxnb, s->nl, s->nn, s->d_counter);
+                     s->list_rb2, s->maxnb, s->nl, s->nn, s->d_counter);
   PB_LAUNCH_CHECK(s->ctx, "k_build_list");
   s->ref_binned = false;
   return PB_OK;
