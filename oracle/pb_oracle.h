@@ -157,6 +157,40 @@ void pbo_gr_setup(float rstep, float rmax, float *rstepinv, uint32_t *rnum);
 void pbo_gr_normalize(const uint64_t *igr, uint32_t rnum, float rstepinv,
                       const float cell[6], uint32_t n, float *gr);
 
+/* ------------------------------------------------------------------ SPH */
+/* Reference physics of cudaParticleSPH_NS (pb_oracle_sph.c): Lucy kernel of
+ * support h, p = c^2 (rho - rho0) with rho0 = 1, physical viscosity.
+ * pinned by: int W 4 pi r^2 dr = 1, single-particle rho = 105 m/(16 pi h^3),
+ * two-particle closed forms, sum m_i a_i = 0 (tests/test_oracle_sph.py). */
+double pbo_sph_w0(double h);
+double pbo_sph_lucy_w(double r, double h, double w0);
+double pbo_sph_lucy_dw(double r, double h, double w1);
+void pbo_sph_density(const double *r, const double *m, const double *mu0, uint32_t n,
+                     double h, double *num_out, double *rhoinv);
+void pbo_sph_accel(const double *r, const double *v, const double *m, const double *c2,
+                   const double *rhoinv, const double *mu, uint32_t n, double h,
+                   double rho0, double *a);
+/* propagateLeapFrog_F4 (kernelfuncs.cu:270-296), FP64 */
+void pbo_lf_step(double *r, double *v, const double *a, const uint8_t *move, uint32_t n,
+                 double dt);
+
+/* ------------------------------------------------------------------ DEM */
+/* cudaParticleDEM restated in FP64 (pb_oracle_dem.c).
+ * pinned by: brute-force geometry, Hertz closed form on two spheres,
+ * Coulomb cap, torque = Rij x Fs, scripted history sequences, rotation
+ * closed form (tests/test_oracle_dem.py). */
+uint32_t pbo_dem_contacts(const double *r, const double *rad, const uint8_t *move, uint32_t n,
+                          uint32_t *rowptr, uint32_t *colidx);
+void pbo_dem_carry(const uint32_t *prow, const uint32_t *pcol, const double *pval,
+                   const uint32_t *crow, const uint32_t *ccol, double *cval, uint32_t n);
+uint32_t pbo_dem_force(const double *r, const double *v, const double *w, const double *rad,
+                       const double *minv, const uint8_t *move, uint32_t n,
+                       const uint32_t *rowptr, const uint32_t *colidx, double *val, double E,
+                       double mu, double s2, double gamma2, double mu_r, double dt, double *F,
+                       double *T);
+void pbo_rot_step(double *w, double *L, const double *T, const double *m0inv,
+                  const uint8_t *move, uint32_t n, double dt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,18 +64,23 @@ def lib():
 def _extra(L):
     """Signatures of the SPH / DEM restatements (pb_oracle_sph.c / _dem.c)."""
     P = C.POINTER
-    u32, d, i32 = C.c_uint32, C.c_double, C.c_int
+    u32, d, u8 = C.c_uint32, C.c_double, C.c_uint8
     for name, res, args in [
-        ("pbo_sph_density", i32, [P(d), P(d), P(d), u32, d, P(d), P(d), P(d)]),
-        ("pbo_sph_accel", i32, [P(d), P(d), P(d), P(d), P(d), P(d), P(d), u32, d, d, P(d)]),
-        ("pbo_dem_contacts", i32, [P(d), P(d), u32, P(u32), P(u32), u32, P(u32)]),
-        ("pbo_dem_force", i32, [P(d), P(d), P(d), P(d), P(d), P(C.c_uint8), u32,
-                                P(u32), P(u32), P(d), d, d, d, d, d, d, P(d), P(d)]),
+        ("pbo_sph_w0", d, [d]),
+        ("pbo_sph_lucy_w", d, [d, d, d]),
+        ("pbo_sph_lucy_dw", d, [d, d, d]),
+        ("pbo_sph_density", None, [P(d), P(d), P(d), u32, d, P(d), P(d)]),
+        ("pbo_sph_accel", None, [P(d), P(d), P(d), P(d), P(d), P(d), u32, d, d, P(d)]),
+        ("pbo_lf_step", None, [P(d), P(d), P(d), P(u8), u32, d]),
+        ("pbo_dem_contacts", u32, [P(d), P(d), P(u8), u32, P(u32), P(u32)]),
+        ("pbo_dem_carry", None, [P(u32), P(u32), P(d), P(u32), P(u32), P(d), u32]),
+        ("pbo_dem_force", u32, [P(d), P(d), P(d), P(d), P(d), P(u8), u32, P(u32), P(u32),
+                                P(d), d, d, d, d, d, d, P(d), P(d)]),
+        ("pbo_rot_step", None, [P(d), P(d), P(d), P(d), P(u8), u32, d]),
     ]:
-        if hasattr(L, name):
-            fn = getattr(L, name)
-            fn.restype = res
-            fn.argtypes = args
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
 
 
 def _p(a, t):
@@ -250,3 +255,101 @@ def gr_normalize(igr, rstepinv, cell6, n):
 
 
 import math  # noqa: E402  (used by lj_forces_cells)
+
+
+# ----------------------------------------------------------------- SPH
+def sph_w0(h):
+    return lib().pbo_sph_w0(h)
+
+
+def sph_lucy_w(r, h):
+    return lib().pbo_sph_lucy_w(r, h, sph_w0(h))
+
+
+def sph_lucy_dw(r, h):
+    return lib().pbo_sph_lucy_dw(r, h, -12.0 * sph_w0(h))
+
+
+def sph_density(r, m, mu0, h):
+    """-> num (n, 3: number density, rho, mu), rhoinv (n)."""
+    r, m, mu0 = f64(r), f64(m), f64(mu0)
+    n = r.shape[0]
+    num = np.zeros((n, 3))
+    rhoinv = np.zeros(n)
+    d = C.c_double
+    lib().pbo_sph_density(_p(r, d), _p(m, d), _p(mu0, d), n, h, _p(num, d), _p(rhoinv, d))
+    return num, rhoinv
+
+
+def sph_accel(r, v, m, c2, rhoinv, mu, h, rho0=1.0):
+    r, v, m, c2, rhoinv, mu = map(f64, (r, v, m, c2, rhoinv, mu))
+    n = r.shape[0]
+    a = np.zeros((n, 3))
+    d = C.c_double
+    lib().pbo_sph_accel(_p(r, d), _p(v, d), _p(m, d), _p(c2, d), _p(rhoinv, d), _p(mu, d), n,
+                        h, rho0, _p(a, d))
+    return a
+
+
+def lf_step(r, v, a, move, dt):
+    r, v, a = f64(r).copy(), f64(v).copy(), f64(a)
+    mv = np.ascontiguousarray(move, dtype=np.uint8)
+    d = C.c_double
+    lib().pbo_lf_step(_p(r, d), _p(v, d), _p(a, d), _p(mv, C.c_uint8), r.shape[0], dt)
+    return r, v
+
+
+# ----------------------------------------------------------------- DEM
+def dem_contacts(r, rad, move):
+    """-> rowptr (n+1), colidx (nnz): strict-overlap contacts of movable i."""
+    r, rad = f64(r), f64(rad)
+    mv = np.ascontiguousarray(move, dtype=np.uint8)
+    n = r.shape[0]
+    row = np.zeros(n + 1, dtype=np.uint32)
+    d, u32 = C.c_double, C.c_uint32
+    nnz = lib().pbo_dem_contacts(_p(r, d), _p(rad, d), _p(mv, C.c_uint8), n, _p(row, u32), None)
+    col = np.zeros(max(nnz, 1), dtype=np.uint32)
+    lib().pbo_dem_contacts(_p(r, d), _p(rad, d), _p(mv, C.c_uint8), n, _p(row, u32),
+                           _p(col, u32))
+    return row, col[:nnz]
+
+
+def dem_carry(prow, pcol, pval, crow, ccol):
+    n = len(crow) - 1
+    prow, pcol, crow, ccol = (np.ascontiguousarray(x, dtype=np.uint32) for x in
+                              (prow, pcol, crow, ccol))
+    pval = f64(pval)
+    cval = np.zeros(max(len(ccol), 1))
+    u32, d = C.c_uint32, C.c_double
+    pc = pcol if len(pcol) else np.zeros(1, dtype=np.uint32)
+    pv = pval if len(pval) else np.zeros(1)
+    cc = ccol if len(ccol) else np.zeros(1, dtype=np.uint32)
+    lib().pbo_dem_carry(_p(prow, u32), _p(pc, u32), _p(pv, d), _p(crow, u32), _p(cc, u32),
+                        _p(cval, d), n)
+    return cval[:len(ccol)]
+
+
+def dem_force(r, v, w, rad, minv, move, row, col, val, E, mu, s2, gamma2, mu_r, dt):
+    """-> F (n,3), T (n,3), updated history val, #Cundall-Strack violations."""
+    r, v, w, rad, minv = map(f64, (r, v, w, rad, minv))
+    mv = np.ascontiguousarray(move, dtype=np.uint8)
+    row = np.ascontiguousarray(row, dtype=np.uint32)
+    col = np.ascontiguousarray(col if len(col) else np.zeros(1), dtype=np.uint32)
+    val = f64(val).copy() if len(val) else np.zeros(1)
+    n = r.shape[0]
+    F = np.zeros((n, 3))
+    T = np.zeros((n, 3))
+    d, u32 = C.c_double, C.c_uint32
+    bad = lib().pbo_dem_force(_p(r, d), _p(v, d), _p(w, d), _p(rad, d), _p(minv, d),
+                              _p(mv, C.c_uint8), n, _p(row, u32), _p(col, u32), _p(val, d),
+                              E, mu, s2, gamma2, mu_r, dt, _p(F, d), _p(T, d))
+    return F, T, val[:int(row[-1])], bad
+
+
+def rot_step(w, L, T, m0inv, move, dt):
+    w, L, T, m0inv = f64(w).copy(), f64(L).copy(), f64(T), f64(m0inv)
+    mv = np.ascontiguousarray(move, dtype=np.uint8)
+    d = C.c_double
+    lib().pbo_rot_step(_p(w, d), _p(L, d), _p(T, d), _p(m0inv, d), _p(mv, C.c_uint8),
+                       w.shape[0], dt)
+    return w, L

@@ -1,2 +1,112 @@
-/* SPH restatement (filled in with the SPH path). TEST INFRASTRUCTURE ONLY. */
+/*
+ * pb_oracle_sph.c -- CPU restatement of cudaParticleSPH_NS (reference
+ * physics: Lucy kernel, p = c^2 (rho - rho0), physical viscosity).
+ *
+ * TEST INFRASTRUCTURE ONLY: used by tests/ as the checker; the product never
+ * links it.  FP64 throughout; O(N^2) pair search (no cells), so the pair set
+ * is fixed by the mathematics (every j with |r_ij| < h, no periodic image)
+ * and not by any cell bookkeeping.
+ */
 #include "pb_oracle.h"
+#include <math.h>
+
+/* w0 = 105 / (16 pi h^7): cudaParticleSPHBase::setSPHProperties,
+ * cudaParticleSPHBase.cu:119-124 */
+double pbo_sph_w0(double h) {
+  return 105.0 / (16.0 * M_PI * h * h * h * h * h * h * h);
+}
+
+/* SPHKernelLucy, kernelinline.hh:173-189: W = w0 (h + 3r)(h - r)^3, 0 beyond h */
+double pbo_sph_lucy_w(double r, double h, double w0) {
+  double t = h - r;
+  if (0 > t) return 0.0;
+  t = t * t * t;
+  return w0 * (h + 3.0 * r) * t;
+}
+
+/* SPHKernelLucyDW, kernelinline.hh:200-215: (dW/dr)/r = w1 (h - r)^2,
+ * w1 = -12 w0 */
+double pbo_sph_lucy_dw(double r, double h, double w1) {
+  const double t = h - r;
+  if (0 > t) return 0.0;
+  return w1 * t * t;
+}
+
+/* SPHcalcDensity<SPHKernelLucy> (kerneltemplate.hh:636-656) driven over
+ * every j including i itself (the reference functor has no i != j test),
+ * with the non-periodic distance2 (kernelinline.hh:19-29), followed by
+ * calcDensityPost (cudaParticleSPH_NS.cu:184-207):
+ *   num = sum W, rho = sum m_j W, mu = (sum mu0_j m_j W) / rho,
+ *   rhoinv = 1 / rho.
+ * r xyz-interleaved; num_out 3 per particle (num, rho, mu). */
+void pbo_sph_density(const double *r, const double *m, const double *mu0, uint32_t n,
+                     double h, double *num_out, double *rhoinv) {
+  const double w0 = pbo_sph_w0(h);
+#pragma omp parallel for schedule(static)
+  for (uint32_t i = 0; i < n; ++i) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (uint32_t j = 0; j < n; ++j) {
+      const double dx = r[3 * i] - r[3 * j], dy = r[3 * i + 1] - r[3 * j + 1],
+                   dz = r[3 * i + 2] - r[3 * j + 2];
+      const double R = sqrt(dx * dx + dy * dy + dz * dz);
+      if (!(R < h)) continue;
+      const double W = pbo_sph_lucy_w(R, h, w0);
+      s0 += W;
+      s1 += m[j] * W;
+      s2 += mu0[j] * m[j] * W;
+    }
+    num_out[3 * i] = s0;
+    num_out[3 * i + 1] = s1;
+    rhoinv[i] = 1.0 / s1;
+    num_out[3 * i + 2] = s2 * rhoinv[i];
+  }
+}
+
+/* SPHNavierStokes<SPHKernelLucyDW> (kerneltemplate.hh:666-709) over every
+ * j != i, with rho0 = 1.0 as cudaParticleSPH_NS::calcAcceleration sets it
+ * (cudaParticleSPH_NS.cu:91):
+ *   mj  = m_j w1 (h - R)^2
+ *   p1  = c2_i rhoinv_i (1 - rho0 rhoinv_i) + c2_j rhoinv_j (1 - rho0 rhoinv_j)
+ *   v1  = (mu_i + mu_j) rhoinv_i rhoinv_j
+ *   a_i = sum mj ((r_j - r_i) p1 - (v_j - v_i) v1)
+ * mu is the field value (num.z after calcDensityPost).  a xyz-interleaved,
+ * overwritten (the reference clears a first). */
+void pbo_sph_accel(const double *r, const double *v, const double *m, const double *c2,
+                   const double *rhoinv, const double *mu, uint32_t n, double h,
+                   double rho0, double *a) {
+  const double w1 = -12.0 * pbo_sph_w0(h);
+#pragma omp parallel for schedule(static)
+  for (uint32_t i = 0; i < n; ++i) {
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    const double pi_ = c2[i] * rhoinv[i] * (1.0 - rho0 * rhoinv[i]);
+    for (uint32_t j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const double dx = r[3 * j] - r[3 * i], dy = r[3 * j + 1] - r[3 * i + 1],
+                   dz = r[3 * j + 2] - r[3 * i + 2];
+      const double R = sqrt(dx * dx + dy * dy + dz * dz);
+      if (!(R < h)) continue;
+      const double mj = m[j] * pbo_sph_lucy_dw(R, h, w1);
+      const double p1 = pi_ + c2[j] * rhoinv[j] * (1.0 - rho0 * rhoinv[j]);
+      const double v1 = (mu[i] + mu[j]) * rhoinv[i] * rhoinv[j];
+      ax += mj * (dx * p1 - (v[3 * j] - v[3 * i]) * v1);
+      ay += mj * (dy * p1 - (v[3 * j + 1] - v[3 * i + 1]) * v1);
+      az += mj * (dz * p1 - (v[3 * j + 2] - v[3 * i + 2]) * v1);
+    }
+    a[3 * i] = ax;
+    a[3 * i + 1] = ay;
+    a[3 * i + 2] = az;
+  }
+}
+
+/* propagateLeapFrog_F4 (kernelfuncs.cu:270-296), FP64:
+ * v += a dt move ; r += v dt move. */
+void pbo_lf_step(double *r, double *v, const double *a, const uint8_t *move, uint32_t n,
+                 double dt) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const double d = dt * (double)move[i];
+    for (int c = 0; c < 3; ++c) {
+      v[3 * i + c] += a[3 * i + c] * d;
+      r[3 * i + c] += v[3 * i + c] * d;
+    }
+  }
+}

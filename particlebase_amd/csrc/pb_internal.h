@@ -100,8 +100,9 @@ struct pb_sys {
   LJTable *d_lj = nullptr;
 
   // fused path: Verlet list
-  uint32_t *nl = nullptr, *nn = nullptr;
-  uint32_t maxnb = 0;
+  uint16_t *nl = nullptr;          // window-local indices, entry k of slot s at nl[k * npad + s]
+  uint32_t *nn = nullptr;          // list length per slot
+  uint32_t maxnb = 0;              // list capacity
   float4 *rbuild = nullptr;
   double4 *r2buf = nullptr, *v2buf = nullptr;   // ping-pong partners
   uint32_t *orig2 = nullptr;

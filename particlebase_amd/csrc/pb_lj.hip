@@ -28,7 +28,7 @@ __global__ void k_wrap(double4 *r, const uint32_t *orig, const uint32_t *waveRep
                        const RepInfo *reps, uint32_t n);
 
 enum { CNT_LAST = 0, CNT_SKIN = 1, CNT_OVF = 2, CNT_MAXNN = 3, CNT_REC = 4, CNT_DISP = 5,
-       CNT_VMAX = 6 };
+       CNT_VMAX = 6, CNT_WIN = 7 };
 
 // ---------------------------------------------------------------- params
 static void fill_table(pb_sys *s) {
@@ -182,9 +182,11 @@ struct WinTable {
   uint32_t nrun, rloc[4], rpref[5], rg0[4], rcol[4];
   uint32_t wsum[TB / 64];
 };
+#define NL_PAD 0xFFFFu     // list padding entry (maps to an inert slot)
+#define PAD_FAR 1.0e10     // coordinate of that slot
 struct Stage {              // pair kernel: FP64 positions with shifts applied
-  double x[WCAP], y[WCAP], z[WCAP];
-  uint8_t t[WCAP];
+  double x[WCAP + 1], y[WCAP + 1], z[WCAP + 1];   // slot WCAP: padding target
+  uint8_t t[WCAP + 1];
   WinTable W;
 };
 #define WCAPB 4096          // build kernel stages FP32 (16 B) particles
@@ -325,6 +327,10 @@ __device__ void stage_window(Stage &S, const ListSeg &G, const RepInfo &R, const
       S.t[q] = (uint8_t)p.w;
     }
   }
+  if (threadIdx.x == 0) {            // padding target: beyond any cutoff
+    S.x[WCAP] = S.y[WCAP] = S.z[WCAP] = PAD_FAR;
+    S.t[WCAP] = 0;
+  }
   __syncthreads();
 }
 
@@ -332,7 +338,11 @@ __device__ void stage_window(Stage &S, const ListSeg &G, const RepInfo &R, const
 // the window did not fit)
 __device__ __forceinline__ double4 window_pos(const Stage &S, const RepInfo &R, const double4 *r,
                                               bool staged, uint32_t q) {
-  if (staged) return make_double4(S.x[q], S.y[q], S.z[q], (double)S.t[q]);
+  if (staged) {
+    q = q == NL_PAD ? WCAP : q;
+    return make_double4(S.x[q], S.y[q], S.z[q], (double)S.t[q]);
+  }
+  if (q == NL_PAD) return make_double4(PAD_FAR, PAD_FAR, PAD_FAR, 0);
   return window_global(S.W, R, r, q);
 }
 
@@ -352,17 +362,29 @@ __device__ __forceinline__ void tile_particle(const WinTable &W, uint32_t u, uin
 // neighbour cells (all staged particles in whole mode) and keeps
 // window-local q with |r_iq| < rb, where rb = rc + skin + a margin covering
 // the FP32 rounding: the list is a superset of the FP64 pairs within
-// rc + skin.  A window larger than the stage reads FP64 from global memory.
+// rc + skin.  A window larger than the stage reads FP64 from global memory;
+// a window of more than 65534 particles (u16 indices) raises CNT_WIN.
+// Entry k of particle gi is nl[k * npad + gi] (neighbour-major: the lanes of
+// a wave, consecutive particles, store and load adjacent entries).
 __global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint32_t *cstart,
                                                    const ListSeg *segs, const RepInfo *reps,
                                                    uint32_t npad, double rlist2, float rb2,
-                                                   uint32_t maxnb, uint32_t *nl, uint32_t *nn,
+                                                   uint32_t maxnb, uint16_t *nl, uint32_t *nn,
                                                    uint32_t *counter) {
   __shared__ StageB S;
   const ListSeg G = segs[pb_xcd_block()];
   const RepInfo &R = reps[G.rep];
   WinTable &W = S.W;
   window_table(W, G, R, cstart);
+  if (W.total >= NL_PAD) {                       // block-uniform
+    if (threadIdx.x == 0) atomicOr(&counter[CNT_WIN], 1u);
+    for (uint32_t u = threadIdx.x; u < W.rpref[W.nrun]; u += blockDim.x) {
+      uint32_t li, gi, run;
+      tile_particle(W, u, li, gi, run);
+      nn[gi] = 0;                                  // no stale rows survive
+    }
+    return;
+  }
   const bool staged = W.total <= WCAPB;
   // window origin (FP64): lower corner of the first window cell
   const double ox = R.lo[0] + ((double)G.ci - 1.0) * R.L[0] / R.nb[0];
@@ -383,7 +405,7 @@ __global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint3
     tile_particle(W, u, li, gi, run);
     uint32_t cnt = 0;
     auto keep = [&](uint32_t q) {
-      if (cnt < maxnb) nl[(size_t)cnt * npad + gi] = q;
+      if (cnt < maxnb) nl[(size_t)cnt * npad + gi] = (uint16_t)q;
       ++cnt;
     };
     auto scan = [&](uint32_t q0, uint32_t q1, bool mi) {
@@ -442,7 +464,8 @@ __global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint3
 struct StepArgs {
   const double4 *rin, *vin;
   double4 *rout, *vout, *Fout;
-  const uint32_t *nl, *nn;
+  const uint16_t *nl;
+  const uint32_t *nn;
   const float4 *rbuild;
   const uint32_t *cstart;
   const ListSeg *segs;
@@ -483,32 +506,40 @@ __device__ __forceinline__ void publish_prev(const StepArgs &A) {
   __hip_atomic_store(&A.mon[slot + 3], A.step, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-template <bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
-__global__ void __launch_bounds__(TB) k_lj_step(StepArgs A) {
-  __shared__ Stage S;
-  publish_prev(A);
-  const uint32_t seg = pb_xcd_block();
-  const ListSeg G = A.segs[seg];
-  const RepInfo &R = A.reps[G.rep];
-  stage_window(S, G, R, A.cstart, A.rin);
-  const bool staged = S.W.total <= WCAP;
-  const bool mi = G.whole != 0;
+// Pairs and integration of one segment's tile particles.  4 lanes per
+// particle, 16 particles per wavefront: lane sl takes list entries
+// 16 b + 4 j + sl (j < 4) of batch b, so every lane has 4 independent pairs
+// in flight and the next batch's list entries are loaded while the current
+// one is evaluated.  Indices past the end of a row are clamped (no extra
+// HBM lines) and replaced by the padding slot, so with FAST (window staged,
+// no minimum image) a pair is a straight-line sequence: the cutoff is a
+// select.  A butterfly over the 4 lanes sums the partial forces and lane 0
+// of each group integrates.
+template <bool FAST, bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
+__device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, const RepInfo &R,
+                                         bool staged, bool mi, double &ke, double &pe_b,
+                                         double &vir_b, float &d2m, float &v2m) {
   const uint32_t ntile = S.W.rpref[S.W.nrun];
   const double L0 = R.L[0], L1 = R.L[1], L2 = R.L[2];
   const double I0 = R.Linv[0], I1 = R.Linv[1], I2 = R.Linv[2];
-
-  double ke = 0.0, pe_b = 0.0, vir_b = 0.0;
-  float d2m = 0.f, v2m = 0.f;
-  for (uint32_t u = threadIdx.x; u < ntile; u += blockDim.x) {
-    uint32_t li, gi, run;
-    tile_particle(S.W, u, li, gi, run);
-    const double4 pi = window_pos(S, R, A.rin, staged, li);
+  const uint32_t lane = pb_lane(), sl = lane & 3u, grp = lane >> 2;
+  const uint32_t wave = threadIdx.x / 64, nwv = blockDim.x / 64;
+  const uint32_t PAD = FAST ? (uint32_t)WCAP : NL_PAD;
+  const size_t ld = A.npad;
+  for (uint32_t ub = wave * 16; ub < ntile; ub += nwv * 16) {     // wave-uniform
+    const uint32_t u = ub + grp;
+    const bool act = u < ntile;
+    uint32_t li = 0, gi = 0, run;
+    if (act) tile_particle(S.W, u, li, gi, run);
+    const uint32_t nni = act ? A.nn[gi] : 0u;
+    double4 pi;
+    if (FAST) pi = make_double4(S.x[li], S.y[li], S.z[li], (double)S.t[li]);
+    else pi = act ? window_pos(S, R, A.rin, staged, li) : make_double4(0, 0, 0, 0);
     const uint32_t ti = ONE_TYPE ? 0u : (uint32_t)pi.w;
-    const uint32_t nni = A.nn[gi];
     // per-particle state for the integration, requested before the pair loop
     double4 p_own = make_double4(0, 0, 0, 0), v_own = p_own;
     float4 rb_own = make_float4(0, 0, 0, 0);
-    if (INTEGRATE) {
+    if (INTEGRATE && act && sl == 0) {
       p_own = A.rin[gi];       // unshifted FP64 position
       v_own = A.vin[gi];
       rb_own = A.rbuild[gi];
@@ -516,9 +547,12 @@ __global__ void __launch_bounds__(TB) k_lj_step(StepArgs A) {
     // forces and energies accumulate in FP64 in both modes; MIXED evaluates
     // each pair in FP32 on the FP64 displacement after the FP64 cutoff test
     double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
-    auto pair = [&](const double4 &pj) {
+    auto pair = [&](uint32_t q) {
+      double4 pj;
+      if (FAST) pj = make_double4(S.x[q], S.y[q], S.z[q], ONE_TYPE ? 0.0 : (double)S.t[q]);
+      else pj = window_pos(S, R, A.rin, staged, q);
       double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-      if (mi) {
+      if (!FAST && mi) {
         dx -= L0 * rint(dx * I0);
         dy -= L1 * rint(dy * I1);
         dz -= L2 * rint(dz * I2);
@@ -526,61 +560,84 @@ __global__ void __launch_bounds__(TB) k_lj_step(StepArgs A) {
       uint32_t t = 0;
       if (!ONE_TYPE) t = ti * PB_MAXT + (uint32_t)pj.w;
       const double r2d = dx * dx + dy * dy + dz * dz;
-      if (r2d < A.rc2) {
-        if (MIXED) {
-          const float ex = (float)dx, ey = (float)dy, ez = (float)dz, r2 = (float)r2d;
-          const float s2 = ONE_TYPE ? A.s2f : A.lj->s2f[t];
-          const float e24 = ONE_TYPE ? A.e24f : A.lj->e24f[t];
-          float ir2 = __builtin_amdgcn_rcpf(r2);
-          ir2 = ir2 * (2.0f - r2 * ir2);               // one Newton step: ~0.5 ulp
-          float s6 = s2 * ir2;
-          s6 = s6 * s6 * s6;
-          const float fr = e24 * s6 * (2.0f * s6 - 1.0f) * ir2;   // kernelinline.hh:90-95
-          fx += (double)(fr * ex);
-          fy += (double)(fr * ey);
-          fz += (double)(fr * ez);
-          if (ENERGY) {
-            const float e4 = ONE_TYPE ? A.e4f : A.lj->e4f[t];
-            const float sh = ONE_TYPE ? A.shiftf : A.lj->shiftf[t];
-            pe += (double)(e4 * s6 * (s6 - 1.0f) - sh);
-            vir += (double)(fr * r2);
-          }
-        } else {
-          const double s2 = ONE_TYPE ? A.s2 : A.lj->s2[t];
-          const double e24 = ONE_TYPE ? A.e24 : A.lj->e24[t];
-          const double ir2 = rcp64(r2d);
-          double s6 = s2 * ir2;
-          s6 = s6 * s6 * s6;
-          const double fr = e24 * s6 * (2.0 * s6 - 1.0) * ir2;
-          fx += fr * dx;
-          fy += fr * dy;
-          fz += fr * dz;
-          if (ENERGY) {
-            const double e4 = ONE_TYPE ? A.e4 : A.lj->e4[t];
-            const double sh = ONE_TYPE ? A.shift : A.lj->shift[t];
-            pe += e4 * s6 * (s6 - 1.0) - sh;
-            vir += fr * r2d;
-          }
+      const bool in = r2d < A.rc2 && (FAST || !mi || q != NL_PAD);
+      if (MIXED) {
+        const float ex = (float)dx, ey = (float)dy, ez = (float)dz;
+        const float r2 = in ? (float)r2d : 1.0f;
+        const float s2 = ONE_TYPE ? A.s2f : A.lj->s2f[t];
+        const float e24 = ONE_TYPE ? A.e24f : A.lj->e24f[t];
+        float ir2 = __builtin_amdgcn_rcpf(r2);
+        ir2 = ir2 * (2.0f - r2 * ir2);               // one Newton step: ~0.5 ulp
+        float s6 = s2 * ir2;
+        s6 = s6 * s6 * s6;
+        float fr = e24 * s6 * (2.0f * s6 - 1.0f) * ir2;   // kernelinline.hh:90-95
+        fr = in ? fr : 0.0f;
+        fx += (double)(fr * ex);
+        fy += (double)(fr * ey);
+        fz += (double)(fr * ez);
+        if (ENERGY) {
+          const float e4 = ONE_TYPE ? A.e4f : A.lj->e4f[t];
+          const float sh = ONE_TYPE ? A.shiftf : A.lj->shiftf[t];
+          const float ep = e4 * s6 * (s6 - 1.0f) - sh;
+          pe += (double)(in ? ep : 0.0f);
+          vir += (double)(fr * r2);
+        }
+      } else {
+        const double r2 = in ? r2d : 1.0;
+        const double s2 = ONE_TYPE ? A.s2 : A.lj->s2[t];
+        const double e24 = ONE_TYPE ? A.e24 : A.lj->e24[t];
+        const double ir2 = rcp64(r2);
+        double s6 = s2 * ir2;
+        s6 = s6 * s6 * s6;
+        double fr = e24 * s6 * (2.0 * s6 - 1.0) * ir2;
+        fr = in ? fr : 0.0;
+        fx += fr * dx;
+        fy += fr * dy;
+        fz += fr * dz;
+        if (ENERGY) {
+          const double e4 = ONE_TYPE ? A.e4 : A.lj->e4[t];
+          const double sh = ONE_TYPE ? A.shift : A.lj->shift[t];
+          const double ep = e4 * s6 * (s6 - 1.0) - sh;
+          pe += in ? ep : 0.0;
+          vir += fr * r2;
         }
       }
     };
-    // The list is streamed from HBM: the next 8 entries are requested while
-    // the current 8 pairs are evaluated, so each wave keeps its list loads
-    // in flight behind the arithmetic.
-    const uint32_t *nlp = A.nl + gi;
-    const size_t ld = A.npad;
-    uint32_t qa[8], qb[8];
+    const uint16_t *col = A.nl + gi;                 // entry k at col[k * ld]
+    const uint32_t last = nni ? nni - 1u : 0u;
+    const uint32_t nbat = (nni + 15u) >> 4;
+    uint32_t qn[4];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) qa[j] = (uint32_t)j < nni ? nlp[(size_t)j * ld] : 0u;
-    for (uint32_t k = 0; k < nni; k += 8) {
-#pragma unroll
-      for (int j = 0; j < 8; ++j) qb[j] = k + 8 + j < nni ? nlp[(size_t)(k + 8 + j) * ld] : 0u;
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        if (k + j < nni) pair(window_pos(S, R, A.rin, staged, qa[j]));
-#pragma unroll
-      for (int j = 0; j < 8; ++j) qa[j] = qb[j];
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t idx = 4u * j + sl;
+      qn[j] = col[(size_t)min(idx, last) * ld];
     }
+    for (uint32_t b = 0; b < nbat; ++b) {
+      uint32_t q[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t idx = 16u * b + 4u * j + sl;
+        q[j] = idx < nni ? qn[j] : PAD;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t idx = 16u * (b + 1) + 4u * j + sl;
+        qn[j] = col[(size_t)min(idx, last) * ld];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) pair(q[j]);
+    }
+#pragma unroll
+    for (int o = 2; o > 0; o >>= 1) {
+      fx += __shfl_xor(fx, o, 64);
+      fy += __shfl_xor(fy, o, 64);
+      fz += __shfl_xor(fz, o, 64);
+      if (ENERGY) {
+        pe += __shfl_xor(pe, o, 64);
+        vir += __shfl_xor(vir, o, 64);
+      }
+    }
+    if (!act || sl != 0) continue;
 
     if (INTEGRATE) {
       double4 p = p_own;
@@ -618,6 +675,27 @@ __global__ void __launch_bounds__(TB) k_lj_step(StepArgs A) {
       vir_b += 0.5 * vir;
     }
   }
+}
+
+template <bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
+__global__ void __launch_bounds__(TB) k_lj_step(StepArgs A) {
+  __shared__ Stage S;
+  publish_prev(A);
+  const uint32_t seg = pb_xcd_block();
+  const ListSeg G = A.segs[seg];
+  const RepInfo &R = A.reps[G.rep];
+  stage_window(S, G, R, A.cstart, A.rin);
+  const bool staged = S.W.total <= WCAP;
+  const bool mi = G.whole != 0;
+
+  double ke = 0.0, pe_b = 0.0, vir_b = 0.0;
+  float d2m = 0.f, v2m = 0.f;
+  if (staged && !mi)          // block-uniform: the common case, branch-free pairs
+    lj_tiles<true, MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE>(A, S, R, staged, mi, ke, pe_b, vir_b,
+                                                                d2m, v2m);
+  else
+    lj_tiles<false, MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE>(A, S, R, staged, mi, ke, pe_b,
+                                                                 vir_b, d2m, v2m);
 
   __shared__ float smx[2][TB / 64];
   __shared__ double sen[3][TB / 64];
@@ -901,6 +979,8 @@ static int build_checked(pb_sys *s, double rlist) {
     if (int rc = rebuild(s)) return rc;
     uint32_t c[16];
     if (int rc = read_counters(s, c)) return rc;
+    if (c[CNT_WIN])
+      return pb_fail(s->ctx, PB_ERR_CELLS, "a list window holds more than 65534 particles");
     if (c[CNT_OVF] == 0) {
       s->last_maxnn = c[CNT_MAXNN];
       return PB_OK;
@@ -1066,7 +1146,7 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
         last_build = step;
         A.nl = s->nl;
         A.nn = s->nn;
-      }
+            }
       const bool energy = ne && (step % ne == 0);
       const bool last = step + 1 == o->nsteps;
       A.rin = s->r;
@@ -1092,6 +1172,10 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
     if ((rc = read_counters(s, c))) goto done;
     skin_bad = c[CNT_SKIN] ? 1 : 0;
     s->last_maxnn = std::max(s->last_maxnn, c[CNT_MAXNN]);
+    if (c[CNT_WIN]) {
+      rc = pb_fail(s->ctx, PB_ERR_CELLS, "a list window holds more than 65534 particles");
+      goto done;
+    }
     if (c[CNT_OVF]) {
       rc = pb_fail(s->ctx, PB_ERR_NLIST, "neighbour list overflow during the run");
       goto done;

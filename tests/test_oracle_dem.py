@@ -1,0 +1,174 @@
+"""Pins of the DEM oracle (cudaParticleDEM + Rotation) by geometry and
+mechanics: contact sets against an independent numpy brute force, the
+Hertz law's scaling, damping direction, restitution below 1, the Coulomb
+cap of the tangential force, torque = Rij x Fs, Newton's third law for equal
+spheres, history carry-over on a scripted contact sequence and the
+rotation update."""
+import numpy as np
+import pytest
+
+E, MU, SIG, MU_R = 2.0e5, 0.4, 0.29, 0.1
+S2 = (2 - SIG) / 2 / (1 - SIG)
+
+
+def gamma2(e):
+    return -np.log(e) / np.sqrt(np.pi ** 2 + np.log(e) ** 2) * 2
+
+
+def random_packing(n, seed, box=6.0, rmin=0.4, rmax=0.6):
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(0, box, (n, 3))
+    rad = rng.uniform(rmin, rmax, n)
+    move = (rng.uniform(size=n) > 0.2).astype(np.uint8)
+    return r, rad, move
+
+
+def test_contacts_brute_force(orc):
+    r, rad, move = random_packing(300, 1)
+    row, col = orc.dem_contacts(r, rad, move)
+    d2 = ((r[:, None, :] - r[None, :, :]) ** 2).sum(-1)
+    hit = d2 < (rad[:, None] + rad[None, :]) ** 2
+    np.fill_diagonal(hit, False)
+    hit &= move[:, None].astype(bool)
+    ref_row = np.concatenate([[0], np.cumsum(hit.sum(1))])
+    assert np.array_equal(row, ref_row)
+    assert np.array_equal(col, np.nonzero(hit)[1])       # ascending j per row
+    assert row[-1] > 50
+
+
+def two_spheres(xi, r1=0.5, r2=0.5, v=(0, 0, 0), w1=(0, 0, 0), w2=(0, 0, 0), move2=1,
+                d=(1.0, 0.0, 0.0)):
+    d = np.asarray(d) / np.linalg.norm(d)
+    r = np.array([[0.0, 0, 0], list((r1 + r2 - xi) * d)])
+    vel = np.array([[0.0, 0, 0], list(v)])
+    w = np.array([list(w1), list(w2)], dtype=float)
+    rad = np.array([r1, r2])
+    move = np.array([1, move2], dtype=np.uint8)
+    return r, vel, w, rad, move
+
+
+def force(orc, r, v, w, rad, move, val=None, minv=(1.0, 1.0), g2=0.0, dt=1e-4):
+    row, col = orc.dem_contacts(r, rad, move)
+    if val is None:
+        val = np.zeros(len(col))
+    F, T, val2, bad = orc.dem_force(r, v, w, rad, np.array(minv), move, row, col, val,
+                                    E, MU, S2, g2, MU_R, dt)
+    return F, T, val2, bad
+
+
+def test_hertz_scaling(orc):
+    # static normal force ~ xi^(3/2) sqrt(Rbar), repulsive along -x on sphere 0
+    f = []
+    for xi in (1e-3, 2e-3, 4e-3):
+        F, T, _, _ = force(orc, *two_spheres(xi))
+        assert F[0, 0] < 0 and np.allclose(F[0, 1:], 0) and np.allclose(T, 0)
+        f.append(-F[0, 0])
+    assert f[1] / f[0] == pytest.approx(2 ** 1.5, rel=1e-12)
+    assert f[2] / f[1] == pytest.approx(2 ** 1.5, rel=1e-12)
+    Fa, _, _, _ = force(orc, *two_spheres(2e-3, 0.5, 0.5))
+    Fb, _, _, _ = force(orc, *two_spheres(2e-3, 1.0, 1.0))      # Rbar doubles
+    assert Fb[0, 0] / Fa[0, 0] == pytest.approx(np.sqrt(2), rel=1e-12)
+    # absolute value: 2/3 E sqrt(Rbar xi) xi with Rbar = r1 r2/(r1+r2)
+    xi, rb = 2e-3, 0.25
+    assert -Fa[0, 0] == pytest.approx(2 / 3 * E * np.sqrt(rb * xi) * xi, rel=1e-12)
+    # a fixed partner doubles the force (Q = 2 - move_j)
+    Fw, _, _, _ = force(orc, *two_spheres(2e-3, move2=0))
+    assert Fw[0, 0] == pytest.approx(2 * Fa[0, 0], rel=1e-14)
+    assert np.all(Fw[1] == 0)                       # fixed rows are empty
+
+
+def test_damping_direction_and_restitution(orc):
+    g2 = gamma2(0.8)
+    Fs, _, _, _ = force(orc, *two_spheres(2e-3), g2=g2)
+    Fc, _, _, _ = force(orc, *two_spheres(2e-3, v=(-0.1, 0, 0)), g2=g2)   # approaching
+    Fo, _, _, _ = force(orc, *two_spheres(2e-3, v=(0.1, 0, 0)), g2=g2)    # separating
+    assert -Fc[0, 0] > -Fs[0, 0] > -Fo[0, 0]
+    # head-on collision integrated with the oracle forces: 0 < e < 1,
+    # smaller for stronger damping
+    def collide(g):
+        x = np.array([[0.0, 0, 0], [1.0 + 1e-9, 0, 0]])
+        v = np.array([[0.5, 0, 0], [-0.5, 0, 0]])
+        rad, move, w = np.array([0.5, 0.5]), np.array([1, 1], np.uint8), np.zeros((2, 3))
+        dt = 2e-6
+        for _ in range(200000):
+            row, col = orc.dem_contacts(x, rad, move)
+            F, _, _, _ = orc.dem_force(x, v, w, rad, np.ones(2), move, row, col,
+                                       np.zeros(len(col)), E, MU, S2, g, MU_R, dt)
+            v = v + F * dt
+            x = x + v * dt
+            if x[1, 0] - x[0, 0] > 1.0 and v[1, 0] > v[0, 0]:
+                break
+        return (v[1, 0] - v[0, 0]) / 1.0
+    e1, e2 = collide(gamma2(0.9)), collide(gamma2(0.6))
+    assert 0 < e2 < e1 < 1.0
+    assert collide(0.0) == pytest.approx(1.0, abs=1e-3)
+
+
+def test_tangential_cap_torque_and_history(orc):
+    # oblique contact: the tangential slip has three non-zero components (a
+    # slip with a zero component gets no tangential force, as in the
+    # reference functor, kernelinline.hh:296)
+    d = np.array([1.0, 2.0, 2.0]) / 3.0
+    base = two_spheres(2e-3, v=(0.0, 0.3, 0.2), w1=(0.1, 0.2, 0.3), d=d)
+    r, v, w, rad, move = base
+    F0, T0, val1, _ = force(orc, r, v, w, rad, move, val=np.zeros(2), dt=1e-3)
+    fn = 2 / 3 * E * np.sqrt(0.25 * 2e-3) * 2e-3
+    Fn = -fn
+    # zero history: no tangential force, only the rolling friction torque
+    assert np.allclose(F0[0], -fn * d, rtol=1e-12, atol=1e-12 * fn)
+    wn = w[0] / np.linalg.norm(w[0])
+    assert np.allclose(T0[0], -MU_R * rad[0] * abs(Fn) * wn, rtol=1e-12)
+    # the history grew by |vt| dt (same for both rows here)
+    assert np.all(val1 > 0)
+    # history beyond xi_s_max: |Fs| = mu F0 exactly (Coulomb cap)
+    big = np.full(2, 1.0)
+    F1, T1, _, _ = force(orc, r, v, w, rad, move, val=big, dt=1e-3)
+    Fs = F1[0] - F0[0]
+    assert np.linalg.norm(Fs) == pytest.approx(MU * abs(Fn), rel=1e-12)
+    assert abs(np.dot(Fs, d)) < 1e-12 * abs(Fn)                # tangential
+    # torque of the tangential force: Rij x Fs, Rij = r_i/(r_i+r_j) (r_j - r_i)
+    Rij = rad[0] / rad.sum() * (r[1] - r[0])
+    assert np.allclose(T1[0] - T0[0], np.cross(Rij, Fs), rtol=1e-12, atol=1e-12)
+    # half the cap: linear in the history
+    xsm = MU * S2 * 2e-3
+    F2, _, _, _ = force(orc, r, v, w, rad, move, val=np.full(2, 0.5 * xsm), dt=1e-3)
+    assert np.linalg.norm(F2[0] - F0[0]) == pytest.approx(0.5 * MU * abs(Fn), rel=1e-12)
+
+
+def test_newton_third_law_equal_spheres(orc):
+    r, v, w, rad, move = two_spheres(3e-3, v=(0.01, 0.3, -0.2), w1=(0.2, -0.1, 0.4),
+                                     w2=(-0.3, 0.2, 0.1), d=(2.0, -1.0, 2.0))
+    w[:] = 0.0        # rolling friction is not a pair force
+    F, T, _, _ = force(orc, r, v, w, rad, move, val=np.full(2, 1e-4), g2=gamma2(0.8))
+    assert np.allclose(F[0], -F[1], rtol=1e-12, atol=1e-12)
+
+
+def test_history_carry_scripted(orc):
+    # step 1: (0,1), (0,2); step 2: (0,2), (0,3), (1,0)
+    prow = np.array([0, 2, 2, 2, 2], np.uint32)
+    pcol = np.array([1, 2], np.uint32)
+    pval = np.array([0.5, 0.25])
+    crow = np.array([0, 2, 3, 3, 3], np.uint32)
+    ccol = np.array([2, 3, 0], np.uint32)
+    cval = orc.dem_carry(prow, pcol, pval, crow, ccol)
+    assert np.array_equal(cval, [0.25, 0.0, 0.0])
+    # step 3: contact (0,2) persists again
+    cval3 = orc.dem_carry(crow, ccol, cval, np.array([0, 1, 1, 1, 1], np.uint32),
+                          np.array([2], np.uint32))
+    assert np.array_equal(cval3, [0.25])
+
+
+def test_rotation_update(orc):
+    rng = np.random.default_rng(4)
+    n = 20
+    T = rng.normal(size=(n, 3))
+    m = rng.uniform(1, 2, n)
+    rad = rng.uniform(0.4, 0.6, n)
+    move = rng.integers(0, 2, n).astype(np.uint8)
+    m0inv = (1 / m) / (0.4 * rad ** 2)
+    dt = 0.01
+    w, L = orc.rot_step(np.zeros((n, 3)), np.zeros((n, 3)), T, m0inv, move, dt)
+    mv = move[:, None]
+    assert np.allclose(L, T * dt * mv, rtol=1e-15)
+    # omega = L / I with the solid-sphere inertia I = 2/5 m r^2
+    assert np.allclose(w, L / (0.4 * m * rad ** 2)[:, None] * mv, rtol=1e-14)
