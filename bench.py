@@ -118,12 +118,27 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--quick", action="store_true", help="small sizes (debug)")
+    ap.add_argument("--profile", action="store_true",
+                    help="only the mixed 1M run (for rocprofv3 kernel traces / counters)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only a device copy of 256 MiB, 5 times (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {a.gpus} is not supported "
               f"(the multi-GPU decomposition is a design, not part of this build)",
               file=sys.stderr)
         sys.exit(2)
+
+    if a.copy_probe:
+        import torch
+        x = torch.empty(32 << 20, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        x.fill_(1.0)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        log("copy probe: 5 x 256 MiB read + 256 MiB written")
+        return
 
     ctx = pb.Context(0)
     name, cus = ctx.info()
@@ -152,6 +167,8 @@ def main():
     frms = float(np.sqrt((f64 ** 2).sum(axis=1).mean()))
     ferr = float(np.abs(f32 - f64).max() / frms)
     del s
+    if a.profile:
+        return
 
     # ---- plain FP64 form
     s = make_system(ctx, w)

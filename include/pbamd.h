@@ -31,6 +31,8 @@ extern "C" {
 #define PB_ERR_NLIST       -5   /* neighbour list overflowed its capacity */
 #define PB_ERR_SKIN        -6   /* a particle moved more than skin/2 between
                                    list rebuilds: pairs may have been missed */
+#define PB_ERR_DT          -7   /* DEM time step above the Cundall-Strack bound
+                                   (the reference asserts, kernelinline.hh:285-286) */
 
 /* precision modes of the pair kernels */
 #define PB_FP64            0    /* pair math, integration, reductions in FP64 */
@@ -49,6 +51,8 @@ extern "C" {
 #define PB_FIELD_L         8    /* double[3n] angular momentum (:24) */
 #define PB_FIELD_T         9    /* double[3n] torque           (:26) */
 #define PB_FIELD_RADIUS   10    /* double[n]  radius (DEM r.w, cudaParticleDEM.cu:221-252) */
+#define PB_FIELD_SPH_NUM  11    /* double[3n] number density, rho, mu (num, cudaParticleSPHBase.hh:24) */
+#define PB_FIELD_SPH_RHOINV 12  /* double[n]  1/rho (rhoinv, cudaParticleSPHBase.hh:27) (get only) */
 
 typedef struct pb_ctx pb_ctx;
 typedef struct pb_sys pb_sys;
@@ -202,6 +206,73 @@ int pb_lj_last_run_timing(pb_sys *sys, double *step_ms, uint32_t *nstep,
  * overflow). */
 int pb_gr_hist(pb_sys *sys, const float *rstep, uint32_t rnum,
                uint64_t *hist);
+
+/* ------------------------------------------------------------------- SPH */
+/* Reference physics of cudaParticleSPH_NS: Lucy kernel of support h,
+ * p = c^2 (rho - rho0) with rho0 = 1, physical viscosity mu; non-periodic
+ * distances.  Single replica.  Cells: pb_setup_cells(sys, h, 0, 0) once and
+ * pb_bin(sys) before every pb_sph_density (calcBlockID). */
+
+/* cudaParticleSPH_NS::setSPHProperties (cudaParticleSPH_NS.cu:56-66) +
+ * cudaParticleSPHBase::setSPHProperties (cudaParticleSPHBase.cu:119-124):
+ * mu0[n] viscosity, c[n] sound speed (c^2 is kept), kernel radius h;
+ * w0 = 105 / (16 pi h^7). */
+int pb_sph_set_properties(pb_sys *sys, const double *mu0, const double *c, double h);
+
+/* cudaParticleSPH_NS::calcDensity + calcDensityPost (cudaParticleSPH_NS.cu:
+ * 136-207; SPHcalcDensity<SPHKernelLucy>, kerneltemplate.hh:636-656):
+ * number density, rho = sum m_j W (self included), 1/rho and the field mu;
+ * read back with PB_FIELD_SPH_NUM / PB_FIELD_SPH_RHOINV. */
+int pb_sph_density(pb_sys *sys);
+
+/* cudaParticleSPH_NS::calcAcceleration (cudaParticleSPH_NS.cu:71-134;
+ * SPHNavierStokes<SPHKernelLucyDW>, kerneltemplate.hh:666-709): a is
+ * overwritten with the pressure + viscous acceleration of the last
+ * pb_sph_density's rho and mu at the current positions and velocities. */
+int pb_sph_acceleration(pb_sys *sys);
+
+/* cudaParticleSPH_NS::inspectDensity (cudaParticleSPH_NS.cu:209-217,
+ * inspectDense_x kernelfuncs.cu:772-801): mean number density of the
+ * movable particles. */
+int pb_sph_inspect_density(pb_sys *sys, double *mean_num);
+
+/* ------------------------------------------------------------------- DEM */
+/* cudaParticleDEM::setDEMProperties, both overloads (cudaParticleDEM.cu:
+ * 221-252): E_eff = E (1 - sigma^2), s2 = (2 - sigma)/2/(1 - sigma), gamma
+ * (the reference's gamma2 = 2 gamma), mu, mu_r; radius r0[n] (original
+ * order) or r0_all when r0 is NULL.  sigma must be < 0.5.  Allocates the
+ * rotational state (w, L, T). */
+int pb_dem_set_properties(pb_sys *sys, double E, double mu, double sigma, double gamma,
+                          double mu_r, const double *r0, double r0_all);
+
+/* cudaParticleRotation::setInertia, both overloads (cudaParticleRotation.cu:
+ * 35-61): w = L = 0, m0inv = minv / (0.4 r0^2). */
+int pb_rot_set_inertia(pb_sys *sys, const double *r0, double r0_all);
+
+/* cudaParticleRotation::TimeEvolution (cudaParticleRotation.cu:29-33,
+ * propagateEulerianRotation_F4 kernelfuncs.cu:966-986):
+ * L += T dt move, w = L m0inv move. */
+int pb_rot_step(pb_sys *sys, double dt);
+
+/* cudaParticleDEM::calcForce(dt) (cudaParticleDEM.cu:29-213): contact CSR
+ * of the movable particles (|r_ij| < r_i + r_j, rows and columns by
+ * original index, columns ascending), tangential history carried over from
+ * the previous call's matrix, Hertz contact force and torque into F and T
+ * (overwritten).  Needs pb_bin (calcBlockID) on the current positions with
+ * cells from pb_setup_cells(sys, rmax, 0, single).  autotune = 0 makes a
+ * step above the Cundall-Strack bound an error (PB_ERR_DT; the reference
+ * asserts unless autotunetimestep).  nnz_out (may be NULL) gets nnz. */
+int pb_dem_calc_force(pb_sys *sys, double dt, int autotune, uint32_t *nnz_out);
+
+/* Download of the latest contact matrix (cudaSparseMat rowPtr / colIdx /
+ * val, cudaSparseMat.hh): rowptr[n+1]; colidx and val need cap >= nnz
+ * (either may be NULL). */
+int pb_dem_get_contacts(pb_sys *sys, uint32_t *rowptr, uint32_t *colidx, double *val,
+                        uint32_t cap, uint32_t *nnz);
+
+/* Contacts of the last pb_dem_calc_force whose time step exceeded the
+ * Cundall-Strack bound (counted even when autotune allowed the step). */
+int pb_dem_last_violations(pb_sys *sys, uint32_t *count);
 
 #ifdef __cplusplus
 }

@@ -25,12 +25,13 @@ PB_ERR_STATE = -3
 PB_ERR_CELLS = -4
 PB_ERR_NLIST = -5
 PB_ERR_SKIN = -6
+PB_ERR_DT = -7
 
 PB_FP64 = 0
 PB_MIXED = 1
 
 FIELD = {"r": 0, "v": 1, "a": 2, "F": 3, "minv": 4, "type": 5, "move": 6,
-         "w": 7, "L": 8, "T": 9, "radius": 10}
+         "w": 7, "L": 8, "T": 9, "radius": 10, "sph_num": 11, "sph_rhoinv": 12}
 
 # symbols declared in include/pbamd.h (checked by the CPU test suite)
 EXPORTS = [
@@ -41,7 +42,11 @@ EXPORTS = [
     "pb_treat_reflect", "pb_vv_step", "pb_lf_vinit", "pb_lf_step", "pb_lf_rollback", "pb_mv2",
     "pb_scale_velocities", "pb_inspect_velocity", "pb_setup_cells", "pb_get_cell_info",
     "pb_bin", "pb_get_bins", "pb_get_neighbor_table", "pb_lj_set_params", "pb_lj_set_cutoff",
-    "pb_lj_force", "pb_coord_counts", "pb_lj_run", "pb_lj_last_run_stats", "pb_lj_last_run_timing", "pb_gr_hist",
+    "pb_lj_force", "pb_coord_counts", "pb_lj_run", "pb_lj_last_run_stats",
+    "pb_lj_last_run_timing", "pb_gr_hist",
+    "pb_sph_set_properties", "pb_sph_density", "pb_sph_acceleration", "pb_sph_inspect_density",
+    "pb_dem_set_properties", "pb_rot_set_inertia", "pb_rot_step", "pb_dem_calc_force",
+    "pb_dem_get_contacts", "pb_dem_last_violations",
 ]
 
 
@@ -119,30 +124,28 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         fn.restype = res
         fn.argtypes = args
     for name, (res, args) in _extra_signatures().items():
-        if hasattr(lib, name):
-            fn = getattr(lib, name)
-            fn.restype = res
-            fn.argtypes = args
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
     _lib = lib
     return lib
 
 
 def _extra_signatures():
-    """Signatures of the SPH / DEM / batch entry points (added per path)."""
+    """Signatures of the SPH / DEM entry points."""
     vp, u32, i32, d = C.c_void_p, C.c_uint32, C.c_int, C.c_double
     P = C.POINTER
     return {
-        "pb_sph_set_properties": (i32, [vp, P(d), P(d), d, i32]),
+        "pb_sph_set_properties": (i32, [vp, P(d), P(d), d]),
         "pb_sph_density": (i32, [vp]),
         "pb_sph_acceleration": (i32, [vp]),
-        "pb_sph_get": (i32, [vp, i32, P(d)]),
-        "pb_dem_set_properties": (i32, [vp, d, d, d, d, d]),
-        "pb_dem_setup_cells": (i32, [vp, d]),
-        "pb_dem_contacts": (i32, [vp, P(u32)]),
-        "pb_dem_get_contacts": (i32, [vp, P(u32), P(u32), P(d)]),
-        "pb_dem_force": (i32, [vp, d]),
-        "pb_rotation_step": (i32, [vp, d]),
-        "pb_set_inertia": (i32, [vp]),
+        "pb_sph_inspect_density": (i32, [vp, P(d)]),
+        "pb_dem_set_properties": (i32, [vp, d, d, d, d, d, P(d), d]),
+        "pb_rot_set_inertia": (i32, [vp, P(d), d]),
+        "pb_rot_step": (i32, [vp, d]),
+        "pb_dem_calc_force": (i32, [vp, d, i32, P(u32)]),
+        "pb_dem_get_contacts": (i32, [vp, P(u32), P(u32), P(d), u32, P(u32)]),
+        "pb_dem_last_violations": (i32, [vp, P(u32)]),
     }
 
 
@@ -258,9 +261,9 @@ class System:
                                        _p(t, C.c_uint32), _p(fx, C.c_uint8)))
 
     def get(self, field: str):
-        if field in ("r", "v", "a", "F", "w", "L", "T"):
+        if field in ("r", "v", "a", "F", "w", "L", "T", "sph_num"):
             out = np.empty((self.n, 3), dtype=np.float64)
-        elif field in ("minv", "radius"):
+        elif field in ("minv", "radius", "sph_rhoinv"):
             out = np.empty(self.n, dtype=np.float64)
         elif field == "type":
             out = np.empty(self.n, dtype=np.uint16)
@@ -402,6 +405,65 @@ class System:
                                                C.byref(nr)))
         return {"step_ms": sm.value, "nstep": ns.value, "rebuild_ms": rm.value,
                 "nrebuild": nr.value}
+
+    # --- SPH (cudaParticleSPH_NS)
+    def sph_set_properties(self, mu0, c, h):
+        mu0 = np.ascontiguousarray(np.broadcast_to(np.asarray(mu0, dtype=np.float64), (self.n,)))
+        c = np.ascontiguousarray(np.broadcast_to(np.asarray(c, dtype=np.float64), (self.n,)))
+        self._c(self.lib.pb_sph_set_properties(self.h, _p(mu0, C.c_double), _p(c, C.c_double),
+                                               float(h)))
+
+    def sph_density(self):
+        self._c(self.lib.pb_sph_density(self.h))
+
+    def sph_acceleration(self):
+        self._c(self.lib.pb_sph_acceleration(self.h))
+
+    def sph_inspect_density(self):
+        x = C.c_double()
+        self._c(self.lib.pb_sph_inspect_density(self.h, C.byref(x)))
+        return x.value
+
+    # --- DEM (cudaParticleDEM / cudaParticleRotation)
+    def dem_set_properties(self, E, mu, sigma, gamma, mu_r, r0):
+        if np.ndim(r0) == 0:
+            self._c(self.lib.pb_dem_set_properties(self.h, E, mu, sigma, gamma, mu_r, None,
+                                                   float(r0)))
+        else:
+            r = np.ascontiguousarray(r0, dtype=np.float64)
+            self._c(self.lib.pb_dem_set_properties(self.h, E, mu, sigma, gamma, mu_r,
+                                                   _p(r, C.c_double), 0.0))
+
+    def rot_set_inertia(self, r0):
+        if np.ndim(r0) == 0:
+            self._c(self.lib.pb_rot_set_inertia(self.h, None, float(r0)))
+        else:
+            r = np.ascontiguousarray(r0, dtype=np.float64)
+            self._c(self.lib.pb_rot_set_inertia(self.h, _p(r, C.c_double), 0.0))
+
+    def rot_step(self, dt):
+        self._c(self.lib.pb_rot_step(self.h, dt))
+
+    def dem_calc_force(self, dt, autotune=False):
+        z = C.c_uint32()
+        self._c(self.lib.pb_dem_calc_force(self.h, dt, int(autotune), C.byref(z)))
+        return z.value
+
+    def dem_contacts(self):
+        z = C.c_uint32()
+        row = np.zeros(self.n + 1, dtype=np.uint32)
+        self._c(self.lib.pb_dem_get_contacts(self.h, _p(row, C.c_uint32), None, None, 0,
+                                             C.byref(z)))
+        col = np.zeros(max(z.value, 1), dtype=np.uint32)
+        val = np.zeros(max(z.value, 1), dtype=np.float64)
+        self._c(self.lib.pb_dem_get_contacts(self.h, _p(row, C.c_uint32), _p(col, C.c_uint32),
+                                             _p(val, C.c_double), col.size, C.byref(z)))
+        return row, col[:z.value], val[:z.value]
+
+    def dem_violations(self):
+        z = C.c_uint32()
+        self._c(self.lib.pb_dem_last_violations(self.h, C.byref(z)))
+        return z.value
 
     # --- g(r)
     def gr_hist(self, rstep, rnum):

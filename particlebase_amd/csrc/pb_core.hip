@@ -166,6 +166,12 @@ extern "C" void pb_sys_destroy(pb_sys *s) {
   pb_free(s->cstart); pb_free(s->scan_tmp); pb_free(s->d_lj);
   pb_free(s->nl); pb_free(s->nn); pb_free(s->rbuild); pb_free(s->r2buf); pb_free(s->v2buf);
   pb_free(s->orig2); pb_free(s->type2); pb_free(s->move2); pb_free(s->a2buf);
+  pb_free(s->sph_mu0); pb_free(s->sph_c2); pb_free(s->sph_rhoinv); pb_free(s->sph_num);
+  pb_free(s->sph_rs); pb_free(s->sph_vs); pb_free(s->sph_ds); pb_free(s->m0inv);
+  pb_free(s->dem_inv);
+  for (int b = 0; b < 2; ++b) {
+    pb_free(s->cm_row[b]); pb_free(s->cm_col[b]); pb_free(s->cm_val[b]);
+  }
   pb_free(s->gr_hist);
   pb_free(s->list_segs);
   if (s->hmon) (void)hipHostFree(s->hmon);
@@ -200,7 +206,7 @@ extern "C" int pb_sys_set_cell(pb_sys *s, const double *cells) {
 }
 
 // host-side packing: original order <-> storage order
-static std::vector<uint32_t> download_orig(pb_sys *s, int *rc) {
+std::vector<uint32_t> pb_download_orig(pb_sys *s, int *rc) {
   std::vector<uint32_t> o(s->npad);
   *rc = PB_OK;
   if (hipMemcpyAsync(o.data(), s->orig, 4 * (size_t)s->npad, hipMemcpyDeviceToHost,
@@ -215,7 +221,7 @@ extern "C" int pb_sys_import(pb_sys *s, const double *r, const double *v, const 
   if (!s || !r || !m) return pb_fail(s ? s->ctx : nullptr, PB_ERR_ARG, "pb_sys_import: r and m required");
   PB_HIP(s->ctx, hipSetDevice(s->ctx->device));
   int rc = PB_OK;
-  std::vector<uint32_t> o = download_orig(s, &rc);
+  std::vector<uint32_t> o = pb_download_orig(s, &rc);
   if (rc) return rc;
   const size_t np = s->npad;
   std::vector<double4> hr(np), hv(np), ha(np);
@@ -262,6 +268,7 @@ static double4 *vec_field(pb_sys *s, int field) {
     case PB_FIELD_W: return s->w;
     case PB_FIELD_L: return s->Lm;
     case PB_FIELD_T: return s->T;
+    case PB_FIELD_SPH_NUM: return s->sph_num;
     default: return nullptr;
   }
 }
@@ -290,7 +297,7 @@ extern "C" int pb_sys_get(pb_sys *s, int field, void *host) {
   if (!s || !host) return PB_ERR_ARG;
   PB_HIP(s->ctx, hipSetDevice(s->ctx->device));
   int rc = PB_OK;
-  std::vector<uint32_t> o = download_orig(s, &rc);
+  std::vector<uint32_t> o = pb_download_orig(s, &rc);
   if (rc) return rc;
   const size_t np = s->npad;
   hipStream_t st = s->ctx->stream;
@@ -309,6 +316,18 @@ extern "C" int pb_sys_get(pb_sys *s, int field, void *host) {
   }
   if (field == PB_FIELD_W || field == PB_FIELD_L || field == PB_FIELD_T)
     return pb_fail(s->ctx, PB_ERR_STATE, "pb_sys_get: rotation state not allocated");
+  if (field == PB_FIELD_SPH_NUM || field == PB_FIELD_SPH_RHOINV) {
+    if (!s->sph_num) return pb_fail(s->ctx, PB_ERR_STATE, "pb_sys_get: no SPH density yet");
+    if (field == PB_FIELD_SPH_RHOINV) {
+      std::vector<double> h(np);
+      PB_HIP(s->ctx, hipMemcpyAsync(h.data(), s->sph_rhoinv, 8 * np, hipMemcpyDeviceToHost, st));
+      PB_HIP(s->ctx, hipStreamSynchronize(st));
+      double *out = (double *)host;
+      for (size_t k = 0; k < np; ++k)
+        if (o[k] != PB_NONE) out[o[k]] = h[k];
+      return PB_OK;
+    }
+  }
   if (field == PB_FIELD_MINV) {
     std::vector<double4> h(np);
     PB_HIP(s->ctx, hipMemcpyAsync(h.data(), s->v, sizeof(double4) * np, hipMemcpyDeviceToHost, st));
@@ -356,7 +375,7 @@ extern "C" int pb_sys_set(pb_sys *s, int field, const void *host) {
   if ((field == PB_FIELD_W || field == PB_FIELD_L || field == PB_FIELD_T) && (rc = pb_ensure_rotation(s)))
     return rc;
   if (field == PB_FIELD_RADIUS && (rc = pb_ensure_radius(s))) return rc;
-  std::vector<uint32_t> o = download_orig(s, &rc);
+  std::vector<uint32_t> o = pb_download_orig(s, &rc);
   if (rc) return rc;
   const size_t np = s->npad;
   hipStream_t st = s->ctx->stream;

@@ -125,6 +125,25 @@ struct pb_sys {
   // g(r)
   uint64_t *gr_hist = nullptr;
   size_t gr_cap = 0;
+
+  // SPH (cudaParticleSPH_NS), storage order unless noted
+  double *sph_mu0 = nullptr, *sph_c2 = nullptr, *sph_rhoinv = nullptr;
+  double4 *sph_num = nullptr;                  // (number density, rho, mu, 0)
+  double4 *sph_rs = nullptr, *sph_vs = nullptr, *sph_ds = nullptr;   // cell-sorted views
+  double sph_h = 0.0, sph_rho0 = 1.0;
+  bool sph_set = false, sph_dens = false;
+
+  // DEM (cudaParticleDEM, cudaParticleRotation)
+  double *m0inv = nullptr;                     // minv / (0.4 r0^2), storage order
+  double dem_E = 0, dem_mu = 0, dem_s2 = 0, dem_gamma2 = 0, dem_mu_r = 0;
+  bool dem_set = false;
+  // contact CSR by original index, double-buffered for the tangential history
+  uint32_t *cm_row[2] = {nullptr, nullptr}, *cm_col[2] = {nullptr, nullptr};
+  double *cm_val[2] = {nullptr, nullptr};
+  uint32_t cm_cap[2] = {0, 0}, cm_nnz[2] = {0, 0};
+  int cm_cur = -1;                             // buffer of the latest CSR, -1 before any
+  uint32_t *dem_inv = nullptr;                 // original index -> storage slot
+  uint32_t dem_bad = 0;                        // Cundall-Strack violations of the last call
 };
 
 // --------------------------------------------------------------- host util
@@ -171,6 +190,11 @@ int pb_exclusive_scan(pb_sys *sys, const uint32_t *in, uint32_t *out, uint32_t n
 int pb_counting_sort(pb_sys *sys, uint32_t ncell, const uint32_t *key);
 // per-replica reduction of K per-wave partial components into out[rep*8+c]
 int pb_reduce_waves(pb_sys *sys, int K);
+// storage slot -> original index (PB_NONE for padding), downloaded
+std::vector<uint32_t> pb_download_orig(pb_sys *sys, int *rc);
+// rotation / radius state, allocated on first use
+int pb_ensure_rotation(pb_sys *sys);
+int pb_ensure_radius(pb_sys *sys);
 
 // ------------------------------------------------------------ device util
 __device__ __forceinline__ uint32_t pb_lane() { return __lane_id(); }

@@ -141,3 +141,100 @@ def lj_replicas(nrep: int = 256, nmin: int = 500, nmax: int = 20000, seed: int =
         nc = int(math.ceil((n / 4.0) ** (1.0 / 3.0) - 1e-12))
         out.append(lj_fcc(nc, rho, T, seed=seed + k, natoms=n))
     return Batch(out)
+
+
+# ------------------------------------------------------------------ SPH
+@dataclass
+class SPHSystem:
+    r: np.ndarray        # (n, 3)
+    m: np.ndarray
+    fixed: np.ndarray    # uint8, 1 = wall particle
+    mu0: np.ndarray
+    c: np.ndarray        # sound speed
+    h: float
+    cell: np.ndarray     # (6,) box
+    dx: float
+
+    @property
+    def n(self):
+        return self.r.shape[0]
+
+
+def sph_dam_break(dx: float = 0.4 / 63, box=(1.6, 0.4, 0.8), column=(0.4, 0.4, 0.8),
+                  layers: int = 3, hfac: float = 1.7, g: float = 9.81, seed: int = 3,
+                  jitter: float = 0.0) -> SPHSystem:
+    """3-D dam break in the reference's SPH units: a fluid column on a cubic
+    lattice of spacing dx in the corner of a box lined with `layers` layers
+    of fixed wall particles.  Density is relative (rho0 = 1, the constant
+    cudaParticleSPH_NS.cu:91 uses), so every particle has mass dx^3; the
+    kernel radius is h = hfac dx (Lucy kernel, support h); the sound speed
+    is c0 = 10 sqrt(2 g H) (ten times the largest free-fall speed).  With
+    dx = 0.4/63 the column holds 63 x 63 x 126 = 500,094 fluid particles.
+    `jitter` (in units of dx) displaces fluid particles by uniforms."""
+    nx, ny, nz = (int(round(column[a] / dx)) for a in range(3))
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    fluid = (np.stack([ix.ravel(), iy.ravel(), iz.ravel()], 1) + 0.5) * dx
+    if jitter > 0:
+        fluid += (uniform53(seed, fluid.size).reshape(-1, 3) - 0.5) * (jitter * dx)
+    # walls: floor and four side walls, `layers` deep, outside the box
+    wx, wy, wz = (int(round(box[a] / dx)) for a in range(3))
+    gx = np.arange(-layers, wx + layers)
+    gy = np.arange(-layers, wy + layers)
+    gz = np.arange(-layers, wz)
+    WX, WY, WZ = np.meshgrid(gx, gy, gz, indexing="ij")
+    Wp = np.stack([WX.ravel(), WY.ravel(), WZ.ravel()], 1)
+    inside = ((Wp[:, 0] >= 0) & (Wp[:, 0] < wx) & (Wp[:, 1] >= 0) & (Wp[:, 1] < wy) &
+              (Wp[:, 2] >= 0))
+    wall = (Wp[~inside] + 0.5) * dx
+    r = np.concatenate([wall, fluid])
+    n, nw = r.shape[0], wall.shape[0]
+    fixed = np.zeros(n, dtype=np.uint8)
+    fixed[:nw] = 1
+    m = np.full(n, dx ** 3)
+    H = column[2]
+    c0 = 10.0 * math.sqrt(2.0 * g * H)
+    mu0 = np.full(n, 1.0e-6)
+    mu0[:nw] = 1.0e-3
+    lo = -layers * dx
+    cell = np.array([lo, box[0] - lo, lo, box[1] - lo, lo, box[2] + layers * dx])
+    return SPHSystem(r=r, m=m, fixed=fixed, mu0=mu0, c=np.full(n, c0), h=hfac * dx, cell=cell,
+                     dx=dx)
+
+
+# ------------------------------------------------------------------ DEM
+@dataclass
+class DEMSystem:
+    r: np.ndarray
+    m: np.ndarray
+    rad: np.ndarray
+    fixed: np.ndarray
+    cell: np.ndarray
+
+    @property
+    def n(self):
+        return self.r.shape[0]
+
+
+def dem_settle(n: int = 200_000, rmin: float = 0.9e-3, rmax: float = 1.1e-3,
+               box=(0.1, 0.1, 0.4), density: float = 7874.0, seed: int = 4,
+               gap: float = 0.05) -> DEMSystem:
+    """Granular settle: n spheres with radii uniform in [rmin, rmax] at
+    non-overlapping random positions in the box.  Centres are drawn as the
+    first n sites of a jittered cubic lattice of pitch 2 rmax (1 + gap),
+    taken in random order, with a jitter small enough that no two spheres
+    overlap; 200,000 spheres of ~1 mm in 0.1 x 0.1 x 0.4 m fill ~21 % of the
+    volume.  Masses are those of solid spheres of the given density."""
+    pitch = 2.0 * rmax * (1.0 + gap)
+    nx, ny, nz = (int(box[a] // pitch) for a in range(3))
+    if nx * ny * nz < n:
+        raise ValueError("box too small for n spheres at this pitch")
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    sites = (np.stack([ix.ravel(), iy.ravel(), iz.ravel()], 1) + 0.5) * pitch
+    order = np.argsort(uniform53(seed, sites.shape[0]))
+    sites = sites[np.sort(order[:n])]
+    jit = (uniform53(seed + 1, 3 * n).reshape(n, 3) - 0.5) * (pitch - 2.0 * rmax)
+    r = sites + jit
+    rad = rmin + (rmax - rmin) * uniform53(seed + 2, n)
+    m = density * 4.0 / 3.0 * math.pi * rad ** 3
+    cell = np.array([0.0, box[0], 0.0, box[1], 0.0, box[2]])
+    return DEMSystem(r=r, m=m, rad=rad, fixed=np.zeros(n, dtype=np.uint8), cell=cell)
