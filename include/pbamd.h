@@ -197,6 +197,16 @@ int pb_lj_last_run_stats(pb_sys *sys, uint32_t *rebuilds, uint32_t *max_nn,
 int pb_lj_last_run_timing(pb_sys *sys, double *step_ms, uint32_t *nstep,
                           double *rebuild_ms, uint32_t *nrebuild);
 
+/* Tile-shape autotuner of the fused LJ run (no reference counterpart; the
+ * reference's launch shapes come from CUDAenv::setThnum, CUDAenv.hh:36-50):
+ * times `steps` fused steps with opts for each candidate tile shape (1 or 2
+ * list-cell columns per tile side x 4 LDS fill targets) from the same saved
+ * state, keeps the fastest for later runs of this system and restores the
+ * state.  Outputs (may be NULL): ms per step of the best candidate, its
+ * columns per side and fill. */
+int pb_lj_autotune(pb_sys *sys, const pb_run_opts *opts, uint32_t steps, double *best_ms,
+                   uint32_t *best_w, double *best_fill);
+
 /* ------------------------------------------------------------------ g(r) */
 /* cudaCalcGR::setup + makePairInfo + calcgr (cudaCalcGR.hh:25-39,
  * cudaCalcGR.cu:30-113, kerneltemplate.hh:423-512, kernelinline.hh:440-448):

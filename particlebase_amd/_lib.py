@@ -43,7 +43,7 @@ EXPORTS = [
     "pb_scale_velocities", "pb_inspect_velocity", "pb_setup_cells", "pb_get_cell_info",
     "pb_bin", "pb_get_bins", "pb_get_neighbor_table", "pb_lj_set_params", "pb_lj_set_cutoff",
     "pb_lj_force", "pb_coord_counts", "pb_lj_run", "pb_lj_last_run_stats",
-    "pb_lj_last_run_timing", "pb_gr_hist",
+    "pb_lj_last_run_timing", "pb_lj_autotune", "pb_gr_hist",
     "pb_sph_set_properties", "pb_sph_density", "pb_sph_acceleration", "pb_sph_inspect_density",
     "pb_dem_set_properties", "pb_rot_set_inertia", "pb_rot_step", "pb_dem_calc_force",
     "pb_dem_get_contacts", "pb_dem_last_violations",
@@ -146,6 +146,7 @@ def _extra_signatures():
         "pb_dem_calc_force": (i32, [vp, d, i32, P(u32)]),
         "pb_dem_get_contacts": (i32, [vp, P(u32), P(u32), P(d), u32, P(u32)]),
         "pb_dem_last_violations": (i32, [vp, P(u32)]),
+        "pb_lj_autotune": (i32, [vp, P(pb_run_opts), u32, P(d), P(u32), P(d)]),
     }
 
 
@@ -390,6 +391,13 @@ class System:
         if check:
             self._c(rc)
         return e[:nrec], rc
+
+    def lj_autotune(self, dt, steps=10, precision=PB_MIXED, skin=0.3, rebuild_every=20):
+        o = pb_run_opts(dt, steps, precision, skin, rebuild_every, 0, 0, 0, 1)
+        ms, w, f = C.c_double(), C.c_uint32(), C.c_double()
+        self._c(self.lib.pb_lj_autotune(self.h, C.byref(o), steps, C.byref(ms), C.byref(w),
+                                        C.byref(f)))
+        return {"ms_per_step": ms.value, "tile_w": w.value, "tile_fill": f.value}
 
     def lj_run_stats(self):
         rb = C.c_uint32()

@@ -114,6 +114,8 @@ struct pb_sys {
   uint32_t *list_segs = nullptr;   // list-build work units (5 words each)
   uint32_t list_nsegs = 0;
   float list_rb2 = 0.f;            // FP32 build radius^2 (rlist + rounding margin)
+  uint32_t tile_w = 2;             // list-cell columns per tile side (1 or 2)
+  double tile_fill = 0.72;         // target fill of the LDS window stage
   uint32_t last_rebuilds = 0, last_maxnn = 0;
   int last_skin_violation = 0;
   double last_step_ms = 0.0, last_rebuild_ms = 0.0;

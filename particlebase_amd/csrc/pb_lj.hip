@@ -168,7 +168,8 @@ __global__ void k_reorder(ReorderArgs A) {
 struct ListSeg {
   uint32_t rep, ci, cj, ck0, nz, tx, ty, whole;
 };
-#define TB 384            // threads per workgroup (6 waves)
+#define TB 384            // list-build workgroup (6 waves)
+#define TBS 384           // step workgroup (6 waves)
 #define WCAP 2816         // staged window particles per workgroup (LDS)
 #define MAXCOL 16         // (tx+2)*(ty+2) window columns, tx, ty <= 2
 #define MAXZ 16           // nz <= MAXZ
@@ -180,7 +181,7 @@ struct WinTable {
   uint32_t eoff[MAXE + 1], eg0[MAXE], ec[MAXE], esh[MAXE];
   uint32_t nent, total, ncol, zs;
   uint32_t nrun, rloc[4], rpref[5], rg0[4], rcol[4];
-  uint32_t wsum[TB / 64];
+  uint32_t wsum[TBS / 64];
 };
 #define NL_PAD 0xFFFFu     // list padding entry (maps to an inert slot)
 #define PAD_FAR 1.0e10     // coordinate of that slot
@@ -195,7 +196,7 @@ struct StageB {             // list build: FP32 window-relative positions
   WinTable W;
 };
 
-// exclusive block scan (blockDim multiple of 64, <= TB)
+// exclusive block scan (blockDim multiple of 64, <= TBS)
 __device__ __forceinline__ uint32_t tile_scan(uint32_t x, uint32_t *wsum, uint32_t *total) {
   const uint32_t lane = pb_lane(), w = threadIdx.x / 64, nw = blockDim.x / 64;
   uint32_t inc = x;
@@ -526,24 +527,46 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
   const uint32_t wave = threadIdx.x / 64, nwv = blockDim.x / 64;
   const uint32_t PAD = FAST ? (uint32_t)WCAP : NL_PAD;
   const size_t ld = A.npad;
+  // Software pipeline across particle groups: while group u is evaluated,
+  // the next group's slot, list length, first list batch and integration
+  // state are already in flight.  The first batch (entries 0..15) is read
+  // without knowing the list length: those entries lie inside the list
+  // array for any slot (capacity >= 16) and are masked by the length later.
+  uint32_t nli = 0, ngi = 0, nnn = 0;
+  uint32_t nq[4] = {0, 0, 0, 0};
+  double4 np_own = make_double4(0, 0, 0, 0), nv_own = np_own;
+  float4 nrb_own = make_float4(0, 0, 0, 0);
+  auto fetch = [&](uint32_t ub) {
+    const uint32_t u = ub + grp;
+    uint32_t run;
+    nli = 0;
+    ngi = 0;
+    nnn = 0;
+    if (u < ntile) {
+      tile_particle(S.W, u, nli, ngi, run);
+      nnn = A.nn[ngi];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) nq[j] = A.nl[(size_t)(4u * j + sl) * ld + ngi];
+      if (INTEGRATE && sl == 0) {
+        np_own = A.rin[ngi];
+        nv_own = A.vin[ngi];
+        nrb_own = A.rbuild[ngi];
+      }
+    }
+  };
+  fetch(wave * 16);
   for (uint32_t ub = wave * 16; ub < ntile; ub += nwv * 16) {     // wave-uniform
     const uint32_t u = ub + grp;
     const bool act = u < ntile;
-    uint32_t li = 0, gi = 0, run;
-    if (act) tile_particle(S.W, u, li, gi, run);
-    const uint32_t nni = act ? A.nn[gi] : 0u;
+    const uint32_t li = nli, gi = ngi, nni = nnn;
+    uint32_t qn[4] = {nq[0], nq[1], nq[2], nq[3]};
+    const double4 p_own = np_own, v_own = nv_own;
+    const float4 rb_own = nrb_own;
+    fetch(ub + nwv * 16);
     double4 pi;
     if (FAST) pi = make_double4(S.x[li], S.y[li], S.z[li], (double)S.t[li]);
     else pi = act ? window_pos(S, R, A.rin, staged, li) : make_double4(0, 0, 0, 0);
     const uint32_t ti = ONE_TYPE ? 0u : (uint32_t)pi.w;
-    // per-particle state for the integration, requested before the pair loop
-    double4 p_own = make_double4(0, 0, 0, 0), v_own = p_own;
-    float4 rb_own = make_float4(0, 0, 0, 0);
-    if (INTEGRATE && act && sl == 0) {
-      p_own = A.rin[gi];       // unshifted FP64 position
-      v_own = A.vin[gi];
-      rb_own = A.rbuild[gi];
-    }
     // forces and energies accumulate in FP64 in both modes; MIXED evaluates
     // each pair in FP32 on the FP64 displacement after the FP64 cutoff test
     double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
@@ -605,13 +628,7 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
     };
     const uint16_t *col = A.nl + gi;                 // entry k at col[k * ld]
     const uint32_t last = nni ? nni - 1u : 0u;
-    const uint32_t nbat = (nni + 15u) >> 4;
-    uint32_t qn[4];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const uint32_t idx = 4u * j + sl;
-      qn[j] = col[(size_t)min(idx, last) * ld];
-    }
+    const uint32_t nbat = act ? (nni + 15u) >> 4 : 0u;
     for (uint32_t b = 0; b < nbat; ++b) {
       uint32_t q[4];
 #pragma unroll
@@ -678,7 +695,7 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
 }
 
 template <bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
-__global__ void __launch_bounds__(TB) k_lj_step(StepArgs A) {
+__global__ void __launch_bounds__(TBS) k_lj_step(StepArgs A) {
   __shared__ Stage S;
   publish_prev(A);
   const uint32_t seg = pb_xcd_block();
@@ -697,8 +714,8 @@ __global__ void __launch_bounds__(TB) k_lj_step(StepArgs A) {
     lj_tiles<false, MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE>(A, S, R, staged, mi, ke, pe_b,
                                                                  vir_b, d2m, v2m);
 
-  __shared__ float smx[2][TB / 64];
-  __shared__ double sen[3][TB / 64];
+  __shared__ float smx[2][TBS / 64];
+  __shared__ double sen[3][TBS / 64];
   if (INTEGRATE) {
     // block maxima of displacement^2 and speed^2; an atomic only when the
     // block beats the running maximum (non-negative floats order as uints)
@@ -802,7 +819,8 @@ static int setup_list_cells(pb_sys *s, double rlist) {
   s->list_ncell = cell0;
   s->list_rlist = rlist;
   // Tiles (segments) of tx x ty columns x nz cells.  nz is chosen so the
-  // expected window (tx+2)(ty+2)(nz+2) cells stays near 3/4 of the LDS stage;
+  // expected window (tx+2)(ty+2)(nz+2) cells fills tile_fill of the LDS
+  // stage (0.72 unless pb_lj_autotune chose otherwise);
   // nz <= nb2 - 2 so a window never holds one cell twice.  A replica with
   // fewer than 3 list cells along an axis is one whole segment.
   std::vector<ListSeg> segs;
@@ -816,9 +834,11 @@ static int setup_list_cells(pb_sys *s, double rlist) {
                        "a replica thinner than 3 list cells must hold at most 2816 particles");
       segs.push_back(ListSeg{q, 0, 0, 0, 1, 1, 1, 1});
     } else {
-      const uint32_t tx = R.nb[0] >= 4 ? 2 : 1, ty = R.nb[1] >= 4 ? 2 : 1;
+      const uint32_t tw = s->tile_w;
+      const uint32_t tx = R.nb[0] >= tw + 2 ? tw : 1, ty = R.nb[1] >= tw + 2 ? tw : 1;
       const double per_cell = (double)R.natoms / nc;
-      int nz = (int)std::floor(0.72 * WCAP / ((tx + 2) * (ty + 2) * std::max(per_cell, 1e-3))) - 2;
+      int nz = (int)std::floor(s->tile_fill * WCAP /
+                               ((tx + 2) * (ty + 2) * std::max(per_cell, 1e-3))) - 2;
       nz = std::max(1, std::min<int>({nz, MAXZ, (int)R.nb[2] - 2}));
       for (uint32_t ci = 0; ci < R.nb[0]; ci += tx)
         for (uint32_t cj = 0; cj < R.nb[1]; cj += ty)
@@ -944,7 +964,7 @@ static int launch_step(pb_sys *s, const StepArgs &A, bool mixed, bool energy, bo
                        bool integ) {
   const bool one = s->lj.ntypes == 1;
   StepKernel k = pick_kernel(mixed, one, energy, wf, integ);
-  hipLaunchKernelGGL(k, dim3(s->list_nsegs), dim3(TB), 0, s->ctx->stream, A);
+  hipLaunchKernelGGL(k, dim3(s->list_nsegs), dim3(TBS), 0, s->ctx->stream, A);
   PB_LAUNCH_CHECK(s->ctx, "k_lj_step");
   return PB_OK;
 }
@@ -1072,7 +1092,7 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
   uint32_t rebuilds = 0;
   int skin_bad = 0;
   // optional HIP-event timing on the context's stream
-  std::vector<hipEvent_t> ev_step, ev_reb;
+  std::vector<hipEvent_t> ev_step, ev_reb, ev_graph;   // graph periods: R steps + rebuild
   auto mk = [&](std::vector<hipEvent_t> &v) -> hipEvent_t {
     hipEvent_t e = nullptr;
     if (hipEventCreate(&e) == hipSuccess) v.push_back(e);
@@ -1106,7 +1126,110 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
     A.mon = s->dmon;
     const double half_skin = 0.5 * o->skin;
     uint32_t last_build = 0;
-    for (uint32_t step = 0; step < o->nsteps; ++step) {
+    uint32_t step = 0;
+    // Graph mode: whole rebuild periods -- R steps followed by the list
+    // rebuild that prepares the next period -- are captured once as a
+    // hipGraph and replayed.  The schedule is the fixed interval R (no host
+    // polling); a particle outside skin/2 is still caught by the step
+    // kernel's check.  The step sequence of a period may leave the state in
+    // the other ping-pong buffer, so up to two graphs (one per buffer
+    // assignment) are kept.  The remaining steps (at least the last one,
+    // which writes F) run eagerly.
+    if (o->use_graph) {
+      const uint32_t R = o->rebuild_every;
+      if (ne && R % ne) {
+        rc = pb_fail(s->ctx, PB_ERR_ARG, "pb_lj_run: with use_graph, energy_every must divide rebuild_every");
+        goto done;
+      }
+      const uint32_t periods = (o->nsteps - 1) / R;
+      hipGraphExec_t gx[2] = {nullptr, nullptr};
+      hipGraph_t gg[2] = {nullptr, nullptr};
+      // a graph is valid for the buffer assignment it was captured with
+      double4 *gkey[2] = {nullptr, nullptr};
+      uint32_t *gkey2[2] = {nullptr, nullptr};
+      A.mon = nullptr;
+      auto capture = [&](int slot) -> int {
+        PB_HIP(s->ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int e = PB_OK;
+        for (uint32_t l = 0; l < R && e == PB_OK; ++l) {
+          A.rin = s->r;
+          A.vin = s->v;
+          A.rout = s->r2buf;
+          A.vout = s->v2buf;
+          A.Fout = s->F;
+          A.step = 0;
+          e = launch_step(s, A, mixed, ne && (l % ne == 0), false, true);
+          std::swap(s->r, s->r2buf);
+          std::swap(s->v, s->v2buf);
+        }
+        if (e == PB_OK) e = rebuild(s);
+        if (e == PB_OK)
+          hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s->d_counter + CNT_DISP, 2u, 0u);
+        hipGraph_t g = nullptr;
+        hipError_t he = hipStreamEndCapture(st, &g);
+        if (e != PB_OK) {
+          if (g) (void)hipGraphDestroy(g);
+          return e;
+        }
+        if (he != hipSuccess) return pb_hip_check(s->ctx, he, "hipStreamEndCapture");
+        gg[slot] = g;
+        he = hipGraphInstantiate(&gx[slot], g, nullptr, nullptr, 0);
+        if (he != hipSuccess) return pb_hip_check(s->ctx, he, "hipGraphInstantiate");
+        return PB_OK;
+      };
+      for (uint32_t p = 0; p < periods && rc == PB_OK; ++p) {
+        int slot = -1;
+        for (int k = 0; k < 2; ++k)
+          if (gx[k] && gkey[k] == s->r && gkey2[k] == s->orig) slot = k;
+        if (slot < 0) {
+          // capture records the launches without running them and applies
+          // the period's buffer swaps to s: undo them before replaying
+          double4 *r0 = s->r, *v0 = s->v, *a0 = s->a;
+          uint32_t *o0 = s->orig;
+          uint16_t *t0 = s->type;
+          uint8_t *m0 = s->move;
+          double4 *r1 = s->r2buf, *v1 = s->v2buf, *a1 = s->a2buf;
+          uint32_t *o1 = s->orig2;
+          uint16_t *t1 = s->type2;
+          uint8_t *m1 = s->move2;
+          slot = gx[0] ? 1 : 0;
+          if ((rc = capture(slot))) break;
+          s->r = r0; s->v = v0; s->a = a0; s->orig = o0; s->type = t0; s->move = m0;
+          s->r2buf = r1; s->v2buf = v1; s->a2buf = a1; s->orig2 = o1; s->type2 = t1;
+          s->move2 = m1;
+          gkey[slot] = r0;
+          gkey2[slot] = o0;
+        }
+        rec(ev_graph);
+        hipError_t he = hipGraphLaunch(gx[slot], st);
+        rec(ev_graph);
+        if (he != hipSuccess) {
+          rc = pb_hip_check(s->ctx, he, "hipGraphLaunch");
+          break;
+        }
+        // the replayed period leaves the buffers as its capture did
+        for (uint32_t l = 0; l < R; ++l) {
+          std::swap(s->r, s->r2buf);
+          std::swap(s->v, s->v2buf);
+        }
+        std::swap(s->r, s->r2buf);          // the rebuild's reorder
+        std::swap(s->v, s->v2buf);
+        std::swap(s->a, s->a2buf);
+        std::swap(s->orig, s->orig2);
+        std::swap(s->type, s->type2);
+        std::swap(s->move, s->move2);
+        rebuilds += 1;
+        step += R;
+        last_build = step;
+      }
+      for (int k = 0; k < 2; ++k) {
+        if (gx[k]) (void)hipGraphExecDestroy(gx[k]);
+        if (gg[k]) (void)hipGraphDestroy(gg[k]);
+      }
+      if (rc) goto done;
+      A.mon = s->dmon;
+    }
+    for (; step < o->nsteps; ++step) {
       // Rebuild schedule: at most every rebuild_every steps, and earlier when
       // the particles may reach skin/2.  Kernel `step-1` published the
       // maxima after step `step-2` when it started, so reading them keeps one
@@ -1207,10 +1330,17 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ev_reb[k], ev_reb[k + 1]) == hipSuccess) s->last_rebuild_ms += ms;
   }
+  // a replayed period counts as rebuild_every steps (its rebuild included)
+  for (size_t k = 0; k + 1 < ev_graph.size(); k += 2) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ev_graph[k], ev_graph[k + 1]) == hipSuccess) s->last_step_ms += ms;
+    s->last_nstep_timed += o->rebuild_every;
+  }
   if (skin_bad) rc = pb_fail(s->ctx, PB_ERR_SKIN, "skin violated: increase skin or rebuild more often");
 done:
   for (hipEvent_t e : ev_step) (void)hipEventDestroy(e);
   for (hipEvent_t e : ev_reb) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ev_graph) (void)hipEventDestroy(e);
   pb_free(ebuf);
   return rc;
 }
@@ -1232,4 +1362,104 @@ extern "C" int pb_lj_last_run_stats(pb_sys *s, uint32_t *rebuilds, uint32_t *max
   if (max_nn) *max_nn = s->last_maxnn;
   if (skin_violation) *skin_violation = s->last_skin_violation;
   return PB_OK;
+}
+
+// ------------------------------------------------------------- autotuner
+// Tile shape of the list build and the step kernel: columns per tile side
+// (1 or 2) and the fill of the LDS stage that sets the tile depth nz.
+// Every candidate runs `steps` fused steps from the same saved state, timed
+// with HIP events on the context's stream; the fastest is kept and the
+// state is restored, so the call leaves the simulation where it was.
+extern "C" int pb_lj_autotune(pb_sys *s, const pb_run_opts *o, uint32_t steps, double *best_ms,
+                              uint32_t *best_w, double *best_fill) {
+  if (!s || !o || steps < 2) return PB_ERR_ARG;
+  if (int rc = check_lj_ready(s)) return rc;
+  PB_HIP(s->ctx, hipSetDevice(s->ctx->device));
+  hipStream_t st = s->ctx->stream;
+  const size_t np = s->npad;
+  double4 *sr = nullptr, *sv = nullptr, *sa = nullptr, *sf = nullptr;
+  uint32_t *so = nullptr;
+  uint16_t *sty = nullptr;
+  uint8_t *smv = nullptr;
+  int rc = PB_OK;
+  auto copy_all = [&](bool save) -> int {
+    auto cp = [&](void *dst, const void *src, size_t bytes) {
+      return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, st);
+    };
+    hipError_t e = hipSuccess;
+    if (save) {
+      e = cp(sr, s->r, 32 * np);
+      if (e == hipSuccess) e = cp(sv, s->v, 32 * np);
+      if (e == hipSuccess) e = cp(sa, s->a, 32 * np);
+      if (e == hipSuccess) e = cp(sf, s->F, 32 * np);
+      if (e == hipSuccess) e = cp(so, s->orig, 4 * np);
+      if (e == hipSuccess) e = cp(sty, s->type, 2 * np);
+      if (e == hipSuccess) e = cp(smv, s->move, np);
+    } else {
+      e = cp(s->r, sr, 32 * np);
+      if (e == hipSuccess) e = cp(s->v, sv, 32 * np);
+      if (e == hipSuccess) e = cp(s->a, sa, 32 * np);
+      if (e == hipSuccess) e = cp(s->F, sf, 32 * np);
+      if (e == hipSuccess) e = cp(s->orig, so, 4 * np);
+      if (e == hipSuccess) e = cp(s->type, sty, 2 * np);
+      if (e == hipSuccess) e = cp(s->move, smv, np);
+    }
+    if (e != hipSuccess) return pb_hip_check(s->ctx, e, "pb_lj_autotune copy");
+    return PB_OK;
+  };
+  const uint32_t ws[2] = {1, 2};
+  const double fills[4] = {0.4, 0.55, 0.72, 0.9};
+  const uint32_t w0 = s->tile_w;
+  const double f0 = s->tile_fill;
+  double best = 1e300;
+  uint32_t bw = w0;
+  double bf = f0;
+  if ((rc = pb_alloc(s->ctx, &sr, np)) || (rc = pb_alloc(s->ctx, &sv, np)) ||
+      (rc = pb_alloc(s->ctx, &sa, np)) || (rc = pb_alloc(s->ctx, &sf, np)) ||
+      (rc = pb_alloc(s->ctx, &so, np)) || (rc = pb_alloc(s->ctx, &sty, np)) ||
+      (rc = pb_alloc(s->ctx, &smv, np)))
+    goto out;
+  if ((rc = copy_all(true))) goto out;
+  for (uint32_t w : ws)
+    for (double f : fills) {
+      s->tile_w = w;
+      s->tile_fill = f;
+      if ((rc = copy_all(false))) goto out;
+      pb_run_opts t = *o;
+      t.nsteps = steps;
+      t.energy_every = 0;
+      t.timing = 1;
+      t.use_graph = 0;
+      rc = pb_lj_run(s, &t, nullptr);
+      if (rc == PB_ERR_SKIN || rc == PB_ERR_CELLS) {   // candidate unusable here
+        rc = PB_OK;
+        continue;
+      }
+      if (rc) goto out;
+      const double ms = (s->last_step_ms + s->last_rebuild_ms) / steps;
+      if (ms < best) {
+        best = ms;
+        bw = w;
+        bf = f;
+      }
+    }
+  if (!(best < 1e300)) {
+    rc = pb_fail(s->ctx, PB_ERR_STATE, "pb_lj_autotune: no tile shape could run");
+    goto out;
+  }
+  s->tile_w = bw;
+  s->tile_fill = bf;
+  if ((rc = copy_all(false))) goto out;
+  PB_HIP(s->ctx, hipStreamSynchronize(st));
+  if (best_ms) *best_ms = best;
+  if (best_w) *best_w = bw;
+  if (best_fill) *best_fill = bf;
+out:
+  if (rc) {
+    s->tile_w = w0;
+    s->tile_fill = f0;
+  }
+  (void)hipStreamSynchronize(st);
+  pb_free(sr); pb_free(sv); pb_free(sa); pb_free(sf); pb_free(so); pb_free(sty); pb_free(smv);
+  return rc;
 }

@@ -184,3 +184,62 @@ def test_small_boxes_dedup(ctx, orc):
     p = orc.lj_params([1.0, 1.0], 1, RC2, True)
     Fo, _, _ = orc.lj_forces_brute(w.r, w.type, w.cell, RC2, p)
     assert np.abs(s.get("F") - Fo).max() < 1e-11 * np.abs(Fo).max()
+
+
+@pytest.mark.parametrize("prec", [pb.PB_FP64, pb.PB_MIXED])
+def test_graph_replay_equals_eager(ctx, prec):
+    # Graph mode replays captured rebuild periods (R steps + list rebuild).
+    # With a skin large enough that the eager run's adaptive schedule never
+    # rebuilds before R steps, both runs build identical lists at identical
+    # steps, so the results are identical bit for bit -- over many replays,
+    # with eager steps (the tail) and a second graph run in between.
+    say(f"hipGraph periods vs eager run, config1, prec={prec}")
+    w = workloads.lj_config1()
+    runs = []
+    for graph in (False, True):
+        s = lj_sys(ctx, w)
+        s.clear_force(True)
+        es = []
+        for nsteps in (53, 24):
+            e, _ = s.lj_run(0.005, nsteps, precision=prec, skin=0.6, rebuild_every=6,
+                            energy_every=3, use_graph=graph)
+            es.append(e)
+            assert s.lj_run_stats()["skin_violation"] == 0
+        runs.append((s.get("r"), s.get("v"), s.get("a"), es))
+    (r0, v0, a0, e0), (r1, v1, a1, e1) = runs
+    assert np.array_equal(r0, r1) and np.array_equal(v0, v1) and np.array_equal(a0, a1)
+    for x, y in zip(e0, e1):
+        assert np.array_equal(x, y)
+
+
+def test_graph_batched_replicas(ctx):
+    say("hipGraph periods on batched replicas vs eager")
+    reps = workloads.lj_replicas(6, 500, 3000, seed=77)
+    r, v, m, t, cells = reps.concat()
+    out = []
+    for graph in (False, True):
+        s = pb.System(ctx, reps.counts)
+        s.set_cell(cells)
+        s.import_particles(r, m, v=v, type=t)
+        s.lj_set_params([1.0, 1.0], 1)
+        s.lj_set_cutoff(RC2, True)
+        s.clear_force(True)
+        e, _ = s.lj_run(0.004, 41, precision=pb.PB_MIXED, skin=0.6, rebuild_every=4,
+                        energy_every=2, use_graph=graph)
+        out.append((s.get("r"), e))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_autotune_restores_state(ctx, orc):
+    say("tile-shape autotuner on config1: state restored, tuned run still matches the oracle")
+    w = workloads.lj_config1()
+    s = lj_sys(ctx, w)
+    s.clear_force(True)
+    r0, v0 = s.get("r"), s.get("v")
+    t = s.lj_autotune(0.005, steps=8, precision=pb.PB_FP64, skin=0.3, rebuild_every=10)
+    assert t["tile_w"] in (1, 2) and 0.3 < t["tile_fill"] < 1.0 and t["ms_per_step"] > 0
+    assert np.array_equal(s.get("r"), r0) and np.array_equal(s.get("v"), v0)
+    s.lj_force(pb.PB_FP64)
+    p = orc.lj_params([1.0, 1.0], 1, RC2, True)
+    Fo, _, _ = orc.lj_forces_cells(w.r, w.type, w.cell, RC2, p)
+    assert np.abs(s.get("F") - Fo).max() < 1e-12 * np.sqrt((Fo ** 2).sum(1).mean())
