@@ -97,6 +97,88 @@ def cpu_baseline(w, budget_s=20.0):
                       f"({el:.1f} s, FP64, OpenMP {cores} threads)"}
 
 
+def bench_sph(ctx, a):
+    """SPH dam break (reference physics), the reference driver's step:
+    bin, density, acceleration, gravity, leap-frog, inspectVelocity,
+    absolute boundary."""
+    w = workloads.sph_dam_break(0.08 if a.quick else 0.4 / 63)
+    s = pb.System(ctx, [w.n])
+    s.set_cell(w.cell)
+    s.import_particles(w.r, w.m, is_fixed=w.fixed)
+    s.sph_set_properties(w.mu0, w.c, w.h)
+    s.setup_cells(w.h, periodic=False)
+    dt = 0.25 * w.h / w.c[0]
+
+    def step():
+        s.bin()
+        s.sph_density()
+        s.sph_acceleration()
+        s.add_acceleration(2, -9.81)
+        s.lf_step(dt)
+        s.inspect_velocity(2 * w.h / dt, 0.04, 0.0025)
+        s.treat_absolute()
+
+    for _ in range(3):
+        step()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.sph_steps):
+        step()
+    ctx.sync()
+    el = time.perf_counter() - t0
+    v = w.n * a.sph_steps / el
+    log(f"sph: N={w.n} ({int((w.fixed == 0).sum())} fluid), {v:.4g} particle-steps/s, "
+        f"{1e3 * el / a.sph_steps:.3f} ms/step")
+    return {"workload": f"SPH dam break, Lucy h=1.7dx, dx={w.dx:.6g}, {int((w.fixed == 0).sum())} "
+                        f"fluid + {int(w.fixed.sum())} wall particles, FP64",
+            "value": v, "unit": "particle-steps/s", "ms_per_step": 1e3 * el / a.sph_steps,
+            "steps": a.sph_steps}
+
+
+def bench_dem(ctx, a):
+    """DEM settle (Hertz contacts with rotation), the reference driver's
+    step: bin, contacts + force, acceleration, gravity, leap-frog, rotation,
+    inspectVelocity, reflecting walls."""
+    w = workloads.dem_settle(20000 if a.quick else 200_000)
+    s = pb.System(ctx, [w.n])
+    s.set_cell(w.cell)
+    s.import_particles(w.r, w.m, is_fixed=w.fixed)
+    gamma = -math.log(0.8) / math.sqrt(math.pi ** 2 + math.log(0.8) ** 2) * 2
+    # Y = 2e8 Pa: the Hertz stiffness 2/3 Y sqrt(R xi) is ~1e4 N/m at 10 um
+    # overlap, the config's linear spring constant
+    s.dem_set_properties(2.0e8, 0.5, 0.29, gamma, 0.1, w.rad)
+    s.rot_set_inertia(w.rad)
+    s.setup_cells(2 * w.rad.max(), periodic=False)
+    dt = 2e-6
+    nnz = [0]
+
+    def step():
+        s.bin()
+        nnz[0] = s.dem_calc_force(dt, autotune=True)
+        s.calc_acceleration()
+        s.add_acceleration(2, -9.8)
+        s.lf_step(dt)
+        s.rot_step(dt)
+        s.inspect_velocity(2 * 0.9e-3 / dt, 0.04, 0.0025)
+        s.treat_reflect()
+
+    for _ in range(3):
+        step()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(a.dem_steps):
+        step()
+    ctx.sync()
+    el = time.perf_counter() - t0
+    v = w.n * a.dem_steps / el
+    log(f"dem: N={w.n}, {v:.4g} particle-steps/s, {1e3 * el / a.dem_steps:.3f} ms/step, "
+        f"nnz={nnz[0]}")
+    return {"workload": f"DEM settle (first steps of the fall), {w.n} spheres r in [0.9, 1.1] mm "
+                        f"in 0.1x0.1x0.4 m, Hertz + Coulomb + rotation, dt 2e-6, FP64",
+            "value": v, "unit": "particle-steps/s",
+            "ms_per_step": 1e3 * el / a.dem_steps, "steps": a.dem_steps, "contacts": nnz[0]}
+
+
 def load_traffic():
     path = os.path.join(ROOT, "profiles", "lj_step_1M_traffic.json")
     if os.path.exists(path):
@@ -117,6 +199,10 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-sph", action="store_true")
+    ap.add_argument("--no-dem", action="store_true")
+    ap.add_argument("--sph-steps", type=int, default=50)
+    ap.add_argument("--dem-steps", type=int, default=50)
     ap.add_argument("--quick", action="store_true", help="small sizes (debug)")
     ap.add_argument("--profile", action="store_true",
                     help="only the mixed 1M run (for rocprofv3 kernel traces / counters)")
@@ -229,6 +315,11 @@ def main():
         out["batched"] = {"replicas": len(reps.systems), "atoms": ntot, "value": vb,
                           "plain_sequential_fp64": vp, "speedup": vb / vp,
                           "steps": a.batch_steps, "plain_steps": a.plain_steps}
+
+    if not a.no_sph:
+        out["sph"] = bench_sph(ctx, a)
+    if not a.no_dem:
+        out["dem"] = bench_dem(ctx, a)
 
     if not a.no_cpu:
         log("cpu baseline ...")

@@ -109,7 +109,8 @@ int main(int argc, char **argv) {
       }
     }
   }
-  std::cout << "done N=" << N << " steps=" << steps << " E0/N=" << e0 << " E1/N=" << e1
-            << " dE/N=" << (e1 - e0) << std::endl;
+  // energies per particle at the first and last record
+  std::cout << "done N=" << N << " steps=" << steps << " E0=" << e0 << " E1=" << e1
+            << " dE=" << (e1 - e0) << std::endl;
   return 0;
 }

@@ -18,7 +18,7 @@ def run(*args):
     exe = os.path.join(BIN, args[0])
     if not os.path.exists(exe):
         pytest.fail(f"{exe} not built (make -C particlebase_amd/cpp)")
-    print(f"\n[case] driver {' '.join(args)}", flush=True)
+    print(f"\n[case] driver {' '.join(map(str, args))}", flush=True)
     p = subprocess.run([exe, *map(str, args[1:])], capture_output=True, text=True, timeout=150)
     assert p.returncode == 0, p.stderr[-2000:]
     line = [x for x in p.stdout.splitlines() if x.startswith("done")][-1]

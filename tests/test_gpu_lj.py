@@ -232,7 +232,7 @@ def test_graph_batched_replicas(ctx):
 
 def test_autotune_restores_state(ctx, orc):
     say("tile-shape autotuner on config1: state restored, tuned run still matches the oracle")
-    w = workloads.lj_config1()
+    w = jittered(workloads.lj_config1(), 0.05, 3)
     s = lj_sys(ctx, w)
     s.clear_force(True)
     r0, v0 = s.get("r"), s.get("v")
