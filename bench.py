@@ -139,9 +139,9 @@ def bench_dem(ctx, a):
     """DEM settle (Hertz contacts with rotation), the reference driver's
     step: bin, contacts + force, acceleration, gravity, leap-frog, rotation,
     inspectVelocity, reflecting walls."""
-    # a compacted start (lattice pitch 2.1 mm < 2 r_max) so that contacts are
-    # present from the first step, as in the settled bed
-    w = workloads.dem_settle(20000 if a.quick else 200_000, gap=-0.045)
+    # a packed bed (the lowest sites of a lattice of pitch 2.1 mm < 2 r_max)
+    # so that contacts are present from the first step, as in the settled bed
+    w = workloads.dem_settle(20000 if a.quick else 200_000, gap=-0.045, fill="bottom")
     s = pb.System(ctx, [w.n])
     s.set_cell(w.cell)
     s.import_particles(w.r, w.m, is_fixed=w.fixed)
@@ -176,7 +176,7 @@ def bench_dem(ctx, a):
     log(f"dem: N={w.n}, {v:.4g} particle-steps/s, {1e3 * el / a.dem_steps:.3f} ms/step, "
         f"nnz={nnz[0]}")
     return {"workload": f"DEM settle, {w.n} spheres r in [0.9, 1.1] mm in 0.1x0.1x0.4 m from a "
-                        f"compacted lattice (pitch 2.1 mm), Hertz + Coulomb + rotation, dt 2e-6, "
+                        f"packed bed (lattice pitch 2.1 mm), Hertz + Coulomb + rotation, dt 2e-6, "
                         f"FP64",
             "value": v, "unit": "particle-steps/s",
             "ms_per_step": 1e3 * el / a.dem_steps, "steps": a.dem_steps, "contacts": nnz[0]}

@@ -217,21 +217,26 @@ class DEMSystem:
 
 def dem_settle(n: int = 200_000, rmin: float = 0.9e-3, rmax: float = 1.1e-3,
                box=(0.1, 0.1, 0.4), density: float = 7874.0, seed: int = 4,
-               gap: float = 0.05) -> DEMSystem:
+               gap: float = 0.05, fill: str = "random") -> DEMSystem:
     """Granular settle: n spheres with radii uniform in [rmin, rmax] at
     non-overlapping random positions in the box.  Centres are drawn as the
     first n sites of a jittered cubic lattice of pitch 2 rmax (1 + gap),
     taken in random order, with a jitter small enough that no two spheres
     overlap; 200,000 spheres of ~1 mm in 0.1 x 0.1 x 0.4 m fill ~21 % of the
-    volume.  Masses are those of solid spheres of the given density."""
+    volume.  fill="bottom" takes the lowest n sites instead (a packed bed;
+    with gap < 0 neighbouring spheres overlap slightly).  Masses are those of
+    solid spheres of the given density."""
     pitch = 2.0 * rmax * (1.0 + gap)
     nx, ny, nz = (int(box[a] // pitch) for a in range(3))
     if nx * ny * nz < n:
         raise ValueError("box too small for n spheres at this pitch")
     ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
     sites = (np.stack([ix.ravel(), iy.ravel(), iz.ravel()], 1) + 0.5) * pitch
-    order = np.argsort(uniform53(seed, sites.shape[0]))
-    sites = sites[np.sort(order[:n])]
+    if fill == "bottom":
+        sites = sites[np.argsort(sites[:, 2], kind="stable")[:n]]
+    else:
+        order = np.argsort(uniform53(seed, sites.shape[0]))
+        sites = sites[np.sort(order[:n])]
     jit = (uniform53(seed + 1, 3 * n).reshape(n, 3) - 0.5) * (pitch - 2.0 * rmax)
     r = sites + jit
     rad = rmin + (rmax - rmin) * uniform53(seed + 2, n)
