@@ -169,7 +169,7 @@ struct ListSeg {
   uint32_t rep, ci, cj, ck0, nz, tx, ty, whole;
 };
 #define TB 384            // list-build workgroup (6 waves)
-#define TBS 384           // step workgroup (6 waves)
+#define TBS 512           // step workgroup (8 waves; two per CU at <= 128 VGPRs)
 #define WCAP 2816         // staged window particles per workgroup (LDS)
 #define MAXCOL 16         // (tx+2)*(ty+2) window columns, tx, ty <= 2
 #define MAXZ 16           // nz <= MAXZ
@@ -261,6 +261,7 @@ __device__ void window_table(WinTable &W, const ListSeg &G, const RepInfo &R,
   uint32_t tot;
   const uint32_t off = tile_scan(c, W.wsum, &tot);
   if (tid < W.nent) W.eoff[tid] = off;
+  __syncthreads();                   // thread 0 reads other threads' offsets below
   if (tid == 0) {
     W.eoff[W.nent] = tot;
     W.total = tot;
@@ -365,8 +366,11 @@ __device__ __forceinline__ void tile_particle(const WinTable &W, uint32_t u, uin
 // the FP32 rounding: the list is a superset of the FP64 pairs within
 // rc + skin.  A window larger than the stage reads FP64 from global memory;
 // a window of more than 65534 particles (u16 indices) raises CNT_WIN.
-// Entry k of particle gi is nl[k * npad + gi] (neighbour-major: the lanes of
-// a wave, consecutive particles, store and load adjacent entries).
+// Layout: the entries of a slot are grouped in blocks of 4 (one 8-byte
+// word); block m of slot gi is word m * npad + gi, so the step kernel's lanes
+// load whole words and the 16 particles of a wavefront read adjacent words.
+// Lists are padded with NL_PAD to a multiple of 16 entries (nn holds the
+// padded length).
 __global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint32_t *cstart,
                                                    const ListSeg *segs, const RepInfo *reps,
                                                    uint32_t npad, double rlist2, float rb2,
@@ -405,8 +409,10 @@ __global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint3
     uint32_t li, gi, run;
     tile_particle(W, u, li, gi, run);
     uint32_t cnt = 0;
+    // entry k of slot gi: u16 element ((k / 4) * npad + gi) * 4 + k % 4
+    uint16_t *row = nl + 4 * (size_t)gi;
     auto keep = [&](uint32_t q) {
-      if (cnt < maxnb) nl[(size_t)cnt * npad + gi] = (uint16_t)q;
+      if (cnt < maxnb) row[(size_t)(cnt >> 2) * 4 * npad + (cnt & 3u)] = (uint16_t)q;
       ++cnt;
     };
     auto scan = [&](uint32_t q0, uint32_t q1, bool mi) {
@@ -452,7 +458,11 @@ __global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint3
           scan(W.eoff[e0], W.eoff[e0 + 3], false);   // three z slots
         }
     }
-    nn[gi] = cnt < maxnb ? cnt : maxnb;
+    // pad to whole batches of 16 with the inert entry
+    const uint32_t used = cnt < maxnb ? cnt : maxnb;
+    const uint32_t padded = (used + 15u) & ~15u;
+    for (uint32_t k = used; k < padded; ++k) row[(size_t)(k >> 2) * 4 * npad + (k & 3u)] = (uint16_t)NL_PAD;
+    nn[gi] = padded;
     if (cnt > maxnb) atomicMax(&counter[CNT_OVF], cnt);
     mymax = max(mymax, cnt);
   }
@@ -465,7 +475,7 @@ __global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint3
 struct StepArgs {
   const double4 *rin, *vin;
   double4 *rout, *vout, *Fout;
-  const uint16_t *nl;
+  const uint64_t *nl;       // 4 u16 entries per word (see k_build_list)
   const uint32_t *nn;
   const float4 *rbuild;
   const uint32_t *cstart;
@@ -530,13 +540,15 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
   // Software pipeline across particle groups: while group u is evaluated,
   // the next group's slot, list length, first list batch and integration
   // state are already in flight.  The first batch (entries 0..15) is read
-  // without knowing the list length: those entries lie inside the list
-  // array for any slot (capacity >= 16) and are masked by the length later.
+  // without knowing the list length: those words lie inside the list array
+  // for any slot (capacity >= 16 entries) and are not used past the length.
   uint32_t nli = 0, ngi = 0, nnn = 0;
-  uint32_t nq[4] = {0, 0, 0, 0};
+  uint64_t nq = 0;
   double4 np_own = make_double4(0, 0, 0, 0), nv_own = np_own;
   float4 nrb_own = make_float4(0, 0, 0, 0);
-  auto fetch = [&](uint32_t ub) {
+  // the next group's slot, length and first list word, needed when its
+  // iteration starts: requested at the start of the current iteration
+  auto fetch_list = [&](uint32_t ub) {
     const uint32_t u = ub + grp;
     uint32_t run;
     nli = 0;
@@ -545,24 +557,28 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
     if (u < ntile) {
       tile_particle(S.W, u, nli, ngi, run);
       nnn = A.nn[ngi];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) nq[j] = A.nl[(size_t)(4u * j + sl) * ld + ngi];
-      if (INTEGRATE && sl == 0) {
-        np_own = A.rin[ngi];
-        nv_own = A.vin[ngi];
-        nrb_own = A.rbuild[ngi];
-      }
+      nq = A.nl[(size_t)sl * ld + ngi];
     }
   };
-  fetch(wave * 16);
+  // the next group's integration state, needed at the end of its iteration:
+  // requested at the end of the current one (one set of registers live)
+  auto fetch_state = [&](uint32_t ub) {
+    if (INTEGRATE && sl == 0 && ub + grp < ntile) {
+      np_own = A.rin[ngi];
+      nv_own = A.vin[ngi];
+      nrb_own = A.rbuild[ngi];
+    }
+  };
+  fetch_list(wave * 16);
+  fetch_state(wave * 16);
   for (uint32_t ub = wave * 16; ub < ntile; ub += nwv * 16) {     // wave-uniform
     const uint32_t u = ub + grp;
     const bool act = u < ntile;
     const uint32_t li = nli, gi = ngi, nni = nnn;
-    uint32_t qn[4] = {nq[0], nq[1], nq[2], nq[3]};
+    uint64_t qw = nq;
     const double4 p_own = np_own, v_own = nv_own;
     const float4 rb_own = nrb_own;
-    fetch(ub + nwv * 16);
+    fetch_list(ub + nwv * 16);
     double4 pi;
     if (FAST) pi = make_double4(S.x[li], S.y[li], S.z[li], (double)S.t[li]);
     else pi = act ? window_pos(S, R, A.rin, staged, li) : make_double4(0, 0, 0, 0);
@@ -585,8 +601,10 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
       const double r2d = dx * dx + dy * dy + dz * dz;
       const bool in = r2d < A.rc2 && (FAST || !mi || q != NL_PAD);
       if (MIXED) {
-        const float ex = (float)dx, ey = (float)dy, ez = (float)dz;
-        const float r2 = in ? (float)r2d : 1.0f;
+        // r2 of a padding entry is large but finite; zero r2 (an image of a
+        // padding entry under the minimum image) gives inf / NaN that the
+        // selects below discard
+        const float r2 = (float)r2d;
         const float s2 = ONE_TYPE ? A.s2f : A.lj->s2f[t];
         const float e24 = ONE_TYPE ? A.e24f : A.lj->e24f[t];
         float ir2 = __builtin_amdgcn_rcpf(r2);
@@ -595,18 +613,19 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
         s6 = s6 * s6 * s6;
         float fr = e24 * s6 * (2.0f * s6 - 1.0f) * ir2;   // kernelinline.hh:90-95
         fr = in ? fr : 0.0f;
-        fx += (double)(fr * ex);
-        fy += (double)(fr * ey);
-        fz += (double)(fr * ez);
+        const double frd = (double)fr;              // f/r, applied to the FP64 displacement
+        fx = fma(frd, dx, fx);
+        fy = fma(frd, dy, fy);
+        fz = fma(frd, dz, fz);
         if (ENERGY) {
           const float e4 = ONE_TYPE ? A.e4f : A.lj->e4f[t];
           const float sh = ONE_TYPE ? A.shiftf : A.lj->shiftf[t];
           const float ep = e4 * s6 * (s6 - 1.0f) - sh;
           pe += (double)(in ? ep : 0.0f);
-          vir += (double)(fr * r2);
+          vir = fma(frd, r2d, vir);
         }
       } else {
-        const double r2 = in ? r2d : 1.0;
+        const double r2 = r2d;
         const double s2 = ONE_TYPE ? A.s2 : A.lj->s2[t];
         const double e24 = ONE_TYPE ? A.e24 : A.lj->e24[t];
         const double ir2 = rcp64(r2);
@@ -626,23 +645,18 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
         }
       }
     };
-    const uint16_t *col = A.nl + gi;                 // entry k at col[k * ld]
-    const uint32_t last = nni ? nni - 1u : 0u;
-    const uint32_t nbat = act ? (nni + 15u) >> 4 : 0u;
+    // batch b: lane sl takes list word 4 b + sl (entries 16 b + 4 sl .. + 3);
+    // lists are padded with the inert entry to whole batches
+    const uint64_t *wp = A.nl + (size_t)(4u + sl) * ld + gi;   // word of the next batch
+    const size_t wstep = 4 * ld;
+    const uint32_t nbat = act ? nni >> 4 : 0u;
     for (uint32_t b = 0; b < nbat; ++b) {
-      uint32_t q[4];
+      const uint32_t lo = (uint32_t)qw, hi = (uint32_t)(qw >> 32);
+      qw = *wp;                                   // one batch ahead (slack words at the end)
+      wp += wstep;
+      uint32_t q[4] = {lo & 0xFFFFu, lo >> 16, hi & 0xFFFFu, hi >> 16};
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t idx = 16u * b + 4u * j + sl;
-        q[j] = idx < nni ? qn[j] : PAD;
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t idx = 16u * (b + 1) + 4u * j + sl;
-        qn[j] = col[(size_t)min(idx, last) * ld];
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) pair(q[j]);
+      for (int j = 0; j < 4; ++j) pair(FAST ? min(q[j], PAD) : q[j]);
     }
 #pragma unroll
     for (int o = 2; o > 0; o >>= 1) {
@@ -654,6 +668,7 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
         vir += __shfl_xor(vir, o, 64);
       }
     }
+    fetch_state(ub + nwv * 16);
     if (!act || sl != 0) continue;
 
     if (INTEGRATE) {
@@ -695,7 +710,7 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
 }
 
 template <bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
-__global__ void __launch_bounds__(TBS) k_lj_step(StepArgs A) {
+__global__ void __launch_bounds__(TBS) __attribute__((amdgpu_waves_per_eu(4))) k_lj_step(StepArgs A) {
   __shared__ Stage S;
   publish_prev(A);
   const uint32_t seg = pb_xcd_block();
@@ -875,7 +890,8 @@ static int ensure_list_buffers(pb_sys *s, uint32_t maxnb) {
       return rc;
   }
   if (s->maxnb < maxnb) {
-    if ((rc = pb_alloc(s->ctx, &s->nl, (size_t)maxnb * s->npad))) return rc;
+    // + 16 entries per slot: the step kernel prefetches one batch past the end
+    if ((rc = pb_alloc(s->ctx, &s->nl, (size_t)(maxnb + 16) * s->npad))) return rc;
     s->maxnb = maxnb;
   }
   return PB_OK;
@@ -918,7 +934,7 @@ static int rebuild(pb_sys *s) {
 static StepArgs make_args(pb_sys *s, double dt) {
   StepArgs A;
   std::memset(&A, 0, sizeof(A));
-  A.nl = s->nl;
+  A.nl = (const uint64_t *)s->nl;
   A.nn = s->nn;
   A.rbuild = s->rbuild;
   A.cstart = s->cstart;
@@ -1267,9 +1283,9 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
         rec(ev_reb);
         ++rebuilds;
         last_build = step;
-        A.nl = s->nl;
+        A.nl = (const uint64_t *)s->nl;
         A.nn = s->nn;
-            }
+      }
       const bool energy = ne && (step % ne == 0);
       const bool last = step + 1 == o->nsteps;
       A.rin = s->r;

@@ -61,7 +61,15 @@ __device__ __forceinline__ void for_neighbors(const SphArgs &A, uint32_t c, F &&
       const int kl = k0 > 0 ? k0 - 1 : 0, kh = k0 + 1 < (int)nz ? k0 + 1 : k0;
       const uint32_t cb = (uint32_t)i2 * nyz + (uint32_t)j2 * nz;
       const uint32_t q0 = A.cstart[cb + (uint32_t)kl], q1 = A.cstart[cb + (uint32_t)kh + 1];
-      for (uint32_t q = q0; q < q1; ++q) f(q);
+      // four candidates per iteration so their loads are in flight together
+      uint32_t q = q0;
+      for (; q + 4 <= q1; q += 4) {
+        f(q);
+        f(q + 1);
+        f(q + 2);
+        f(q + 3);
+      }
+      for (; q < q1; ++q) f(q);
     }
   }
 }
@@ -78,13 +86,12 @@ __global__ void __launch_bounds__(256) k_sph_density(SphArgs A, double4 *num, do
     const double4 pj = A.rs[q];
     const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
     const double R = sqrt(dx * dx + dy * dy + dz * dz);
-    const double t = A.h - R;
-    if (t > 0.0) {                                  // kernelinline.hh:182-188
-      const double W = A.w0 * (A.h + 3.0 * R) * (t * t * t);
-      n0 += W;
-      rho += pj.w * W;
-      mr += A.vs[q].w * pj.w * W;
-    }
+    const double t = fmax(A.h - R, 0.0);            // W = 0 beyond h, kernelinline.hh:182-188
+    const double W = A.w0 * (A.h + 3.0 * R) * (t * t * t);
+    const double mW = pj.w * W;
+    n0 += W;
+    rho += mW;
+    mr = fma(A.vs[q].w, mW, mr);
   });
   const double ri = 1.0 / rho;
   const double mu = mr * ri;
@@ -114,16 +121,14 @@ __global__ void __launch_bounds__(256) k_sph_accel(SphArgs A, double4 *acc) {
     const double4 pj = A.rs[q];
     const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
     const double R = sqrt(dx * dx + dy * dy + dz * dz);
-    const double t = A.h - R;
-    if (t > 0.0 && q != k) {
-      const double4 vj = A.vs[q], dj = A.ds[q];
-      const double mj = pj.w * (A.w1 * t * t);    // m_j (dW/dr)/r, kernelinline.hh:209-214
-      const double p1 = di.y + dj.y;
-      const double v1 = (di.z + dj.z) * di.x * dj.x;
-      ax += mj * (dx * p1 - (vj.x - vi.x) * v1);
-      ay += mj * (dy * p1 - (vj.y - vi.y) * v1);
-      az += mj * (dz * p1 - (vj.z - vi.z) * v1);
-    }
+    const double t = (q != k) ? fmax(A.h - R, 0.0) : 0.0;   // no self term, 0 beyond h
+    const double4 vj = A.vs[q], dj = A.ds[q];
+    const double mj = pj.w * (A.w1 * t * t);      // m_j (dW/dr)/r, kernelinline.hh:209-214
+    const double p1 = di.y + dj.y;
+    const double v1 = (di.z + dj.z) * di.x * dj.x;
+    ax += mj * (dx * p1 - (vj.x - vi.x) * v1);
+    ay += mj * (dy * p1 - (vj.y - vi.y) * v1);
+    az += mj * (dz * p1 - (vj.z - vi.z) * v1);
   });
   acc[s] = make_double4(ax, ay, az, 0.0);
 }

@@ -243,3 +243,25 @@ def test_autotune_restores_state(ctx, orc):
     p = orc.lj_params([1.0, 1.0], 1, RC2, True)
     Fo, _, _ = orc.lj_forces_cells(w.r, w.type, w.cell, RC2, p)
     assert np.abs(s.get("F") - Fo).max() < 1e-12 * np.sqrt((Fo ** 2).sum(1).mean())
+
+
+def test_force_1M_vs_oracle(ctx, orc):
+    # the full 1M-atom workload: every atom's force on the first call of a
+    # fresh system and again after a fused run (window tables of all 3087
+    # segments, XCD-remapped blocks)
+    say("lj force N=1,048,576 jittered, FP64 and mixed, fresh system and after a run")
+    w = jittered(workloads.lj_fcc(64, 0.8442, 0.72, seed=2), 0.05, 1)
+    p = orc.lj_params([1.0, 1.0], 1, RC2, True)
+    Fo, _, _ = orc.lj_forces_cells(w.r, w.type, w.cell, RC2, p)
+    frms = np.sqrt((Fo ** 2).sum(axis=1).mean())
+    s = lj_sys(ctx, w)
+    for prec, tol in ((pb.PB_FP64, 1e-12), (pb.PB_MIXED, 3 * MIXED_TOL), (pb.PB_FP64, 1e-12)):
+        s.lj_force(prec, energies=False)
+        assert np.abs(s.get("F") - Fo).max() / frms < tol, prec
+    s.clear_force(True)
+    s.lj_run(0.005, 30, precision=pb.PB_MIXED, skin=0.3, rebuild_every=20, energy_every=0)
+    r = s.get("r")
+    Fo, _, _ = orc.lj_forces_cells(r, w.type, w.cell, RC2, p)
+    for prec, tol in ((pb.PB_FP64, 1e-12), (pb.PB_MIXED, 3 * MIXED_TOL)):
+        s.lj_force(prec, energies=False)
+        assert np.abs(s.get("F") - Fo).max() / frms < tol, prec
