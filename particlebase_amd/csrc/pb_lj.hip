@@ -409,25 +409,41 @@ __global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint3
     uint32_t li, gi, run;
     tile_particle(W, u, li, gi, run);
     uint32_t cnt = 0;
-    // entry k of slot gi: u16 element ((k / 4) * npad + gi) * 4 + k % 4
+    // entry k of slot gi: u16 element ((k / 4) * npad + gi) * 4 + k % 4;
+    // wp points at the word of entry cnt
     uint16_t *row = nl + 4 * (size_t)gi;
+    uint16_t *wp = row;
+    const size_t wstride = 4 * (size_t)npad;
     auto keep = [&](uint32_t q) {
-      if (cnt < maxnb) row[(size_t)(cnt >> 2) * 4 * npad + (cnt & 3u)] = (uint16_t)q;
+      if (cnt < maxnb) wp[cnt & 3u] = (uint16_t)q;
       ++cnt;
+      if ((cnt & 3u) == 0) wp += wstride;
     };
     auto scan = [&](uint32_t q0, uint32_t q1, bool mi) {
       if (staged) {
         const float4 pi = S.p[li];
-        for (uint32_t q = q0; q < q1; ++q) {
-          const float4 pj = S.p[q];
+        auto near = [&](const float4 &pj) {
           float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
           if (mi) {
             dx -= Lx * rintf(dx * Ix);
             dy -= Ly * rintf(dy * Iy);
             dz -= Lz * rintf(dz * Iz);
           }
-          if (dx * dx + dy * dy + dz * dz < rb2 && q != li) keep(q);
+          return dx * dx + dy * dy + dz * dz < rb2;
+        };
+        // four candidates per iteration: their LDS reads are issued together
+        uint32_t q = q0;
+        for (; q + 4 <= q1; q += 4) {
+          const float4 a = S.p[q], b = S.p[q + 1], c = S.p[q + 2], d = S.p[q + 3];
+          const bool ha = near(a) && q != li, hb = near(b) && q + 1 != li,
+                     hc = near(c) && q + 2 != li, hd = near(d) && q + 3 != li;
+          if (ha) keep(q);
+          if (hb) keep(q + 1);
+          if (hc) keep(q + 2);
+          if (hd) keep(q + 3);
         }
+        for (; q < q1; ++q)
+          if (near(S.p[q]) && q != li) keep(q);
       } else {
         const double4 pi = window_global(W, R, r, li);
         for (uint32_t q = q0; q < q1; ++q) {

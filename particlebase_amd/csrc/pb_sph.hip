@@ -121,14 +121,16 @@ __global__ void __launch_bounds__(256) k_sph_accel(SphArgs A, double4 *acc) {
     const double4 pj = A.rs[q];
     const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
     const double R = sqrt(dx * dx + dy * dy + dz * dz);
-    const double t = (q != k) ? fmax(A.h - R, 0.0) : 0.0;   // no self term, 0 beyond h
-    const double4 vj = A.vs[q], dj = A.ds[q];
-    const double mj = pj.w * (A.w1 * t * t);      // m_j (dW/dr)/r, kernelinline.hh:209-214
-    const double p1 = di.y + dj.y;
-    const double v1 = (di.z + dj.z) * di.x * dj.x;
-    ax += mj * (dx * p1 - (vj.x - vi.x) * v1);
-    ay += mj * (dy * p1 - (vj.y - vi.y) * v1);
-    az += mj * (dz * p1 - (vj.z - vi.z) * v1);
+    const double t = A.h - R;
+    if (t > 0.0 && q != k) {                      // no self term, 0 beyond h
+      const double4 vj = A.vs[q], dj = A.ds[q];
+      const double mj = pj.w * (A.w1 * t * t);    // m_j (dW/dr)/r, kernelinline.hh:209-214
+      const double p1 = di.y + dj.y;
+      const double v1 = (di.z + dj.z) * di.x * dj.x;
+      ax += mj * (dx * p1 - (vj.x - vi.x) * v1);
+      ay += mj * (dy * p1 - (vj.y - vi.y) * v1);
+      az += mj * (dz * p1 - (vj.z - vi.z) * v1);
+    }
   });
   acc[s] = make_double4(ax, ay, az, 0.0);
 }
