@@ -317,18 +317,39 @@ __device__ __forceinline__ double4 window_global(const WinTable &W, const RepInf
 }
 
 // Stage the window in FP64 (pair kernel) when it fits.
+// Copy every window entry (cell) into its LDS range, image shift applied:
+// 16 lanes per entry, 4 entries per wavefront at a time, so the copies of
+// different cells are in flight together (no per-particle entry search).
+template <typename F>
+__device__ __forceinline__ void for_window_particles(const WinTable &W, const RepInfo &R,
+                                                     const double4 *r, F &&f) {
+  const uint32_t lane = pb_lane(), g = lane >> 4, l16 = lane & 15u;
+  const uint32_t wave = threadIdx.x / 64, nwv = blockDim.x / 64;
+  for (uint32_t e0 = wave * 4; e0 < W.nent; e0 += nwv * 4) {
+    const uint32_t e = e0 + g;
+    if (e >= W.nent) continue;
+    const uint32_t c = W.ec[e], src = W.eg0[e], dst = W.eoff[e], sh = W.esh[e];
+    const double sx = (double)((int)(sh & 3u) - 1) * R.L[0];
+    const double sy = (double)((int)((sh >> 2) & 3u) - 1) * R.L[1];
+    const double sz = (double)((int)((sh >> 4) & 3u) - 1) * R.L[2];
+    for (uint32_t k = l16; k < c; k += 16) {
+      const double4 p = r[src + k];
+      f(dst + k, make_double4(p.x + sx, p.y + sy, p.z + sz, p.w));
+    }
+  }
+}
+
+// Stage the window in FP64 (pair kernel) when it fits.
 __device__ void stage_window(Stage &S, const ListSeg &G, const RepInfo &R, const uint32_t *cstart,
                              const double4 *r) {
   window_table(S.W, G, R, cstart);
-  if (S.W.total <= WCAP) {
-    for (uint32_t q = threadIdx.x; q < S.W.total; q += blockDim.x) {
-      const double4 p = window_global(S.W, R, r, q);
+  if (S.W.total <= WCAP)
+    for_window_particles(S.W, R, r, [&](uint32_t q, const double4 &p) {
       S.x[q] = p.x;
       S.y[q] = p.y;
       S.z[q] = p.z;
       S.t[q] = (uint8_t)p.w;
-    }
-  }
+    });
   if (threadIdx.x == 0) {            // padding target: beyond any cutoff
     S.x[WCAP] = S.y[WCAP] = S.z[WCAP] = PAD_FAR;
     S.t[WCAP] = 0;
@@ -396,10 +417,9 @@ __global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint3
   const double oy = R.lo[1] + ((double)G.cj - 1.0) * R.L[1] / R.nb[1];
   const double oz = R.lo[2] + ((double)G.ck0 - 1.0) * R.L[2] / R.nb[2];
   if (staged)
-    for (uint32_t q = threadIdx.x; q < W.total; q += blockDim.x) {
-      const double4 p = window_global(W, R, r, q);
+    for_window_particles(W, R, r, [&](uint32_t q, const double4 &p) {
       S.p[q] = make_float4((float)(p.x - ox), (float)(p.y - oy), (float)(p.z - oz), 0.f);
-    }
+    });
   __syncthreads();
   const uint32_t ntile = W.rpref[W.nrun];
   const float Lx = (float)R.L[0], Ly = (float)R.L[1], Lz = (float)R.L[2];
