@@ -622,6 +622,7 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
     // forces and energies accumulate in FP64 in both modes; MIXED evaluates
     // each pair in FP32 on the FP64 displacement after the FP64 cutoff test
     double fx = 0, fy = 0, fz = 0, pe = 0, vir = 0;
+    float pef = 0.f, virf = 0.f;   // MIXED energies: per-lane FP32 sums of ~20 pairs
     auto pair = [&](uint32_t q) {
       double4 pj;
       if (FAST) pj = make_double4(S.x[q], S.y[q], S.z[q], ONE_TYPE ? 0.0 : (double)S.t[q]);
@@ -657,8 +658,8 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
           const float e4 = ONE_TYPE ? A.e4f : A.lj->e4f[t];
           const float sh = ONE_TYPE ? A.shiftf : A.lj->shiftf[t];
           const float ep = e4 * s6 * (s6 - 1.0f) - sh;
-          pe += (double)(in ? ep : 0.0f);
-          vir = fma(frd, r2d, vir);
+          pef += in ? ep : 0.0f;
+          virf = fmaf(fr, r2, virf);
         }
       } else {
         const double r2 = r2d;
@@ -693,6 +694,10 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
       uint32_t q[4] = {lo & 0xFFFFu, lo >> 16, hi & 0xFFFFu, hi >> 16};
 #pragma unroll
       for (int j = 0; j < 4; ++j) pair(FAST ? min(q[j], PAD) : q[j]);
+    }
+    if (MIXED && ENERGY) {
+      pe = (double)pef;
+      vir = (double)virf;
     }
 #pragma unroll
     for (int o = 2; o > 0; o >>= 1) {
@@ -746,7 +751,7 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
 }
 
 template <bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
-__global__ void __launch_bounds__(TBS) __attribute__((amdgpu_waves_per_eu(4))) k_lj_step(StepArgs A) {
+__device__ __forceinline__ void lj_step_impl(const StepArgs &A) {
   __shared__ Stage S;
   publish_prev(A);
   const uint32_t seg = pb_xcd_block();
@@ -838,6 +843,19 @@ __global__ void __launch_bounds__(TBS) __attribute__((amdgpu_waves_per_eu(4))) k
       }
     }
   }
+}
+
+// Two entry points over the same body: the common variants fit 128 VGPRs
+// (4 wavefronts per SIMD, two 512-thread workgroups per CU); the FP64 and
+// multi-type energy variants need more registers and run without the cap
+// (no spills) at lower occupancy.
+template <bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
+__global__ void __launch_bounds__(TBS) __attribute__((amdgpu_waves_per_eu(4))) k_lj_step(StepArgs A) {
+  lj_step_impl<MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE>(A);
+}
+template <bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
+__global__ void __launch_bounds__(TBS) k_lj_step_wide(StepArgs A) {
+  lj_step_impl<MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE>(A);
 }
 
 // ------------------------------------------------------------- host side
@@ -998,13 +1016,22 @@ static StepArgs make_args(pb_sys *s, double dt) {
 
 typedef void (*StepKernel)(StepArgs);
 
+template <bool MIXED, bool ONE, bool E, bool W, bool I>
+static StepKernel pick1() {
+  // variants that do not fit 128 VGPRs without spilling
+  if constexpr ((E && (!MIXED || !ONE)) || (!MIXED && !ONE))
+    return k_lj_step_wide<MIXED, E, W, ONE, I>;
+  else
+    return k_lj_step<MIXED, E, W, ONE, I>;
+}
+
 template <bool MIXED, bool ONE>
 static StepKernel pick3(bool energy, bool wf, bool integ) {
   if (integ) {
-    if (energy) return wf ? k_lj_step<MIXED, true, true, ONE, true> : k_lj_step<MIXED, true, false, ONE, true>;
-    return wf ? k_lj_step<MIXED, false, true, ONE, true> : k_lj_step<MIXED, false, false, ONE, true>;
+    if (energy) return wf ? pick1<MIXED, ONE, true, true, true>() : pick1<MIXED, ONE, true, false, true>();
+    return wf ? pick1<MIXED, ONE, false, true, true>() : pick1<MIXED, ONE, false, false, true>();
   }
-  return energy ? k_lj_step<MIXED, true, true, ONE, false> : k_lj_step<MIXED, false, true, ONE, false>;
+  return energy ? pick1<MIXED, ONE, true, true, false>() : pick1<MIXED, ONE, false, true, false>();
 }
 
 static StepKernel pick_kernel(bool mixed, bool one, bool energy, bool wf, bool integ) {
