@@ -10,9 +10,10 @@
 // cell order so list neighbours are close in HBM.  Between rebuilds one
 // kernel per step computes the pair forces from the list, integrates
 // (velocity Verlet in its kick-drift form, FP64) and -- on energy steps --
-// reduces KE, PE and the virial per replica in FP64 with a deterministic
-// last-workgroup reduction.  The 81N partial-force buffer, the reduce27 pass
-// and every host synchronisation of the reference loop are gone.
+// writes per-segment KE, PE and virial partials that a one-workgroup kernel
+// sums per replica in FP64 in a fixed order (deterministic).  The 81N
+// partial-force buffer, the reduce27 pass and every host synchronisation of
+// the reference loop are gone.
 #include "pb_internal.h"
 #include <algorithm>
 #include <atomic>
@@ -810,7 +811,7 @@ __device__ __forceinline__ void lj_step_impl(const StepArgs &A) {
     }
     if (ENERGY) {
       // per-segment partials (a segment never spans two replicas), summed
-      // per replica in a fixed order by the last workgroup to finish
+      // per replica in a fixed order by k_energy_reduce after this kernel
       double acc[3] = {0, 0, 0};
       for (uint32_t w = 0; w < nw; ++w)
         for (int c = 0; c < 3; ++c) acc[c] += sen[c][w];
@@ -818,29 +819,6 @@ __device__ __forceinline__ void lj_step_impl(const StepArgs &A) {
       P[0] = acc[0];
       P[1] = acc[1];
       P[2] = acc[2];
-    }
-  }
-  if (ENERGY) {
-    __shared__ bool last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      __threadfence();
-      const uint32_t done = atomicAdd(&A.counter[CNT_LAST], 1u);
-      last = (done == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (last) {
-      __threadfence();
-      const uint32_t rec = __hip_atomic_load(&A.counter[CNT_REC], __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT);
-      pb_block_reduce_reps(A.partials, 3, A.reps, A.nrep, A.energy + (size_t)rec * 8 * A.nrep,
-                           true, S.x);   // the stage is no longer needed
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        A.counter[CNT_REC] = rec + 1;
-        A.counter[CNT_LAST] = 0;
-        __threadfence();
-      }
     }
   }
 }
@@ -1014,6 +992,19 @@ static StepArgs make_args(pb_sys *s, double dt) {
   return A;
 }
 
+// Energy record of a step: per-replica sums of the per-segment partials in a
+// fixed order (deterministic), into record counter[CNT_REC], which is then
+// advanced.  A separate launch: stream order makes the partials visible.
+__global__ void __launch_bounds__(256) k_energy_reduce(const double *partials, const RepInfo *reps,
+                                                       uint32_t nrep, double *energy,
+                                                       uint32_t *counter) {
+  __shared__ double sm[4 * 256];
+  const uint32_t rec = counter[CNT_REC];
+  pb_block_reduce_reps(partials, 3, reps, nrep, energy + (size_t)rec * 8 * nrep, true, sm);
+  __syncthreads();
+  if (threadIdx.x == 0) counter[CNT_REC] = rec + 1;
+}
+
 typedef void (*StepKernel)(StepArgs);
 
 template <bool MIXED, bool ONE, bool E, bool W, bool I>
@@ -1044,6 +1035,9 @@ static int launch_step(pb_sys *s, const StepArgs &A, bool mixed, bool energy, bo
   const bool one = s->lj.ntypes == 1;
   StepKernel k = pick_kernel(mixed, one, energy, wf, integ);
   hipLaunchKernelGGL(k, dim3(s->list_nsegs), dim3(TBS), 0, s->ctx->stream, A);
+  if (energy)
+    hipLaunchKernelGGL(k_energy_reduce, dim3(1), dim3(256), 0, s->ctx->stream, A.partials, A.reps,
+                       A.nrep, A.energy, A.counter);
   PB_LAUNCH_CHECK(s->ctx, "k_lj_step");
   return PB_OK;
 }
