@@ -284,6 +284,36 @@ int pb_dem_get_contacts(pb_sys *sys, uint32_t *rowptr, uint32_t *colidx, double 
  * Cundall-Strack bound (counted even when autotune allowed the step). */
 int pb_dem_last_violations(pb_sys *sys, uint32_t *count);
 
+/* ------------------------------------------------------------ thermostats */
+/* GaussianThermo::TimeEvolution (GaussianThermo.cu:29-83; kernels
+ * kernelfuncs.cu:372-477): isokinetic velocity Verlet on (r, v, Fold = a)
+ * with F(t) in F.  The implicit velocity equation the reference solves by
+ * Newton-Raphson has the closed-form solution v(t) = A (1 - dt/2 g),
+ * xi(t) = g / (1 - dt/2 g), g = sum F.A / sum m A.A per replica, which is
+ * the loop's converged result.  tol is the loop's threshold (setE): when
+ * its first pass, taken with the previous xi, moves v by an rms of at most
+ * tol (always when that xi is 0) the reference stops after that pass, and
+ * so does this step.  No host synchronisation.  xi of every replica
+ * persists on the device (GaussianThermo::xi, starts at 0). */
+int pb_gt_step(pb_sys *sys, double dt, double tol);
+/* GaussianThermo::xi (GaussianThermo.hh:16): get_xi[nrep] (may be NULL)
+ * receives the current values, then set_xi[nrep] (may be NULL) replaces
+ * them (restart).  bad_steps (may be NULL) counts the steps since the last
+ * set whose implicit equation had no solution (1 - dt/2 g <= 0; such a step
+ * is taken with xi = 0, where the reference's Newton loop exits). */
+int pb_gt_xi(pb_sys *sys, const double *set_xi, double *get_xi, uint32_t *bad_steps);
+/* adjustVelocities (cudaParticleMD.cu:177-198, GaussianThermo.cu:98-119;
+ * adjustVelocity_F4/_VD kernelfuncs.cu:546-699): per replica and axis,
+ * v = (v - mean) v1 / sqrt(var); v1 = sqrt(kB T / m0) is the caller's.
+ * stats (may be NULL) receives 6 per replica: the means and variances
+ * before the adjustment.  A component with zero variance is an error. */
+int pb_adjust_velocities(pb_sys *sys, double v1, double *stats);
+/* cudaParticleMD::constTemp (cudaParticleMD.cu:140-155, correctConstTemp_F4
+ * kernelfuncs.cu:533-544): lambda = sum F.v / sum m v^2 per replica,
+ * F -= lambda m v.  mv2[nrep], lambda[nrep] (may be NULL) receive sum m v^2
+ * and lambda (the reference returns T = mv2 / 3 N kB). */
+int pb_const_temp(pb_sys *sys, double *mv2, double *lambda);
+
 #ifdef __cplusplus
 }
 #endif

@@ -191,6 +191,18 @@ uint32_t pbo_dem_force(const double *r, const double *v, const double *w, const 
 void pbo_rot_step(double *w, double *L, const double *T, const double *m0inv,
                   const uint8_t *move, uint32_t n, double dt);
 
+/* ------------------------------------------------------------- thermostats */
+/* GaussianThermo / adjustVelocities / constTemp restated in FP64
+ * (pb_oracle_thermo.c); xyz-interleaved arrays.
+ * pinned by: the closed-form Gaussian fixed point, the isokinetic property,
+ * zero mean / target variance, sum F.v = 0 after constTemp
+ * (tests/test_oracle_thermo.py). */
+int pbo_gt_step(double *r, double *v, const double *F, double *Fold, const double *minv,
+                uint32_t n, double dt, double *xi_io, double tol);
+void pbo_adjust_velocities(double *v, uint32_t n, double v1, double *stats);
+double pbo_const_temp(const double *v, double *F, const double *minv, uint32_t n,
+                      double *lambda_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,7 +62,8 @@ def lib():
 
 
 def _extra(L):
-    """Signatures of the SPH / DEM restatements (pb_oracle_sph.c / _dem.c)."""
+    """Signatures of the SPH / DEM / thermostat restatements (pb_oracle_sph.c,
+    _dem.c, _thermo.c)."""
     P = C.POINTER
     u32, d, u8 = C.c_uint32, C.c_double, C.c_uint8
     for name, res, args in [
@@ -77,6 +78,9 @@ def _extra(L):
         ("pbo_dem_force", u32, [P(d), P(d), P(d), P(d), P(d), P(u8), u32, P(u32), P(u32),
                                 P(d), d, d, d, d, d, d, P(d), P(d)]),
         ("pbo_rot_step", None, [P(d), P(d), P(d), P(d), P(u8), u32, d]),
+        ("pbo_gt_step", C.c_int, [P(d), P(d), P(d), P(d), P(d), u32, d, P(d), d]),
+        ("pbo_adjust_velocities", None, [P(d), u32, d, P(d)]),
+        ("pbo_const_temp", d, [P(d), P(d), P(d), u32, P(d)]),
     ]:
         fn = getattr(L, name)
         fn.restype = res
@@ -353,3 +357,33 @@ def rot_step(w, L, T, m0inv, move, dt):
     lib().pbo_rot_step(_p(w, d), _p(L, d), _p(T, d), _p(m0inv, d), _p(mv, C.c_uint8),
                        w.shape[0], dt)
     return w, L
+
+
+# ----------------------------------------------------------- thermostats
+def gt_step(r, v, F, Fold, minv, dt, xi, tol=1e-14):
+    """GaussianThermo::TimeEvolution (Newton-Raphson form) -> r, v, Fold, xi,
+    Newton passes (-1: not converged)."""
+    r, v, F, Fold, minv = f64(r).copy(), f64(v).copy(), f64(F), f64(Fold).copy(), f64(minv)
+    d = C.c_double
+    x = d(xi)
+    it = lib().pbo_gt_step(_p(r, d), _p(v, d), _p(F, d), _p(Fold, d), _p(minv, d), minv.size,
+                           dt, C.byref(x), tol)
+    return r, v, Fold, x.value, it
+
+
+def adjust_velocities(v, v1):
+    """adjustVelocity_F4 / _VD -> v, stats (means[3], variances[3])."""
+    v = f64(v).copy()
+    st = np.zeros(6)
+    d = C.c_double
+    lib().pbo_adjust_velocities(_p(v, d), v.shape[0], v1, _p(st, d))
+    return v, st
+
+
+def const_temp(v, F, minv):
+    """cudaParticleMD::constTemp -> corrected F, sum m v^2, lambda."""
+    v, F, minv = f64(v), f64(F).copy(), f64(minv)
+    d = C.c_double
+    lam = d()
+    mv2 = lib().pbo_const_temp(_p(v, d), _p(F, d), _p(minv, d), minv.size, C.byref(lam))
+    return F, mv2, lam.value

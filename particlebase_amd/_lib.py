@@ -47,6 +47,7 @@ EXPORTS = [
     "pb_sph_set_properties", "pb_sph_density", "pb_sph_acceleration", "pb_sph_inspect_density",
     "pb_dem_set_properties", "pb_rot_set_inertia", "pb_rot_step", "pb_dem_calc_force",
     "pb_dem_get_contacts", "pb_dem_last_violations",
+    "pb_gt_step", "pb_gt_xi", "pb_adjust_velocities", "pb_const_temp",
 ]
 
 
@@ -132,7 +133,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
 
 
 def _extra_signatures():
-    """Signatures of the SPH / DEM entry points."""
+    """Signatures of the SPH / DEM / thermostat entry points."""
     vp, u32, i32, d = C.c_void_p, C.c_uint32, C.c_int, C.c_double
     P = C.POINTER
     return {
@@ -143,6 +144,10 @@ def _extra_signatures():
         "pb_dem_set_properties": (i32, [vp, d, d, d, d, d, P(d), d]),
         "pb_rot_set_inertia": (i32, [vp, P(d), d]),
         "pb_rot_step": (i32, [vp, d]),
+        "pb_gt_step": (i32, [vp, d, d]),
+        "pb_gt_xi": (i32, [vp, P(d), P(d), P(u32)]),
+        "pb_adjust_velocities": (i32, [vp, d, P(d)]),
+        "pb_const_temp": (i32, [vp, P(d), P(d)]),
         "pb_dem_calc_force": (i32, [vp, d, i32, P(u32)]),
         "pb_dem_get_contacts": (i32, [vp, P(u32), P(u32), P(d), u32, P(u32)]),
         "pb_dem_last_violations": (i32, [vp, P(u32)]),
@@ -321,6 +326,35 @@ class System:
 
     def scale_velocities(self, s):
         self._c(self.lib.pb_scale_velocities(self.h, s))
+
+    # --- thermostats (GaussianThermo, adjustVelocities, constTemp)
+    def gt_step(self, dt, tol=1e-8):
+        self._c(self.lib.pb_gt_step(self.h, dt, tol))
+
+    def gt_xi(self, set_xi=None):
+        """-> (xi per replica, bad step count); set_xi replaces xi afterwards."""
+        out = np.empty(self.nrep, dtype=np.float64)
+        bad = C.c_uint32()
+        sx = None
+        if set_xi is not None:
+            sx = np.ascontiguousarray(np.broadcast_to(np.asarray(set_xi, dtype=np.float64),
+                                                      (self.nrep,)))
+        self._c(self.lib.pb_gt_xi(self.h, None if sx is None else _p(sx, C.c_double),
+                                  _p(out, C.c_double), C.byref(bad)))
+        return out, bad.value
+
+    def adjust_velocities(self, v1):
+        """-> stats (nrep, 6): means and variances before the adjustment."""
+        st = np.empty((self.nrep, 6), dtype=np.float64)
+        self._c(self.lib.pb_adjust_velocities(self.h, v1, _p(st, C.c_double)))
+        return st
+
+    def const_temp(self):
+        """-> (sum m v^2, lambda) per replica; F corrected in place."""
+        mv2 = np.empty(self.nrep, dtype=np.float64)
+        lam = np.empty(self.nrep, dtype=np.float64)
+        self._c(self.lib.pb_const_temp(self.h, _p(mv2, C.c_double), _p(lam, C.c_double)))
+        return mv2, lam
 
     def inspect_velocity(self, vlim, lim_u, lim_l):
         res = C.c_int()

@@ -7,6 +7,7 @@
 // counterpart as one call).
 #pragma once
 #include <cmath>
+#include <iostream>
 #include <numeric>
 #include <vector>
 #include "cudaCutoffBlock.hh"
@@ -79,11 +80,36 @@ public:
   double lastVirialW(void) const { return lastVirial; }
 
   // scaleTemp, cudaParticleMD.cu:168-175
-  real scaleTemp(real Temp) {
+  virtual real scaleTemp(real Temp) {
     const real T0 = calcTemp();
     const real s = std::sqrt(static_cast<real>(3 * N - 1) / static_cast<real>(3 * N) * Temp / T0);
     pbamd::check(pb_scale_velocities(sys, s), "scaleTemp");
     return s;
+  }
+
+  // adjustVelocities, cudaParticleMD.cu:177-198: zero mean and variance
+  // kB T / m0 per axis; debug prints the statistics before the adjustment
+  virtual void adjustVelocities(real Temp, bool debug = false) {
+    const double v1 = std::sqrt((double)kB * Temp / m0);
+    double st[6];
+    if (debug)
+      std::cerr << std::endl << std::endl << "adjustVelocity currentTemp\t" << calcTemp()
+                << "\tTarget Temp " << Temp << std::endl;
+    pbamd::check(pb_adjust_velocities(sys, v1, st), "adjustVelocities");
+    if (debug) {
+      std::cerr << "velocity statistics";
+      for (int i = 0; i < 6; ++i) std::cerr << "\t" << st[i];
+      std::cerr << std::endl << "after adjusted T=" << calcTemp() << std::endl;
+    }
+  }
+
+  // constTemp, cudaParticleMD.cu:140-155: F -= lambda m v with
+  // lambda = sum F.v / sum m v^2; returns the temperature
+  real constTemp(void) {
+    double m2 = 0, lam = 0;
+    pbamd::check(pb_const_temp(sys, &m2, &lam), "constTemp");
+    mv2 = (real)m2;
+    return (real)(m2 / (3.0 * N * kB));
   }
 
   // setLJparams, cudaParticleMD.cu:200-202 (sigma, epsilon per type pair)

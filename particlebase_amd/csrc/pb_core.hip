@@ -119,7 +119,7 @@ extern "C" int pb_sys_create(pb_ctx *ctx, uint32_t nrep, const uint32_t *counts,
       (rc = pb_alloc(ctx, &s->orig, np)) || (rc = pb_alloc(ctx, &s->waveRep, s->nwave)) ||
       (rc = pb_alloc(ctx, &s->d_reps, nrep)) ||
       (rc = pb_alloc(ctx, &s->partials, 4 * (size_t)s->nwave)) ||
-      (rc = pb_alloc(ctx, &s->d_counter, 16)) || (rc = pb_alloc(ctx, &s->d_out, 8 * (size_t)nrep)) ||
+      (rc = pb_alloc(ctx, &s->d_counter, 16)) || (rc = pb_alloc(ctx, &s->d_out, 16 * (size_t)nrep)) ||
       (rc = pb_alloc(ctx, &s->d_lj, 1))) {
     pb_sys_destroy(s);
     return rc;
@@ -161,6 +161,7 @@ extern "C" void pb_sys_destroy(pb_sys *s) {
   pb_free(s->w); pb_free(s->Lm); pb_free(s->T); pb_free(s->radius);
   pb_free(s->move); pb_free(s->type); pb_free(s->orig); pb_free(s->waveRep);
   pb_free(s->d_reps); pb_free(s->partials); pb_free(s->d_counter); pb_free(s->d_out);
+  pb_free(s->gt); pb_free(s->gt_bad);
   pb_free(s->ref_table); pb_free(s->ref_bid); pb_free(s->ref_pid); pb_free(s->ref_bindex);
   pb_free(s->cellid); pb_free(s->rank); pb_free(s->perm); pb_free(s->ccount);
   pb_free(s->cstart); pb_free(s->scan_tmp); pb_free(s->d_lj);

@@ -73,7 +73,7 @@ struct pb_sys {
   RepInfo *d_reps = nullptr;
   double *partials = nullptr;     // per-wave partial sums, 4 per wave
   uint32_t *d_counter = nullptr;  // last-block counters / flags (16 words)
-  double *d_out = nullptr;        // small reduction outputs (8 per replica)
+  double *d_out = nullptr;        // small reduction outputs (16 per replica)
 
   // reference cell grid (pb_setup_cells), single replica
   bool ref_cells = false;
@@ -123,6 +123,11 @@ struct pb_sys {
   // host-mapped ring through which step kernels publish the previous step's
   // max displacement / speed (drives the rebuild schedule without a sync)
   uint32_t *hmon = nullptr, *dmon = nullptr;
+
+  // Gaussian thermostat: per replica {xi, 1/c} of the last step, and the
+  // count of steps whose implicit velocity equation had no solution
+  double *gt = nullptr;
+  uint32_t *gt_bad = nullptr;
 
   // g(r)
   uint64_t *gr_hist = nullptr;

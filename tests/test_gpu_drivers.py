@@ -1,6 +1,7 @@
 """The C++ drivers (particlebase_amd/cpp/drivers) run the reference's driver
 call sequences through the class layer and the C ABI on the GPU: LJ
-(testMD1.cu order, per-call API and fused), SPH dam break (testSPH5.cu
+(testMD1.cu order, per-call API and fused), liquid argon under the Gaussian
+thermostat (testMD2.cu order), SPH dam break (testSPH5.cu
 order with AdaptiveTime) and DEM settle (testDEM5.cu order with
 AdaptiveTime).  Each must finish and report a physically sane end state."""
 import os
@@ -45,3 +46,15 @@ def test_dem_settle():
     d = run("dem_settle", 3000, 200)
     assert int(d["steps"]) == 200
     assert 0.0 < float(d["zmean"]) < 0.4
+
+
+def test_md_gauss():
+    # testMD2 order: liquid argon under the Gaussian thermostat
+    d = run("md_gauss", 300)
+    assert int(d["N"]) == 4924 and int(d["steps"]) == 300
+    assert int(d["failed"]) == 0 and int(d["scaled"]) == 0
+    # isokinetic: T after the first thermostatted step is held
+    t0 = float(d["T0"])
+    assert 70.0 < t0 < 100.0
+    assert abs(float(d["T1"]) / t0 - 1) < 0.01
+    assert float(d["Tmax"]) / float(d["Tmin"]) < 1.02
