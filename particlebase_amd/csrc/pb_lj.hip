@@ -191,9 +191,9 @@ struct Stage {              // pair kernel: FP64 positions with shifts applied
   uint8_t t[WCAP + 1];
   WinTable W;
 };
-#define WCAPB 4096          // build kernel stages FP32 (16 B) particles
+#define WCAPB WCAP          // build kernel stage (FP32, 16 B a particle): three workgroups per CU
 struct StageB {             // list build: FP32 window-relative positions
-  float4 p[WCAPB];
+  float4 p[WCAPB + 8];       // 8 slack entries read by the last group of tests
   WinTable W;
 };
 
@@ -429,55 +429,84 @@ __global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint3
   for (uint32_t u = threadIdx.x; u < ntile; u += blockDim.x) {
     uint32_t li, gi, run;
     tile_particle(W, u, li, gi, run);
+    // Kept entries are packed four to a 64-bit word in registers; word m of
+    // slot gi is written whole at wrow[m * npad].  Words past the capacity
+    // land in the slack word wcap (the allocation holds maxnb + 16 entries
+    // per slot); an overflow is reported and the list rebuilt larger.
+    uint64_t *wrow = reinterpret_cast<uint64_t *>(nl) + gi;
+    const uint32_t wcap = maxnb >> 2;
+    uint64_t word = 0;
     uint32_t cnt = 0;
-    // entry k of slot gi: u16 element ((k / 4) * npad + gi) * 4 + k % 4;
-    // wp points at the word of entry cnt
-    uint16_t *row = nl + 4 * (size_t)gi;
-    uint16_t *wp = row;
-    const size_t wstride = 4 * (size_t)npad;
-    auto keep = [&](uint32_t q) {
-      if (cnt < maxnb) wp[cnt & 3u] = (uint16_t)q;
+    auto append = [&](uint32_t q) {
+      word |= (uint64_t)q << (16u * (cnt & 3u));
       ++cnt;
-      if ((cnt & 3u) == 0) wp += wstride;
+      if ((cnt & 3u) == 0u) {
+        wrow[(size_t)min((cnt >> 2) - 1u, wcap) * npad] = word;
+        word = 0;
+      }
     };
+    // staged candidates [q0, q1): 32 distance tests into a bit mask, then
+    // the hits appended in order (the append runs once per hit, not once
+    // per candidate)
+    auto scan_staged = [&](uint32_t q0, uint32_t q1, bool mi) {
+      const float4 pi = S.p[li];
+      auto near = [&](const float4 &pj) {
+        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        if (mi) {
+          dx -= Lx * rintf(dx * Ix);
+          dy -= Ly * rintf(dy * Iy);
+          dz -= Lz * rintf(dz * Iz);
+        }
+        return dx * dx + dy * dy + dz * dz < rb2;
+      };
+      for (uint32_t c0 = q0; c0 < q1; c0 += 32) {
+        uint32_t m = 0;
+        if (c0 + 32 <= q1) {
+#pragma unroll 8
+          for (uint32_t k = 0; k < 32; ++k) m |= (near(S.p[c0 + k]) ? 1u : 0u) << k;
+        } else {
+          // tail: groups of 8 tests; a group may read up to 7 entries past
+          // q1 (the stage's slack), whose bits are cleared
+          for (uint32_t j = 0; j < q1 - c0; j += 8) {
+#pragma unroll
+            for (uint32_t k = 0; k < 8; ++k) m |= (near(S.p[c0 + j + k]) ? 1u : 0u) << (j + k);
+          }
+          m &= (1u << (q1 - c0)) - 1u;
+        }
+        while (m) {
+          append(c0 + (uint32_t)__builtin_ctz(m));
+          m &= m - 1u;
+        }
+      }
+    };
+    // window read from global memory (window larger than the stage)
+    auto scan_global = [&](uint32_t q0, uint32_t q1, bool mi) {
+      const double4 pi = window_global(W, R, r, li);
+      for (uint32_t q = q0; q < q1; ++q) {
+        const double4 pj = window_global(W, R, r, q);
+        double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        if (mi) {
+          dx -= R.L[0] * rint(dx * R.Linv[0]);
+          dy -= R.L[1] * rint(dy * R.Linv[1]);
+          dz -= R.L[2] * rint(dz * R.Linv[2]);
+        }
+        if (dx * dx + dy * dy + dz * dz < rlist2) append(q);
+      }
+    };
+    // candidates [q0, q1) except the particle itself
     auto scan = [&](uint32_t q0, uint32_t q1, bool mi) {
-      if (staged) {
-        const float4 pi = S.p[li];
-        auto near = [&](const float4 &pj) {
-          float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-          if (mi) {
-            dx -= Lx * rintf(dx * Ix);
-            dy -= Ly * rintf(dy * Iy);
-            dz -= Lz * rintf(dz * Iz);
-          }
-          return dx * dx + dy * dy + dz * dz < rb2;
-        };
-        // four candidates per iteration: their LDS reads are issued together
-        uint32_t q = q0;
-        for (; q + 4 <= q1; q += 4) {
-          const float4 a = S.p[q], b = S.p[q + 1], c = S.p[q + 2], d = S.p[q + 3];
-          const bool ha = near(a) && q != li, hb = near(b) && q + 1 != li,
-                     hc = near(c) && q + 2 != li, hd = near(d) && q + 3 != li;
-          if (ha) keep(q);
-          if (hb) keep(q + 1);
-          if (hc) keep(q + 2);
-          if (hd) keep(q + 3);
+      if (li >= q0 && li < q1) {
+        if (staged) {
+          scan_staged(q0, li, mi);
+          scan_staged(li + 1, q1, mi);
+        } else {
+          scan_global(q0, li, mi);
+          scan_global(li + 1, q1, mi);
         }
-        for (; q < q1; ++q)
-          if (near(S.p[q]) && q != li) keep(q);
+      } else if (staged) {
+        scan_staged(q0, q1, mi);
       } else {
-        const double4 pi = window_global(W, R, r, li);
-        for (uint32_t q = q0; q < q1; ++q) {
-          if (q == li) continue;
-          const double4 pj = window_global(W, R, r, q);
-          double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-          if (mi) {
-            dx -= R.L[0] * rint(dx * R.Linv[0]);
-            dy -= R.L[1] * rint(dy * R.Linv[1]);
-            dz -= R.L[2] * rint(dz * R.Linv[2]);
-          }
-          if (dx * dx + dy * dy + dz * dz < rlist2) keep(q);
-        }
+        scan_global(q0, q1, mi);
       }
     };
     if (G.whole) {
@@ -495,10 +524,13 @@ __global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint3
           scan(W.eoff[e0], W.eoff[e0 + 3], false);   // three z slots
         }
     }
-    // pad to whole batches of 16 with the inert entry
+    // close the partial word and pad to whole batches of 16 with the inert
+    // entry (nn holds the padded length)
     const uint32_t used = cnt < maxnb ? cnt : maxnb;
     const uint32_t padded = (used + 15u) & ~15u;
-    for (uint32_t k = used; k < padded; ++k) row[(size_t)(k >> 2) * 4 * npad + (k & 3u)] = (uint16_t)NL_PAD;
+    if ((cnt & 3u) != 0u && cnt < maxnb)
+      wrow[(size_t)(cnt >> 2) * npad] = word | (~0ull << (16u * (cnt & 3u)));
+    for (uint32_t m = (used + 3u) >> 2; m < (padded >> 2); ++m) wrow[(size_t)m * npad] = ~0ull;
     nn[gi] = padded;
     if (cnt > maxnb) atomicMax(&counter[CNT_OVF], cnt);
     mymax = max(mymax, cnt);
