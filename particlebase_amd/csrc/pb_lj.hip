@@ -192,8 +192,9 @@ struct Stage {              // pair kernel: FP64 positions with shifts applied
   WinTable W;
 };
 #define WCAPB WCAP          // build kernel stage (FP32, 16 B a particle): three workgroups per CU
-struct StageB {             // list build: FP32 window-relative positions
-  float4 p[WCAPB + 8];       // 8 slack entries read by the last group of tests
+struct StageB {             // list build: FP32 window-relative positions, 12 B a
+  float2 xy[WCAPB + 4];      // particle (four workgroups per CU); 4 slack
+  float z[WCAPB + 4];        // entries read by the last group of tests
   WinTable W;
 };
 
@@ -419,7 +420,8 @@ __global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint3
   const double oz = R.lo[2] + ((double)G.ck0 - 1.0) * R.L[2] / R.nb[2];
   if (staged)
     for_window_particles(W, R, r, [&](uint32_t q, const double4 &p) {
-      S.p[q] = make_float4((float)(p.x - ox), (float)(p.y - oy), (float)(p.z - oz), 0.f);
+      S.xy[q] = make_float2((float)(p.x - ox), (float)(p.y - oy));
+      S.z[q] = (float)(p.z - oz);
     });
   __syncthreads();
   const uint32_t ntile = W.rpref[W.nrun];
@@ -449,9 +451,11 @@ __global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint3
     // the hits appended in order (the append runs once per hit, not once
     // per candidate)
     auto scan_staged = [&](uint32_t q0, uint32_t q1, bool mi) {
-      const float4 pi = S.p[li];
-      auto near = [&](const float4 &pj) {
-        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      const float2 pixy = S.xy[li];
+      const float piz = S.z[li];
+      auto near = [&](uint32_t q) {
+        const float2 pj = S.xy[q];
+        float dx = pixy.x - pj.x, dy = pixy.y - pj.y, dz = piz - S.z[q];
         if (mi) {
           dx -= Lx * rintf(dx * Ix);
           dy -= Ly * rintf(dy * Iy);
@@ -462,14 +466,14 @@ __global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint3
       for (uint32_t c0 = q0; c0 < q1; c0 += 32) {
         uint32_t m = 0;
         if (c0 + 32 <= q1) {
-#pragma unroll 8
-          for (uint32_t k = 0; k < 32; ++k) m |= (near(S.p[c0 + k]) ? 1u : 0u) << k;
+#pragma unroll 4
+          for (uint32_t k = 0; k < 32; ++k) m |= (near(c0 + k) ? 1u : 0u) << k;
         } else {
-          // tail: groups of 8 tests; a group may read up to 7 entries past
+          // tail: groups of 4 tests; a group may read up to 3 entries past
           // q1 (the stage's slack), whose bits are cleared
-          for (uint32_t j = 0; j < q1 - c0; j += 8) {
+          for (uint32_t j = 0; j < q1 - c0; j += 4) {
 #pragma unroll
-            for (uint32_t k = 0; k < 8; ++k) m |= (near(S.p[c0 + j + k]) ? 1u : 0u) << (j + k);
+            for (uint32_t k = 0; k < 4; ++k) m |= (near(c0 + j + k) ? 1u : 0u) << (j + k);
           }
           m &= (1u << (q1 - c0)) - 1u;
         }
