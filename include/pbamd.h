@@ -314,6 +314,32 @@ int pb_adjust_velocities(pb_sys *sys, double v1, double *stats);
  * and lambda (the reference returns T = mv2 / 3 N kB). */
 int pb_const_temp(pb_sys *sys, double *mv2, double *lambda);
 
+/* ------------------------------------------- MD neighbour list, annealing */
+/* cudaParticleMD::makeNList (cudaParticleMD.cu:338-394; calcCoordMD_F4 and
+ * makeJlist_WithBlock_F4 with MDNeighborRegister, kerneltemplate.hh:725-775,
+ * kernelinline.hh:387-412): CSR list over the 27-cell grid of
+ * pb_setup_cells after pb_bin, rows and columns by original index, every
+ * j != i with FP32 distance2p < rmax2t (rmax2t = (sqrt(rmax2) + thickness)^2),
+ * columns ascending.  nnz_out (may be NULL) receives the entry count. */
+int pb_nlist_build(pb_sys *sys, float rmax2t, uint32_t *nnz_out);
+/* cudaSparseMat rowPtr / colIdx of the last pb_nlist_build: rowptr[n+1];
+ * colidx needs cap >= nnz (either may be NULL). */
+int pb_nlist_get(pb_sys *sys, uint32_t *rowptr, uint32_t *colidx, uint32_t cap, uint32_t *nnz);
+/* cudaParticleMD::calcForce with useNList (cudaParticleMD.cu:20-40,
+ * calcF_IJpairWithList_F4 kerneltemplate.hh:136-168 with MDpairForce
+ * kerneltemplate.hh:524-565): FP64 LJ forces (F overwritten) from the list's
+ * pairs with r^2 below the cutoff of pb_lj_set_cutoff, minimum image;
+ * pe, virial (may be NULL) = 1/2 sum phi, 1/2 sum f.r.  Single replica. */
+int pb_lj_force_nlist(pb_sys *sys, double *pe, double *virial);
+/* The soft-core force of cudaParticleMD::initialAnnealing
+ * (cudaParticleMD.cu:205-283; potentialSC kernelinline.hh:140-158 through
+ * MDpairForce): F_i = sum_j f0 cos(r/rc pi/2) (r_i - r_j)/r over pairs of the
+ * 27-cell grid (after pb_bin) with r <= rc and r^2 < rmax2; F overwritten. */
+int pb_softcore_force(pb_sys *sys, double rc, double f0, double rmax2);
+/* propagateEuler_F4 (kernelfuncs.cu:174-192): r += v dt, v += a dt for
+ * movable particles. */
+int pb_euler_step(pb_sys *sys, double dt);
+
 #ifdef __cplusplus
 }
 #endif

@@ -441,6 +441,147 @@ int pbo_coord_counts_f32(const float *r, uint32_t n, const float cell[6],
   return 0;
 }
 
+static int cmp_u32(const void *a, const void *b) {
+  const uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+  return (x > y) - (x < y);
+}
+
+int pbo_nlist_f32(const float *r, uint32_t n, const float cell[6], float rmax_cells,
+                  float rmax2t, uint32_t *rowptr, uint32_t *colidx, uint32_t cap,
+                  uint32_t *nnz) {
+  /* makeNList, cudaParticleMD.cu:338-394: counts as pbo_coord_counts_f32,
+   * then the same traversal fills each row (makeJlist_WithBlock_F4,
+   * kerneltemplate.hh:725-775); rows sorted ascending. */
+  uint32_t *cnt = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+  if (pbo_coord_counts_f32(r, n, cell, rmax_cells, rmax2t, cnt)) {
+    free(cnt);
+    return -1;
+  }
+  rowptr[0] = 0;
+  for (uint32_t i = 0; i < n; ++i) rowptr[i + 1] = rowptr[i] + cnt[i];
+  free(cnt);
+  *nnz = rowptr[n];
+  if (!colidx || cap < rowptr[n]) return 0;
+  uint32_t bn[3];
+  float bl[6];
+  pbo_setup_cells(cell, rmax_cells, 1, bn, bl);
+  const uint32_t ncell = bn[0] * bn[1] * bn[2];
+  uint32_t *bid = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t *pid = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t *bs = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t *bindex = (uint32_t *)malloc(sizeof(uint32_t) * (ncell + 1));
+  uint32_t *bend = (uint32_t *)malloc(sizeof(uint32_t) * (ncell + 1));
+  uint32_t *table = (uint32_t *)malloc(sizeof(uint32_t) * (size_t)ncell * 27);
+  pbo_calc_bid(r, n, bl, bn, bid);
+  pbo_sort_bins(bid, n, ncell, pid, bs, bindex);
+  {
+    uint32_t next = n;
+    for (long c = (long)ncell - 1; c >= 0; --c) {
+      bend[c] = next;
+      if (bindex[c] != UINT32_MAX) next = bindex[c];
+    }
+  }
+  pbo_neighbor_table(bn, 1, 1, table);
+  float c[3];
+  half_cell2(cell, c);
+#pragma omp parallel for schedule(static)
+  for (long I = 0; I < (long)ncell; ++I) {
+    if (bindex[I] == UINT32_MAX) continue;
+    for (uint32_t i0 = bindex[I]; i0 < bend[I]; ++i0) {
+      const uint32_t i = pid[i0];
+      uint32_t k = rowptr[i];
+      for (int J27 = 0; J27 < 27; ++J27) {
+        const uint32_t J = table[I * 27 + J27];
+        if (bindex[J] == UINT32_MAX) continue;
+        for (uint32_t j0 = bindex[J]; j0 < bend[J]; ++j0) {
+          const uint32_t j = pid[j0];
+          if (j == i) continue;
+          const float R2 = pbo_distance2p(r[3 * i], r[3 * i + 1], r[3 * i + 2],
+                                          r[3 * j], r[3 * j + 1], r[3 * j + 2],
+                                          c[0], c[1], c[2]);
+          if (R2 < rmax2t) colidx[k++] = j;
+        }
+      }
+      qsort(colidx + rowptr[i], rowptr[i + 1] - rowptr[i], sizeof(uint32_t), cmp_u32);
+    }
+  }
+  free(bid); free(pid); free(bs); free(bindex); free(bend); free(table);
+  return 0;
+}
+
+void pbo_lj_forces_list(const double *r, const uint16_t *type, uint32_t n,
+                        const double cell[6], double rc2, const pbo_lj_params *p,
+                        const uint32_t *rowptr, const uint32_t *colidx, double *F,
+                        double *pe, double *virial) {
+  /* calcF_IJpairWithList_F4, kerneltemplate.hh:136-168, FP64 */
+  const double L[3] = {cell[1] - cell[0], cell[3] - cell[2], cell[5] - cell[4]};
+  double pe_s = 0.0, w_s = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : pe_s, w_s)
+  for (long i = 0; i < (long)n; ++i) {
+    double f[3] = {0, 0, 0}, ph = 0, w = 0;
+    for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+      const uint32_t j = colidx[k];
+      double d[3];
+      for (int a = 0; a < 3; ++a) {
+        d[a] = r[3 * i + a] - r[3 * j + a];
+        d[a] -= L[a] * nearbyint(d[a] / L[a]);
+      }
+      const double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (R2 < rc2) {
+        double phi;
+        const double fr = lj_pair(R2, type[i] * PBO_MAXTYPES + type[j], p, &phi);
+        for (int a = 0; a < 3; ++a) f[a] += fr * d[a];
+        ph += phi;
+        w += fr * R2;
+      }
+    }
+    for (int a = 0; a < 3; ++a) F[3 * i + a] = f[a];
+    pe_s += ph;
+    w_s += w;
+  }
+  if (pe) *pe = 0.5 * pe_s;
+  if (virial) *virial = 0.5 * w_s;
+}
+
+void pbo_softcore_forces(const double *r, uint32_t n, const double cell[6], double rc,
+                         double f0, double rmax2, double *F) {
+  /* potentialSC::F (kernelinline.hh:152-157) through MDpairForce
+   * (kerneltemplate.hh:530-564), FP64, all pairs (minimum image) */
+  const double L[3] = {cell[1] - cell[0], cell[3] - cell[2], cell[5] - cell[4]};
+  const double pi2 = 0.5 * 3.14159265358979323846;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < (long)n; ++i) {
+    double f[3] = {0, 0, 0};
+    for (uint32_t j = 0; j < n; ++j) {
+      if ((long)j == i) continue;
+      double d[3];
+      for (int a = 0; a < 3; ++a) {
+        d[a] = r[3 * i + a] - r[3 * j + a];
+        d[a] -= L[a] * nearbyint(d[a] / L[a]);
+      }
+      const double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (!(R2 < rmax2)) continue;
+      const double R = sqrt(R2);
+      if (R > rc) continue;
+      const double fr = f0 * cos(R / rc * pi2) / R;
+      for (int a = 0; a < 3; ++a) f[a] += fr * d[a];
+    }
+    for (int a = 0; a < 3; ++a) F[3 * i + a] = f[a];
+  }
+}
+
+void pbo_euler_step(double *r, double *v, const double *a, const uint8_t *move, uint32_t n,
+                    double dt) {
+  /* propagateEuler_F4, kernelfuncs.cu:174-192 */
+  for (uint32_t i = 0; i < n; ++i) {
+    const double d = move[i] ? dt : 0.0;
+    for (int c = 0; c < 3; ++c) {
+      r[3 * i + c] += v[3 * i + c] * d;
+      v[3 * i + c] += a[3 * i + c] * d;
+    }
+  }
+}
+
 void pbo_gr_setup(float rstep, float rmax, float *rstepinv, uint32_t *rnum) {
   /* cudaCalcGR::setup, cudaCalcGR.hh:25-34 */
   *rstepinv = (float)(1.0 / (double)rstep);

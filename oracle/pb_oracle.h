@@ -191,6 +191,31 @@ uint32_t pbo_dem_force(const double *r, const double *v, const double *w, const 
 void pbo_rot_step(double *w, double *L, const double *T, const double *m0inv,
                   const uint8_t *move, uint32_t n, double dt);
 
+/* ------------------------------------------- MD neighbour list, annealing */
+/* makeNList (cudaParticleMD.cu:338-394): CSR list by original index over the
+ * periodic 27-cell grid of rmax_cells, j != i with FP32 distance2p < rmax2t,
+ * rows ascending.  rowptr[n+1] always filled, *nnz = entries; colidx filled
+ * when cap >= nnz.  Returns -1 on bad cells.
+ * pinned by: numpy FP32 brute force over all pairs (tests/test_oracle_md.py). */
+int pbo_nlist_f32(const float *r, uint32_t n, const float cell[6], float rmax_cells,
+                  float rmax2t, uint32_t *rowptr, uint32_t *colidx, uint32_t cap,
+                  uint32_t *nnz);
+/* calcF_IJpairWithList_F4 in FP64: LJ forces of the list's pairs below rc2.
+ * pinned by: equality with pbo_lj_forces_brute for a complete list, and a
+ * list with pairs removed. */
+void pbo_lj_forces_list(const double *r, const uint16_t *type, uint32_t n,
+                        const double cell[6], double rc2, const pbo_lj_params *p,
+                        const uint32_t *rowptr, const uint32_t *colidx, double *F,
+                        double *pe, double *virial);
+/* initialAnnealing's soft-core force (potentialSC), FP64, all pairs.
+ * pinned by: two-particle closed form, zero beyond rc, sum F = 0. */
+void pbo_softcore_forces(const double *r, uint32_t n, const double cell[6], double rc,
+                         double f0, double rmax2, double *F);
+/* propagateEuler_F4 (kernelfuncs.cu:174-192), FP64.
+ * pinned by: constant-acceleration closed form after k steps. */
+void pbo_euler_step(double *r, double *v, const double *a, const uint8_t *move, uint32_t n,
+                    double dt);
+
 /* ------------------------------------------------------------- thermostats */
 /* GaussianThermo / adjustVelocities / constTemp restated in FP64
  * (pb_oracle_thermo.c); xyz-interleaved arrays.

@@ -81,6 +81,12 @@ def _extra(L):
         ("pbo_gt_step", C.c_int, [P(d), P(d), P(d), P(d), P(d), u32, d, P(d), d]),
         ("pbo_adjust_velocities", None, [P(d), u32, d, P(d)]),
         ("pbo_const_temp", d, [P(d), P(d), P(d), u32, P(d)]),
+        ("pbo_nlist_f32", C.c_int, [P(C.c_float), u32, P(C.c_float), C.c_float, C.c_float,
+                                    P(u32), P(u32), u32, P(u32)]),
+        ("pbo_softcore_forces", None, [P(d), u32, P(d), d, d, d, P(d)]),
+        ("pbo_lj_forces_list", None, [P(d), P(C.c_uint16), u32, P(d), d, C.c_void_p, P(u32),
+                                      P(u32), P(d), P(d), P(d)]),
+        ("pbo_euler_step", None, [P(d), P(d), P(d), P(u8), u32, d]),
     ]:
         fn = getattr(L, name)
         fn.restype = res
@@ -387,3 +393,53 @@ def const_temp(v, F, minv):
     lam = d()
     mv2 = lib().pbo_const_temp(_p(v, d), _p(F, d), _p(minv, d), minv.size, C.byref(lam))
     return F, mv2, lam.value
+
+
+# ------------------------------------------ MD neighbour list, annealing
+def nlist_f32(r32, cell6, rmax_cells, rmax2t):
+    """makeNList -> rowptr (n+1), colidx (nnz) by original index, rows ascending."""
+    r = f32(r32).reshape(-1)
+    n = r.size // 3
+    row = np.zeros(n + 1, dtype=np.uint32)
+    z = C.c_uint32()
+    c6 = f32(cell6)
+    L = lib()
+    args = (_p(r, C.c_float), n, _p(c6, C.c_float), np.float32(rmax_cells), np.float32(rmax2t),
+            _p(row, C.c_uint32))
+    if L.pbo_nlist_f32(*args, None, 0, C.byref(z)):
+        raise ValueError("oracle cells invalid")
+    col = np.zeros(max(z.value, 1), dtype=np.uint32)
+    L.pbo_nlist_f32(*args, _p(col, C.c_uint32), col.size, C.byref(z))
+    return row, col[:z.value]
+
+
+def lj_forces_list(r, type_, cell6, rc2, params, rowptr, colidx):
+    r = f64(r)
+    n = r.shape[0]
+    t = np.ascontiguousarray(type_, dtype=np.uint16)
+    F = np.zeros((n, 3))
+    pe, w = C.c_double(), C.c_double()
+    rp = np.ascontiguousarray(rowptr, dtype=np.uint32)
+    ci = np.ascontiguousarray(colidx, dtype=np.uint32)
+    if ci.size == 0:
+        ci = np.zeros(1, dtype=np.uint32)
+    lib().pbo_lj_forces_list(_p(r, C.c_double), _p(t, C.c_uint16), n, _p(f64(cell6), C.c_double),
+                             rc2, C.byref(params), _p(rp, C.c_uint32), _p(ci, C.c_uint32),
+                             _p(F, C.c_double), C.byref(pe), C.byref(w))
+    return F, pe.value, w.value
+
+
+def softcore_forces(r, cell6, rc, f0, rmax2):
+    r = f64(r)
+    F = np.zeros_like(r)
+    lib().pbo_softcore_forces(_p(r, C.c_double), r.shape[0], _p(f64(cell6), C.c_double), rc, f0,
+                              rmax2, _p(F, C.c_double))
+    return F
+
+
+def euler_step(r, v, a, move, dt):
+    r, v, a = f64(r).copy(), f64(v).copy(), f64(a)
+    mv = np.ascontiguousarray(move, dtype=np.uint8)
+    d = C.c_double
+    lib().pbo_euler_step(_p(r, d), _p(v, d), _p(a, d), _p(mv, C.c_uint8), r.shape[0], dt)
+    return r, v

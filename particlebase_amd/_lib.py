@@ -48,6 +48,7 @@ EXPORTS = [
     "pb_dem_set_properties", "pb_rot_set_inertia", "pb_rot_step", "pb_dem_calc_force",
     "pb_dem_get_contacts", "pb_dem_last_violations",
     "pb_gt_step", "pb_gt_xi", "pb_adjust_velocities", "pb_const_temp",
+    "pb_nlist_build", "pb_nlist_get", "pb_lj_force_nlist", "pb_softcore_force", "pb_euler_step",
 ]
 
 
@@ -148,6 +149,11 @@ def _extra_signatures():
         "pb_gt_xi": (i32, [vp, P(d), P(d), P(u32)]),
         "pb_adjust_velocities": (i32, [vp, d, P(d)]),
         "pb_const_temp": (i32, [vp, P(d), P(d)]),
+        "pb_nlist_build": (i32, [vp, C.c_float, P(u32)]),
+        "pb_nlist_get": (i32, [vp, P(u32), P(u32), u32, P(u32)]),
+        "pb_lj_force_nlist": (i32, [vp, P(d), P(d)]),
+        "pb_softcore_force": (i32, [vp, d, d, d]),
+        "pb_euler_step": (i32, [vp, d]),
         "pb_dem_calc_force": (i32, [vp, d, i32, P(u32)]),
         "pb_dem_get_contacts": (i32, [vp, P(u32), P(u32), P(d), u32, P(u32)]),
         "pb_dem_last_violations": (i32, [vp, P(u32)]),
@@ -355,6 +361,32 @@ class System:
         lam = np.empty(self.nrep, dtype=np.float64)
         self._c(self.lib.pb_const_temp(self.h, _p(mv2, C.c_double), _p(lam, C.c_double)))
         return mv2, lam
+
+    # --- MD neighbour list (makeNList / calcForce with useNList), annealing
+    def nlist_build(self, rmax2t):
+        z = C.c_uint32()
+        self._c(self.lib.pb_nlist_build(self.h, float(rmax2t), C.byref(z)))
+        return z.value
+
+    def nlist_get(self):
+        """-> rowptr (n+1), colidx (nnz), by original index."""
+        z = C.c_uint32()
+        row = np.zeros(self.n + 1, dtype=np.uint32)
+        self._c(self.lib.pb_nlist_get(self.h, _p(row, C.c_uint32), None, 0, C.byref(z)))
+        col = np.zeros(max(z.value, 1), dtype=np.uint32)
+        self._c(self.lib.pb_nlist_get(self.h, None, _p(col, C.c_uint32), col.size, C.byref(z)))
+        return row, col[:z.value]
+
+    def lj_force_nlist(self):
+        pe, w = C.c_double(), C.c_double()
+        self._c(self.lib.pb_lj_force_nlist(self.h, C.byref(pe), C.byref(w)))
+        return pe.value, w.value
+
+    def softcore_force(self, rc, f0, rmax2):
+        self._c(self.lib.pb_softcore_force(self.h, rc, f0, rmax2))
+
+    def euler_step(self, dt):
+        self._c(self.lib.pb_euler_step(self.h, dt))
 
     def inspect_velocity(self, vlim, lim_u, lim_l):
         res = C.c_int()

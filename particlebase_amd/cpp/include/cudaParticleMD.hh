@@ -36,6 +36,7 @@ public:
   real kB = 1.0;
   real rmax2 = 0.0;
   real thickness = 0.0;
+  bool useNlist = false;
   bool shifted = false;          //!< shift energies so phi(rc) = 0 (forces unchanged)
   int precision = PB_FP64;       //!< PB_FP64 or PB_MIXED pair math
 
@@ -51,9 +52,62 @@ public:
     cudaSortedPositions::setupSortedPositions();
   }
 
+  // calcForce, cudaParticleMD.cu:24-65: from the neighbour list when
+  // useNList() was called (the list of the last makeNList), else the cells
   void calcForce(void) override {
     syncCutoff();
-    pbamd::check(pb_lj_force(sys, precision, &lastPE, &lastVirial), "cudaParticleMD::calcForce");
+    if (useNlist)
+      pbamd::check(pb_lj_force_nlist(sys, &lastPE, &lastVirial), "cudaParticleMD::calcForce");
+    else
+      pbamd::check(pb_lj_force(sys, precision, &lastPE, &lastVirial), "cudaParticleMD::calcForce");
+  }
+
+  // useNList / makeNList, cudaParticleMD.hh:167-175, cudaParticleMD.cu:338-394:
+  // CSR list of the pairs within sqrt(rmax2) + thickness
+  void useNList(void) { useNlist = true; }
+  void makeNList(void) {
+    if (!useNlist) return;
+    const real rt = std::sqrt(rmax2) + thickness;
+    calcBlockID();
+    pbamd::check(pb_nlist_build(sys, rt * rt, nullptr), "cudaParticleMD::makeNList");
+  }
+  // the list as cudaSparseMat's rowPtr / colIdx (original indices)
+  void getNList(std::vector<uint32_t> &rowPtr, std::vector<uint32_t> &colIdx) {
+    uint32_t nnz = 0;
+    rowPtr.resize(N + 1);
+    pbamd::check(pb_nlist_get(sys, rowPtr.data(), nullptr, 0, &nnz), "getNList");
+    colIdx.resize(nnz);
+    pbamd::check(pb_nlist_get(sys, nullptr, colIdx.data(), nnz, &nnz), "getNList");
+  }
+
+  // initialAnnealing, cudaParticleMD.cu:205-283: anealstep Euler steps with
+  // the soft-core force of radius _rc (i+1)/anealstep and strength _f0
+  // under constTemp, scaleTemp(T), then anealstep velocity-Verlet steps
+  // with LJ under constTemp, and scaleTemp(T) again
+  void initialAnnealing(uint32_t anealstep, real dt, real _rc, real _f0, real T) {
+    std::cerr << "performs initial annealing by soft core potential to rmax= " << _rc
+              << " with steps " << anealstep << std::endl;
+    for (uint32_t i = 0; i < anealstep; ++i) {
+      const double rc = (double)_rc * (i + 1) / anealstep;
+      calcBlockID();
+      pbamd::check(pb_softcore_force(sys, rc, _f0, rmax2), "initialAnnealing soft core");
+      constTemp();
+      pbamd::check(pb_calc_acceleration(sys), "initialAnnealing calcA");
+      pbamd::check(pb_euler_step(sys, dt), "initialAnnealing Euler");
+      treatPeriodicCondition();
+    }
+    std::cerr << "temperature scaling: " << scaleTemp(T) << std::endl;
+    std::cerr << "with LJ " << std::endl;
+    pbamd::check(pb_clear_force(sys, 1), "initialAnnealing clear Fold");
+    syncCutoff();
+    for (uint32_t i = 0; i < anealstep; ++i) {
+      calcBlockID();
+      pbamd::check(pb_lj_force(sys, precision, &lastPE, &lastVirial), "initialAnnealing LJ");
+      constTemp();
+      pbamd::check(pb_vv_step(sys, dt), "initialAnnealing VV");
+      treatPeriodicCondition();
+    }
+    std::cerr << "temperature scaling: " << scaleTemp(T) << std::endl;
   }
 
   void setM(void) {}

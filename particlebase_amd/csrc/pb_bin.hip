@@ -427,3 +427,337 @@ extern "C" int pb_gr_hist(pb_sys *s, const float *rstep, uint32_t rnum, uint64_t
   if (e != hipSuccess) return pb_hip_check(s->ctx, e, "pb_gr_hist");
   return PB_OK;
 }
+
+__global__ void k_clear4_bin(double4 *x, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
+// ------------------------------------------- MD neighbour list (CSR) path
+// cudaParticleMD::useNList / makeNList (cudaParticleMD.cu:338-394;
+// calcCoordMD_F4 counts, makeJlist_WithBlock_F4 kerneltemplate.hh:725-775,
+// MDNeighborRegister kernelinline.hh:404-412) and the forces of
+// calcF_IJpairWithList_F4 (kerneltemplate.hh:136-168).  Rows and columns are
+// original particle indices; a row holds every j != i of the 27 cells with
+// FP32 distance2p < rmax2t, in ascending order (the reference leaves them
+// in traversal order).
+
+// visit the storage slots sj of the 27 cells around cell c (periodic wrap
+// as calc_BlockNeighbor; non-periodic cells outside the box are skipped)
+template <typename Fn>
+__device__ __forceinline__ void for_27_cells(const RefGrid &g, uint32_t c, int periodic,
+                                             const uint32_t *start, const uint32_t *perm, Fn &&fn) {
+  const uint32_t nyz = g.nb[1] * g.nb[2];
+  const int i0 = (int)(c / nyz), j0 = (int)((c / g.nb[2]) % g.nb[1]), k0 = (int)(c % g.nb[2]);
+  for (int a = -1; a <= 1; ++a) {
+    int i2 = i0 + a;
+    if (i2 < 0 || i2 >= (int)g.nb[0]) {
+      if (!periodic) continue;
+      i2 = (i2 + (int)g.nb[0]) % (int)g.nb[0];
+    }
+    for (int b = -1; b <= 1; ++b) {
+      int j2 = j0 + b;
+      if (j2 < 0 || j2 >= (int)g.nb[1]) {
+        if (!periodic) continue;
+        j2 = (j2 + (int)g.nb[1]) % (int)g.nb[1];
+      }
+      for (int d = -1; d <= 1; ++d) {
+        int k2 = k0 + d;
+        if (k2 < 0 || k2 >= (int)g.nb[2]) {
+          if (!periodic) continue;
+          k2 = (k2 + (int)g.nb[2]) % (int)g.nb[2];
+        }
+        const uint32_t J = (uint32_t)k2 + (uint32_t)j2 * g.nb[2] + (uint32_t)i2 * nyz;
+        for (uint32_t jj = start[J]; jj < start[J + 1]; ++jj) fn(perm[jj]);
+      }
+    }
+  }
+}
+
+struct NlistArgs {
+  const double4 *r;
+  const uint32_t *perm, *cellid, *start, *orig;
+  uint32_t nreal;
+  RefGrid g;
+  float cx, cy, cz, rmax2t;
+  int periodic;
+};
+
+__global__ void k_nlist_count(NlistArgs A, uint32_t *rowcnt) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= A.nreal) return;
+  const uint32_t si = A.perm[k];
+  const double4 pi = A.r[si];
+  const float xi = (float)pi.x, yi = (float)pi.y, zi = (float)pi.z;
+  uint32_t cnt = 0;
+  for_27_cells(A.g, A.cellid[si], A.periodic, A.start, A.perm, [&](uint32_t sj) {
+    if (sj == si) return;
+    const double4 pj = A.r[sj];
+    if (distance2p(xi, yi, zi, (float)pj.x, (float)pj.y, (float)pj.z, A.cx, A.cy, A.cz) < A.rmax2t)
+      ++cnt;
+  });
+  rowcnt[A.orig[si]] = cnt;
+}
+
+__global__ void k_nlist_fill(NlistArgs A, const uint32_t *rowptr, uint32_t *col) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= A.nreal) return;
+  const uint32_t si = A.perm[k];
+  const double4 pi = A.r[si];
+  const float xi = (float)pi.x, yi = (float)pi.y, zi = (float)pi.z;
+  const uint32_t b = rowptr[A.orig[si]];
+  uint32_t e = b;
+  for_27_cells(A.g, A.cellid[si], A.periodic, A.start, A.perm, [&](uint32_t sj) {
+    if (sj == si) return;
+    const double4 pj = A.r[sj];
+    if (distance2p(xi, yi, zi, (float)pj.x, (float)pj.y, (float)pj.z, A.cx, A.cy, A.cz) < A.rmax2t)
+      col[e++] = A.orig[sj];
+  });
+  // ascending columns (insertion sort of the row)
+  for (uint32_t i = b + 1; i < e; ++i) {
+    const uint32_t x = col[i];
+    uint32_t j = i;
+    while (j > b && col[j - 1] > x) {
+      col[j] = col[j - 1];
+      --j;
+    }
+    col[j] = x;
+  }
+}
+
+__global__ void k_inverse_perm(const uint32_t *orig, uint32_t n, uint32_t *inv) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n && orig[s] != PB_NONE) inv[orig[s]] = s;
+}
+
+// FP64 minimum-image displacement (the distance2p fold of MDpairForce,
+// kerneltemplate.hh:538-543, without its FP32 rounding)
+__device__ __forceinline__ double min_image(double d, double L, int periodic) {
+  if (periodic && d * d > 0.25 * L * L) d += d < 0 ? L : -L;
+  return d;
+}
+
+struct PairArgs {
+  const double4 *r;
+  const uint32_t *orig, *inv;
+  const uint32_t *rowptr, *col;     // list path
+  const uint32_t *perm, *cellid, *start;   // cell path
+  uint32_t npad;
+  RefGrid g;
+  double L[3];
+  int periodic;
+  const LJTable *lj;
+  double rc2;
+  double sc_rc, sc_f0;              // soft core
+  double4 *F;
+  double *partials;
+};
+
+// LJ forces, PE and virial from the CSR list (pairs with r^2 < rc2)
+__global__ void k_lj_force_nlist(PairArgs A) {
+  const uint32_t si = blockIdx.x * blockDim.x + threadIdx.x;
+  double pe = 0.0, vir = 0.0;
+  if (si < A.npad && A.orig[si] != PB_NONE) {
+    const uint32_t row = A.orig[si];
+    const double4 pi = A.r[si];
+    const uint32_t ti = (uint32_t)pi.w;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (uint32_t k = A.rowptr[row]; k < A.rowptr[row + 1]; ++k) {
+      const double4 pj = A.r[A.inv[A.col[k]]];
+      const double dx = min_image(pi.x - pj.x, A.L[0], A.periodic);
+      const double dy = min_image(pi.y - pj.y, A.L[1], A.periodic);
+      const double dz = min_image(pi.z - pj.z, A.L[2], A.periodic);
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (!(r2 < A.rc2)) continue;
+      const uint32_t t = ti * PB_MAXT + (uint32_t)pj.w;
+      double s6 = A.lj->s2[t] / r2;
+      s6 = s6 * s6 * s6;
+      const double fr = A.lj->e24[t] * s6 * (2.0 * s6 - 1.0) / r2;
+      fx += fr * dx;
+      fy += fr * dy;
+      fz += fr * dz;
+      pe += 0.5 * (A.lj->e4[t] * s6 * (s6 - 1.0) - A.lj->shift[t]);
+      vir += 0.5 * fr * r2;
+    }
+    A.F[si] = make_double4(fx, fy, fz, 0.0);
+  }
+  pe = pb_wave_sum(pe);
+  vir = pb_wave_sum(vir);
+  if (pb_lane() == 0 && si < A.npad) {
+    A.partials[4 * (si / PB_WAVE) + 0] = pe;
+    A.partials[4 * (si / PB_WAVE) + 1] = vir;
+  }
+}
+
+// Soft-core forces of initialAnnealing (potentialSC, kernelinline.hh:140-158,
+// through MDpairForce, kerneltemplate.hh:524-565) over the 27 cells:
+// F_i = sum_j f0 cos(r/rc pi/2) (r_i - r_j)/r for r <= rc and r^2 < rmax2.
+__global__ void k_softcore_force(PairArgs A, uint32_t nreal) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nreal) return;
+  const uint32_t si = A.perm[k];
+  const double4 pi = A.r[si];
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  const double hpi = 0.5 * M_PI / A.sc_rc;
+  for_27_cells(A.g, A.cellid[si], A.periodic, A.start, A.perm, [&](uint32_t sj) {
+    if (sj == si) return;
+    const double4 pj = A.r[sj];
+    const double dx = min_image(pi.x - pj.x, A.L[0], A.periodic);
+    const double dy = min_image(pi.y - pj.y, A.L[1], A.periodic);
+    const double dz = min_image(pi.z - pj.z, A.L[2], A.periodic);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (!(r2 < A.rc2)) return;
+    const double r = sqrt(r2);
+    if (r > A.sc_rc) return;
+    const double f = A.sc_f0 * cos(r * hpi) / r;
+    fx += f * dx;
+    fy += f * dy;
+    fz += f * dz;
+  });
+  A.F[si] = make_double4(fx, fy, fz, 0.0);
+}
+
+static int need_binned_grid(pb_sys *s, const char *who) {
+  if (!s->ref_cells || s->ref_single)
+    return pb_fail(s->ctx, PB_ERR_STATE, std::string(who) + ": needs a 27-neighbour cell grid");
+  if (!s->ref_binned) return pb_fail(s->ctx, PB_ERR_STATE, std::string(who) + ": pb_bin first");
+  return PB_OK;
+}
+
+static NlistArgs nlist_args(pb_sys *s, float rmax2t) {
+  NlistArgs A;
+  A.r = s->r;
+  A.perm = s->perm;
+  A.cellid = s->cellid;
+  A.start = s->cstart;
+  A.orig = s->orig;
+  A.nreal = s->n;
+  A.g = ref_grid(s);
+  float c[3];
+  half_cell2(s, 0, c);
+  A.cx = c[0];
+  A.cy = c[1];
+  A.cz = c[2];
+  A.rmax2t = rmax2t;
+  A.periodic = s->ref_periodic;
+  return A;
+}
+
+extern "C" int pb_nlist_build(pb_sys *s, float rmax2t, uint32_t *nnz_out) {
+  if (!s || !(rmax2t > 0.0f)) return PB_ERR_ARG;
+  if (int rc = need_binned_grid(s, "pb_nlist_build")) return rc;
+  const float rc = sqrtf(rmax2t);
+  for (int a = 0; a < 3; ++a)
+    if (s->blocklen[a] < rc)
+      return pb_fail(s->ctx, PB_ERR_CELLS, "pb_nlist_build: cell length below the list radius");
+  hipStream_t st = s->ctx->stream;
+  const uint32_t n = s->n;
+  int rc2;
+  if (!s->md_row && (rc2 = pb_alloc(s->ctx, &s->md_row, (size_t)n + 1))) return rc2;
+  uint32_t *cnt = nullptr;
+  if ((rc2 = pb_alloc(s->ctx, &cnt, (size_t)n + 1))) return rc2;
+  const NlistArgs A = nlist_args(s, rmax2t);
+  hipLaunchKernelGGL(k_nlist_count, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A, cnt);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    pb_free(cnt);
+    return pb_hip_check(s->ctx, e, "k_nlist_count");
+  }
+  rc2 = pb_exclusive_scan(s, cnt, s->md_row, n);
+  pb_free(cnt);
+  if (rc2) return rc2;
+  uint32_t nnz = 0;
+  PB_HIP(s->ctx, hipMemcpyAsync(&nnz, s->md_row + n, 4, hipMemcpyDeviceToHost, st));
+  PB_HIP(s->ctx, hipStreamSynchronize(st));
+  if (s->md_cap < nnz || !s->md_col) {
+    if ((rc2 = pb_alloc(s->ctx, &s->md_col, (size_t)nnz + 1))) return rc2;
+    s->md_cap = nnz;
+  }
+  hipLaunchKernelGGL(k_nlist_fill, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A, s->md_row, s->md_col);
+  PB_LAUNCH_CHECK(s->ctx, "k_nlist_fill");
+  s->md_nnz = nnz;
+  s->md_valid = true;
+  if (nnz_out) *nnz_out = nnz;
+  return PB_OK;
+}
+
+extern "C" int pb_nlist_get(pb_sys *s, uint32_t *rowptr, uint32_t *colidx, uint32_t cap,
+                            uint32_t *nnz) {
+  if (!s) return PB_ERR_ARG;
+  if (!s->md_valid) return pb_fail(s->ctx, PB_ERR_STATE, "pb_nlist_get: pb_nlist_build first");
+  hipStream_t st = s->ctx->stream;
+  if (nnz) *nnz = s->md_nnz;
+  if (rowptr)
+    PB_HIP(s->ctx, hipMemcpyAsync(rowptr, s->md_row, 4 * ((size_t)s->n + 1), hipMemcpyDeviceToHost, st));
+  if (colidx) {
+    if (cap < s->md_nnz) return pb_fail(s->ctx, PB_ERR_ARG, "pb_nlist_get: colidx too small");
+    PB_HIP(s->ctx, hipMemcpyAsync(colidx, s->md_col, 4 * (size_t)s->md_nnz, hipMemcpyDeviceToHost, st));
+  }
+  PB_HIP(s->ctx, hipStreamSynchronize(st));
+  return PB_OK;
+}
+
+static PairArgs pair_args(pb_sys *s) {
+  PairArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.r = s->r;
+  A.orig = s->orig;
+  A.npad = s->npad;
+  A.g = ref_grid(s);
+  for (int a = 0; a < 3; ++a) A.L[a] = s->cells[2 * a + 1] - s->cells[2 * a];
+  A.periodic = s->ref_periodic;
+  A.lj = s->d_lj;
+  A.rc2 = s->rc2;
+  A.F = s->F;
+  A.partials = s->partials;
+  A.perm = s->perm;
+  A.cellid = s->cellid;
+  A.start = s->cstart;
+  return A;
+}
+
+extern "C" int pb_lj_force_nlist(pb_sys *s, double *pe, double *virial) {
+  if (!s) return PB_ERR_ARG;
+  if (!s->md_valid) return pb_fail(s->ctx, PB_ERR_STATE, "pb_lj_force_nlist: pb_nlist_build first");
+  if (!s->lj_set || !(s->rc2 > 0)) return pb_fail(s->ctx, PB_ERR_STATE, "LJ parameters or cutoff not set");
+  if (s->nrep != 1) return pb_fail(s->ctx, PB_ERR_STATE, "pb_lj_force_nlist: single replica only");
+  hipStream_t st = s->ctx->stream;
+  int rc;
+  if (!s->md_inv && (rc = pb_alloc(s->ctx, &s->md_inv, (size_t)s->n))) return rc;
+  hipLaunchKernelGGL(k_inverse_perm, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->orig,
+                     s->npad, s->md_inv);
+  PairArgs A = pair_args(s);
+  A.inv = s->md_inv;
+  A.rowptr = s->md_row;
+  A.col = s->md_col;
+  A.periodic = 1;       // MD boxes are periodic (MDpairForce folds every pair)
+  hipLaunchKernelGGL(k_lj_force_nlist, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, A);
+  PB_LAUNCH_CHECK(s->ctx, "k_lj_force_nlist");
+  if (pe || virial) {
+    if ((rc = pb_reduce_waves(s, 2))) return rc;
+    double h[8];
+    PB_HIP(s->ctx, hipMemcpyAsync(h, s->d_out, sizeof(h), hipMemcpyDeviceToHost, st));
+    PB_HIP(s->ctx, hipStreamSynchronize(st));
+    if (pe) *pe = h[0];
+    if (virial) *virial = h[1];
+  }
+  return PB_OK;
+}
+
+extern "C" int pb_softcore_force(pb_sys *s, double rc, double f0, double rmax2) {
+  if (!s || !(rc > 0) || !(rmax2 > 0)) return PB_ERR_ARG;
+  if (int e = need_binned_grid(s, "pb_softcore_force")) return e;
+  for (int a = 0; a < 3; ++a)
+    if ((double)s->blocklen[a] < std::min(rc, std::sqrt(rmax2)))
+      return pb_fail(s->ctx, PB_ERR_CELLS, "pb_softcore_force: cell length below the soft-core radius");
+  PairArgs A = pair_args(s);
+  A.rc2 = rmax2;
+  A.sc_rc = rc;
+  A.sc_f0 = f0;
+  hipLaunchKernelGGL(k_clear4_bin, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, s->ctx->stream,
+                     s->F, s->npad);
+  hipLaunchKernelGGL(k_softcore_force, dim3(pb_blocks(s->n, 256)), dim3(256), 0, s->ctx->stream, A,
+                     s->n);
+  PB_LAUNCH_CHECK(s->ctx, "pb_softcore_force");
+  return PB_OK;
+}

@@ -162,6 +162,7 @@ extern "C" void pb_sys_destroy(pb_sys *s) {
   pb_free(s->move); pb_free(s->type); pb_free(s->orig); pb_free(s->waveRep);
   pb_free(s->d_reps); pb_free(s->partials); pb_free(s->d_counter); pb_free(s->d_out);
   pb_free(s->gt); pb_free(s->gt_bad);
+  pb_free(s->md_row); pb_free(s->md_col); pb_free(s->md_inv);
   pb_free(s->ref_table); pb_free(s->ref_bid); pb_free(s->ref_pid); pb_free(s->ref_bindex);
   pb_free(s->cellid); pb_free(s->rank); pb_free(s->perm); pb_free(s->ccount);
   pb_free(s->cstart); pb_free(s->scan_tmp); pb_free(s->d_lj);
@@ -495,6 +496,25 @@ __global__ void k_vv_step(double4 *r, double4 *v, const double4 *F, double4 *Fol
   r[s] = R;
 }
 
+__global__ void k_euler_step(double4 *r, double4 *v, const double4 *a, const uint8_t *move,
+                             const uint32_t *orig, uint32_t n, double dt) {
+  // propagateEuler_F4, kernelfuncs.cu:174-192 (FP64): r += v dt, v += a dt
+  // for movable particles
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n || orig[s] == PB_NONE) return;
+  const double d = move[s] ? dt : 0.0;
+  double4 R = r[s], V = v[s];
+  const double4 A = a[s];
+  R.x += V.x * d;
+  R.y += V.y * d;
+  R.z += V.z * d;
+  V.x += A.x * d;
+  V.y += A.y * d;
+  V.z += A.z * d;
+  r[s] = R;
+  v[s] = V;
+}
+
 __device__ __forceinline__ double wrap1(double x, double c0, double c1) {
   // applyPeriodicCondition_F4 (kernelfuncs.cu:95-129) in FP64; a value that
   // rounds onto the upper face is put on the lower one.
@@ -758,6 +778,13 @@ extern "C" int pb_vv_step(pb_sys *s, double dt) {
   if (!s) return PB_ERR_ARG;
   hipLaunchKernelGGL(k_vv_step, PB_GRID(s->npad), s->r, s->v, s->F, s->a, s->orig, s->npad, dt);
   PB_LAUNCH_CHECK(s->ctx, "pb_vv_step");
+  return PB_OK;
+}
+
+extern "C" int pb_euler_step(pb_sys *s, double dt) {
+  if (!s) return PB_ERR_ARG;
+  hipLaunchKernelGGL(k_euler_step, PB_GRID(s->npad), s->r, s->v, s->a, s->move, s->orig, s->npad, dt);
+  PB_LAUNCH_CHECK(s->ctx, "pb_euler_step");
   return PB_OK;
 }
 

@@ -129,6 +129,11 @@ struct pb_sys {
   double *gt = nullptr;
   uint32_t *gt_bad = nullptr;
 
+  // MD neighbour list (cudaParticleMD::makeNList): CSR by original index
+  uint32_t *md_row = nullptr, *md_col = nullptr, *md_inv = nullptr;
+  uint32_t md_cap = 0, md_nnz = 0;
+  bool md_valid = false;
+
   // g(r)
   uint64_t *gr_hist = nullptr;
   size_t gr_cap = 0;

@@ -1,6 +1,7 @@
 """The C++ drivers (particlebase_amd/cpp/drivers) run the reference's driver
 call sequences through the class layer and the C ABI on the GPU: LJ
-(testMD1.cu order, per-call API and fused), liquid argon under the Gaussian
+(testMD1.cu order, per-call API and fused; and the testMD1 gas with
+initialAnnealing and the neighbour-list path), liquid argon under the Gaussian
 thermostat (testMD2.cu order), SPH dam break (testSPH5.cu
 order with AdaptiveTime) and DEM settle (testDEM5.cu order with
 AdaptiveTime).  Each must finish and report a physically sane end state."""
@@ -58,3 +59,15 @@ def test_md_gauss():
     assert 70.0 < t0 < 100.0
     assert abs(float(d["T1"]) / t0 - 1) < 0.01
     assert float(d["Tmax"]) / float(d["Tmin"]) < 1.02
+
+
+@pytest.mark.parametrize("mode", ["cells", "nlist"])
+def test_md_gas(mode):
+    # testMD1 order: random N2/O2 gas, adjustVelocities, initialAnnealing
+    # (shortened to 200 + 200 steps), 100 main-loop steps
+    d = run("md_gas", 100, 200, mode)
+    assert int(d["N"]) == 10618 and int(d["steps"]) == 100
+    # annealing separated the random start (LJ sigma ~ 3.2 A) and set T
+    assert float(d["dmin"]) > 2.0
+    assert abs(float(d["Tanneal"]) / 353.0 - 1) < 0.05
+    assert 300.0 < float(d["Tmin"]) and float(d["Tmax"]) < 1.2 * 353.0
