@@ -280,6 +280,14 @@ int pb_dem_calc_force(pb_sys *sys, double dt, int autotune, uint32_t *nnz_out);
 int pb_dem_get_contacts(pb_sys *sys, uint32_t *rowptr, uint32_t *colidx, double *val,
                         uint32_t cap, uint32_t *nnz);
 
+/* Restores a contact matrix (restart from a checkpoint: the cudaSparseMat
+ * fields of cudaParticleDEM's archive, cudaParticleDEM.hh:124-180,
+ * cudaSparseMat.hh:70-122) as the latest one, so the next
+ * pb_dem_calc_force carries its tangential history.  rowptr[n+1] over
+ * original indices, columns ascending, val[nnz]. */
+int pb_dem_set_contacts(pb_sys *sys, const uint32_t *rowptr, const uint32_t *colidx,
+                        const double *val, uint32_t nnz);
+
 /* Contacts of the last pb_dem_calc_force whose time step exceeded the
  * Cundall-Strack bound (counted even when autotune allowed the step). */
 int pb_dem_last_violations(pb_sys *sys, uint32_t *count);

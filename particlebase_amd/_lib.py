@@ -49,6 +49,7 @@ EXPORTS = [
     "pb_dem_get_contacts", "pb_dem_last_violations",
     "pb_gt_step", "pb_gt_xi", "pb_adjust_velocities", "pb_const_temp",
     "pb_nlist_build", "pb_nlist_get", "pb_lj_force_nlist", "pb_softcore_force", "pb_euler_step",
+    "pb_dem_set_contacts",
 ]
 
 
@@ -154,6 +155,7 @@ def _extra_signatures():
         "pb_lj_force_nlist": (i32, [vp, P(d), P(d)]),
         "pb_softcore_force": (i32, [vp, d, d, d]),
         "pb_euler_step": (i32, [vp, d]),
+        "pb_dem_set_contacts": (i32, [vp, P(u32), P(u32), P(d), u32]),
         "pb_dem_calc_force": (i32, [vp, d, i32, P(u32)]),
         "pb_dem_get_contacts": (i32, [vp, P(u32), P(u32), P(d), u32, P(u32)]),
         "pb_dem_last_violations": (i32, [vp, P(u32)]),
@@ -533,6 +535,13 @@ class System:
         self._c(self.lib.pb_dem_get_contacts(self.h, _p(row, C.c_uint32), _p(col, C.c_uint32),
                                              _p(val, C.c_double), col.size, C.byref(z)))
         return row, col[:z.value], val[:z.value]
+
+    def dem_set_contacts(self, row, col, val):
+        row = np.ascontiguousarray(row, dtype=np.uint32)
+        col = np.ascontiguousarray(col, dtype=np.uint32)
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        self._c(self.lib.pb_dem_set_contacts(self.h, _p(row, C.c_uint32), _p(col, C.c_uint32),
+                                             _p(val, C.c_double), int(col.size)))
 
     def dem_violations(self):
         z = C.c_uint32()

@@ -43,6 +43,9 @@ public:
   int nDevices(void) { return 1; }
   T &operator[](int) { return *P; }
   void setup(void) { setThnum(*P); }
+  // CUDAenv::readSerialization / writeSerialization (CUDAenv.hh:268-285)
+  void readSerialization(const std::string &fn) { P->readSerialization(fn); }
+  void writeSerialization(const std::string &fn) const { P->writeSerialization(fn); }
   void exchangeForce(void) {}
   void exchangeAccelerations(void) {}
 

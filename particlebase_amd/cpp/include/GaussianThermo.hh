@@ -16,7 +16,22 @@ protected:
 public:
   GaussianThermo() = default;
 
-  void setup(int n) { cudaParticleMD::setup(n); }
+  void setup(int n) override { cudaParticleMD::setup(n); }
+
+  // GaussianThermo fields (GaussianThermo.hh:92-120): xi and _e; velocityD
+  // is the engine's FP64 velocity, stored by cudaParticleBase
+  void save(pbamd::OArchive &a) const override {
+    cudaParticleMD::save(a);
+    double xi = 0;
+    pbamd::check(pb_gt_xi(sys, nullptr, &xi, nullptr), "GaussianThermo::save");
+    a.scalar("GaussianThermo.xi", xi);
+    a.scalar("GaussianThermo._e", _e);
+  }
+  void load(const pbamd::IArchive &a) override {
+    cudaParticleMD::load(a);
+    setXi(a.scalar<double>("GaussianThermo.xi"));
+    _e = a.scalar<real>("GaussianThermo._e");
+  }
 
   // v(t) = v(t-dt) + dt/2m (F(t)+F(t-dt)) - (v(t-dt) xi(t-dt) + v(t) xi(t)) dt/2
   // xi(t) = sum F(t).v(t) / sum m v(t)^2

@@ -16,6 +16,28 @@ protected:
   bool SingleParticleBlock = false;
   bool uninitialized = true;
   bool periodicCells = true;
+  real rmaxCells = 0;
+
+  // cudaCutoffBlock fields (cudaCutoffBlock.hh:135-240); the sort arrays
+  // (bid, pid, bindex) and the neighbour table are not stored: the grid is
+  // set up again from rmax and the next calcBlockID rebuilds them
+  void saveBlock(pbamd::OArchive &a) const {
+    a.scalar("cudaCutoffBlock.totalNumBlock", totalNumBlock);
+    a.put("cudaCutoffBlock.blocklen", blocklen);
+    a.put("cudaCutoffBlock.blocknum", blocknum);
+    a.scalar("cudaCutoffBlock.SingleParticleBlock", (uint8_t)SingleParticleBlock);
+    a.scalar("cudaCutoffBlock.uninitialized", (uint8_t)uninitialized);
+    a.scalar("cudaCutoffBlock.rmax", rmaxCells);
+    a.scalar("cudaCutoffBlock.periodic", (uint8_t)periodicCells);
+  }
+  void loadBlock(const pbamd::IArchive &a) {
+    SingleParticleBlock = a.scalar<uint8_t>("cudaCutoffBlock.SingleParticleBlock") != 0;
+    if (a.scalar<uint8_t>("cudaCutoffBlock.uninitialized")) return;
+    setupCutoffBlock(a.scalar<real>("cudaCutoffBlock.rmax"),
+                     a.scalar<uint8_t>("cudaCutoffBlock.periodic") != 0);
+    if (a.get<uint32_t>("cudaCutoffBlock.blocknum") != blocknum)
+      throw std::runtime_error("readSerialization: cell grid differs");
+  }
 
 public:
   // cudaCutoffBlock::setupCutoffBlock, cudaCutoffBlock.cu:75-231
@@ -27,6 +49,7 @@ public:
     for (int i = 0; i < 6; ++i) blocklen[i] = bl[i];
     totalNumBlock = blocknum[0] * blocknum[1] * blocknum[2];
     periodicCells = periodic;
+    rmaxCells = rmax;
     uninitialized = false;
   }
   // calcBID_F4 + sort + makeBindex (cudaCutoffBlock.cu:233-258)

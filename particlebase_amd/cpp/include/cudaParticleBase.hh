@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <vector>
 #include "ParticleBase.hh"
+#include "Serialization.hh"
 #include "cudaFrame.hh"
 
 class cudaParticleBase : public cudaFrame {
@@ -40,8 +41,9 @@ public:
     if (sys) pb_sys_destroy(sys);
   }
 
-  // cudaParticleBase::setup, cudaParticleBase.cu:18-66
-  void setup(int n) {
+  // cudaParticleBase::setup, cudaParticleBase.cu:18-66 (virtual so that a
+  // restart sets up the most derived class)
+  virtual void setup(int n) {
     N = (uint32_t)n;
     TMP.resize(4 * (size_t)n);
     TMP2.resize(n);
@@ -72,8 +74,81 @@ public:
                  "import");
   }
 
+  // checkpoint / restart (CUDAenv::writeSerialization / readSerialization,
+  // CUDAenv.hh:268-285): every class writes its own fields after those of
+  // its bases, in the reference's order
+  virtual void save(pbamd::OArchive &a) const { saveBase(a); }
+  virtual void load(const pbamd::IArchive &a) { loadBase(a); }
+  void writeSerialization(const std::string &fn) const {
+    pbamd::OArchive a(fn);
+    save(a);
+  }
+  void readSerialization(const std::string &fn) {
+    pbamd::IArchive a(fn);
+    load(a);
+  }
+
   virtual void calcForce(void) = 0;
   void TimeEvolution(real) {}
+
+  // cudaFrame + cudaParticleBase fields (cudaFrame.hh:95-101,
+  // cudaParticleBase.hh:228-354), FP64 as the engine keeps them
+  void saveBase(pbamd::OArchive &a) const {
+    a.put("cudaFrame.cellsize", cell, 9);
+    a.scalar("cudaFrame.timestep", timestep);
+    a.scalar("cudaParticleBase.particleNumN", N);
+    std::vector<double> x(3 * (size_t)N);
+    const std::pair<const char *, int> vecs[4] = {{"positionR", PB_FIELD_R},
+                                                  {"velocityV", PB_FIELD_V},
+                                                  {"accelerationA", PB_FIELD_A},
+                                                  {"forceF", PB_FIELD_F}};
+    for (const auto &f : vecs) {
+      pbamd::check(pb_sys_get(sys, f.second, x.data()), "writeSerialization");
+      a.put(std::string("cudaParticleBase.") + f.first, x);
+    }
+    std::vector<double> mi(N);
+    pbamd::check(pb_sys_get(sys, PB_FIELD_MINV, mi.data()), "writeSerialization");
+    a.put("cudaParticleBase.massInverseMinv", mi);
+    std::vector<uint16_t> t(N);
+    pbamd::check(pb_sys_get(sys, PB_FIELD_TYPE, t.data()), "writeSerialization");
+    a.put("cudaParticleBase.typeID", t);
+    std::vector<uint8_t> mv(N);
+    pbamd::check(pb_sys_get(sys, PB_FIELD_MOVE, mv.data()), "writeSerialization");
+    a.put("cudaParticleBase.moveFlag", mv);
+  }
+  void loadBase(const pbamd::IArchive &a) {
+    const uint32_t n = a.scalar<uint32_t>("cudaParticleBase.particleNumN");
+    if (!sys) setup((int)n);
+    else if (n != N) throw std::runtime_error("readSerialization: particle count differs");
+    const std::vector<real> c = a.get<real>("cudaFrame.cellsize");
+    if (c.size() != 9) throw std::runtime_error("readSerialization: cellsize");
+    setCell(c.data());
+    timestep = a.scalar<uint32_t>("cudaFrame.timestep");
+    const auto r = a.get<double>("cudaParticleBase.positionR");
+    const auto v = a.get<double>("cudaParticleBase.velocityV");
+    const auto ac = a.get<double>("cudaParticleBase.accelerationA");
+    const auto f = a.get<double>("cudaParticleBase.forceF");
+    const auto mi = a.get<double>("cudaParticleBase.massInverseMinv");
+    const auto t = a.get<uint16_t>("cudaParticleBase.typeID");
+    const auto mv = a.get<uint8_t>("cudaParticleBase.moveFlag");
+    const size_t n3 = 3 * (size_t)N;
+    if (r.size() != n3 || v.size() != n3 || ac.size() != n3 || f.size() != n3 || mi.size() != N ||
+        t.size() != N || mv.size() != N)
+      throw std::runtime_error("readSerialization: array sizes");
+    std::vector<double> m(N);
+    std::vector<uint32_t> t32(N);
+    std::vector<uint8_t> fixed(N);
+    for (size_t i = 0; i < N; ++i) {
+      m[i] = 1.0 / mi[i];
+      t32[i] = t[i];
+      fixed[i] = mv[i] ? 0 : 1;
+    }
+    pbamd::check(pb_sys_import(sys, r.data(), v.data(), ac.data(), m.data(), t32.data(), fixed.data()),
+                 "readSerialization");
+    // 1/m exactly as saved, and the force of the last step
+    pbamd::check(pb_sys_set(sys, PB_FIELD_MINV, mi.data()), "readSerialization");
+    pbamd::check(pb_sys_set(sys, PB_FIELD_F, f.data()), "readSerialization");
+  }
 
   void getPosition(void) { fetch(PB_FIELD_R); }
   void getAcceleration(void) { fetch(PB_FIELD_A); }

@@ -8,14 +8,51 @@
 #include "cudaParticleLF.hh"
 
 class cudaParticleRotation : public virtual cudaParticleBase {
+protected:
+  std::vector<double> inertiaR;    // radii given to setInertia (restarts rebuild m0inv)
+  bool inertiaSet = false;
+
+  // cudaParticleRotation fields (cudaParticleRotation.hh:68-125); m0inv is
+  // rebuilt from the radii, exactly as setInertia computed it
+  void saveRotation(pbamd::OArchive &a) const {
+    a.scalar("cudaParticleRotation.inertiaSet", (uint8_t)inertiaSet);
+    if (!inertiaSet) return;
+    a.put("cudaParticleRotation.inertiaRadius", inertiaR);
+    std::vector<double> x(3 * (size_t)N);
+    const std::pair<const char *, int> vecs[3] = {{"angularVelocityW", PB_FIELD_W},
+                                                  {"angularMomentumL", PB_FIELD_L},
+                                                  {"torqueT", PB_FIELD_T}};
+    for (const auto &f : vecs) {
+      pbamd::check(pb_sys_get(sys, f.second, x.data()), "writeSerialization");
+      a.put(std::string("cudaParticleRotation.") + f.first, x);
+    }
+  }
+  void loadRotation(const pbamd::IArchive &a) {
+    if (!a.scalar<uint8_t>("cudaParticleRotation.inertiaSet")) return;
+    inertiaR = a.get<double>("cudaParticleRotation.inertiaRadius");
+    if (inertiaR.size() != N) throw std::runtime_error("readSerialization: inertia radii");
+    pbamd::check(pb_rot_set_inertia(sys, inertiaR.data(), 0.0), "readSerialization");
+    inertiaSet = true;
+    const char *names[3] = {"angularVelocityW", "angularMomentumL", "torqueT"};
+    const int fields[3] = {PB_FIELD_W, PB_FIELD_L, PB_FIELD_T};
+    for (int k = 0; k < 3; ++k) {
+      const auto x = a.get<double>(std::string("cudaParticleRotation.") + names[k]);
+      if (x.size() != 3 * (size_t)N) throw std::runtime_error("readSerialization: rotation state");
+      pbamd::check(pb_sys_set(sys, fields[k], x.data()), "readSerialization");
+    }
+  }
+
 public:
   // cudaParticleRotation::setInertia (cudaParticleRotation.cu:35-61)
   void setInertia(const std::vector<real> &r0) {
-    std::vector<double> r(r0.begin(), r0.end());
-    pbamd::check(pb_rot_set_inertia(sys, r.data(), 0.0), "setInertia");
+    inertiaR.assign(r0.begin(), r0.end());
+    pbamd::check(pb_rot_set_inertia(sys, inertiaR.data(), 0.0), "setInertia");
+    inertiaSet = true;
   }
   void setInertia(real r0_all) {
+    inertiaR.assign(N, (double)r0_all);
     pbamd::check(pb_rot_set_inertia(sys, nullptr, r0_all), "setInertia");
+    inertiaSet = true;
   }
   // cudaParticleRotation::TimeEvolution (cudaParticleRotation.cu:29-33)
   void TimeEvolution(real dt) { pbamd::check(pb_rot_step(sys, dt), "Rotation::TimeEvolution"); }
@@ -26,13 +63,17 @@ class cudaParticleDEM : public cudaParticleLF,
                         virtual public cudaCutoffBlock {
 protected:
   uint32_t lastNNZ = 0;
+  // the arguments of setDEMProperties, kept for restarts
+  double demE = 0, demMu = 0, demSigma = 0, demGamma = 0, demMuR = 0;
+  std::vector<double> demR;
+  bool demSet = false;
 
 public:
   bool autotunetimestep = false;   //!< true when AdaptiveTime drives dt (no Cundall-Strack abort)
 
   cudaParticleDEM() { SingleParticleBlock = true; }   // cudaParticleDEM.hh:37
 
-  void setup(int n) { cudaParticleBase::setup(n); }
+  void setup(int n) override { cudaParticleBase::setup(n); }
 
   // the one-argument form is the DEM force (cudaParticleDEM.hh:53-54)
   void calcForce(void) override {}
@@ -50,14 +91,81 @@ public:
 
   // both overloads of setDEMProperties (cudaParticleDEM.cu:221-252)
   void setDEMProperties(real E, real mu, real sigma, real gamma, real mu_r, real r0_all) {
-    pbamd::check(pb_dem_set_properties(sys, E, mu, sigma, gamma, mu_r, nullptr, r0_all),
-                 "setDEMProperties");
+    setDEMProperties(E, mu, sigma, gamma, mu_r, std::vector<real>(N, r0_all));
   }
   void setDEMProperties(real E, real mu, real sigma, real gamma, real mu_r,
                         const std::vector<real> &r0) {
-    std::vector<double> r(r0.begin(), r0.end());
-    pbamd::check(pb_dem_set_properties(sys, E, mu, sigma, gamma, mu_r, r.data(), 0.0),
+    demR.assign(r0.begin(), r0.end());
+    demE = E;
+    demMu = mu;
+    demSigma = sigma;
+    demGamma = gamma;
+    demMuR = mu_r;
+    pbamd::check(pb_dem_set_properties(sys, demE, demMu, demSigma, demGamma, demMuR, demR.data(), 0.0),
                  "setDEMProperties");
+    demSet = true;
+  }
+
+  // cudaParticleDEM fields (cudaParticleDEM.hh:124-180) after cudaParticleLF,
+  // cudaParticleRotation and the cells: the material constants as given,
+  // the radii, and the latest contact matrix (cudaSparseMat) whose val
+  // holds the tangential history
+  void save(pbamd::OArchive &a) const override {
+    saveBase(a);
+    saveBlock(a);
+    saveRotation(a);
+    a.scalar("cudaParticleDEM.demSet", (uint8_t)demSet);
+    a.scalar("cudaParticleDEM.autotunetimestep", (uint8_t)autotunetimestep);
+    if (!demSet) return;
+    a.scalar("cudaParticleDEM.E", demE);
+    a.scalar("cudaParticleDEM.mu", demMu);
+    a.scalar("cudaParticleDEM.sigma", demSigma);
+    a.scalar("cudaParticleDEM.gamma", demGamma);
+    a.scalar("cudaParticleDEM.mu_r", demMuR);
+    a.put("cudaParticleDEM.radius", demR);
+    uint32_t nnz = 0;
+    std::vector<uint32_t> row(N + 1), col;
+    std::vector<double> val;
+    if (pb_dem_get_contacts(sys, row.data(), nullptr, nullptr, 0, &nnz) != PB_OK) {
+      a.scalar("cudaSparseMat.NNZ", (uint32_t)0xFFFFFFFFu);   // no matrix yet
+      return;
+    }
+    col.resize(nnz);
+    val.resize(nnz);
+    pbamd::check(pb_dem_get_contacts(sys, row.data(), col.data(), val.data(), nnz, &nnz),
+                 "writeSerialization");
+    a.scalar("cudaSparseMat.NNZ", nnz);
+    a.put("cudaSparseMat.rowPtr", row);
+    a.put("cudaSparseMat.colIdx", col);
+    a.put("cudaSparseMat.val", val);
+  }
+  void load(const pbamd::IArchive &a) override {
+    loadBase(a);
+    loadBlock(a);
+    autotunetimestep = a.scalar<uint8_t>("cudaParticleDEM.autotunetimestep") != 0;
+    if (a.scalar<uint8_t>("cudaParticleDEM.demSet")) {
+      demR = a.get<double>("cudaParticleDEM.radius");
+      if (demR.size() != N) throw std::runtime_error("readSerialization: DEM radii");
+      demE = a.scalar<double>("cudaParticleDEM.E");
+      demMu = a.scalar<double>("cudaParticleDEM.mu");
+      demSigma = a.scalar<double>("cudaParticleDEM.sigma");
+      demGamma = a.scalar<double>("cudaParticleDEM.gamma");
+      demMuR = a.scalar<double>("cudaParticleDEM.mu_r");
+      pbamd::check(pb_dem_set_properties(sys, demE, demMu, demSigma, demGamma, demMuR, demR.data(), 0.0),
+                   "readSerialization");
+      demSet = true;
+    }
+    loadRotation(a);
+    if (demSet) {
+      const uint32_t nnz = a.scalar<uint32_t>("cudaSparseMat.NNZ");
+      if (nnz != 0xFFFFFFFFu) {
+        const auto row = a.get<uint32_t>("cudaSparseMat.rowPtr");
+        const auto col = a.get<uint32_t>("cudaSparseMat.colIdx");
+        const auto val = a.get<double>("cudaSparseMat.val");
+        pbamd::check(pb_dem_set_contacts(sys, row.data(), col.data(), val.data(), nnz),
+                     "readSerialization");
+      }
+    }
   }
 
   void selectBlocks(void) {}

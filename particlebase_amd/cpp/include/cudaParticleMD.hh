@@ -40,7 +40,30 @@ public:
   bool shifted = false;          //!< shift energies so phi(rc) = 0 (forces unchanged)
   int precision = PB_FP64;       //!< PB_FP64 or PB_MIXED pair math
 
-  void setup(int n) { cudaParticleVV::setup(n); }
+  void setup(int n) override { cudaParticleVV::setup(n); }
+
+  // cudaParticleMD fields (cudaParticleMD.hh:195-240) after particleVV and
+  // cutoffBlock; the engine's own settings follow
+  void save(pbamd::OArchive &a) const override {
+    saveBase(a);
+    saveBlock(a);
+    a.scalar("cudaParticleMD.BoltzmannConstKb", kB);
+    a.scalar("cudaParticleMD.rmax2", rmax2);
+    a.scalar("cudaParticleMD.m0", m0);
+    a.scalar("cudaParticleMD.thickness", thickness);
+    a.scalar("cudaParticleMD.shifted", (uint8_t)shifted);
+    a.scalar("cudaParticleMD.precision", (int32_t)precision);
+  }
+  void load(const pbamd::IArchive &a) override {
+    loadBase(a);
+    loadBlock(a);
+    kB = a.scalar<real>("cudaParticleMD.BoltzmannConstKb");
+    rmax2 = a.scalar<real>("cudaParticleMD.rmax2");
+    m0 = a.scalar<real>("cudaParticleMD.m0");
+    thickness = a.scalar<real>("cudaParticleMD.thickness");
+    shifted = a.scalar<uint8_t>("cudaParticleMD.shifted") != 0;
+    precision = a.scalar<int32_t>("cudaParticleMD.precision");
+  }
 
   // calcBlockID "steals" both parents (cudaParticleMD.hh:64-67)
   void calcBlockID(void) {

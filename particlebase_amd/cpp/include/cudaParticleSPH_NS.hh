@@ -4,6 +4,7 @@
 // The sortedOutput flags of the reference exist for its multi-GPU exchange;
 // one GPU owns every particle here, so they are accepted and have no effect.
 #pragma once
+#include <cmath>
 #include <valarray>
 #include "cudaCutoffBlock.hh"
 #include "cudaParticleLF.hh"
@@ -12,9 +13,10 @@ class cudaParticleSPH_NS : public cudaParticleLF, virtual public cudaCutoffBlock
 protected:
   real h = 0;
   double meanNum = 0.0;
+  std::vector<double> mu0s, cs;    // setSPHProperties arguments, kept for restarts
 
 public:
-  void setup(int n) {
+  void setup(int n) override {
     cudaParticleBase::setup(n);
     TMP.resize(8 * (size_t)n);       // x, y, z, _, num, rho, mu, _ (cudaParticleSPH_NS.cu:13-27)
   }
@@ -32,7 +34,35 @@ public:
       c[i] = c1[i];
     }
     h = _h;
+    mu0s = m;
+    cs = c;
     pbamd::check(pb_sph_set_properties(sys, m.data(), c.data(), _h), "setSPHProperties");
+  }
+
+  // cudaParticleSPHBase / cudaParticleSPH_NS fields (cudaParticleSPHBase.hh:
+  // 83-106, cudaParticleSPH_NS.hh:129-175): h, w0, viscosity and sound
+  // speed (c itself is kept too, so a restart sets c^2 bit for bit)
+  void save(pbamd::OArchive &a) const override {
+    saveBase(a);
+    saveBlock(a);
+    a.scalar("cudaParticleSPHBase.h", h);
+    a.scalar("cudaParticleSPHBase.w0", 105.0 / (16.0 * M_PI * std::pow((double)h, 7)));
+    a.put("cudaParticleSPH_NS.viscosityMu", mu0s);
+    std::vector<double> c2(cs.size());
+    for (size_t i = 0; i < cs.size(); ++i) c2[i] = cs[i] * cs[i];
+    a.put("cudaParticleSPH_NS.soundVelocityC2", c2);
+    a.put("cudaParticleSPH_NS.soundVelocityC", cs);
+  }
+  void load(const pbamd::IArchive &a) override {
+    loadBase(a);
+    loadBlock(a);
+    h = a.scalar<real>("cudaParticleSPHBase.h");
+    mu0s = a.get<double>("cudaParticleSPH_NS.viscosityMu");
+    cs = a.get<double>("cudaParticleSPH_NS.soundVelocityC");
+    if (!mu0s.empty()) {
+      if (mu0s.size() != N || cs.size() != N) throw std::runtime_error("readSerialization: SPH arrays");
+      pbamd::check(pb_sph_set_properties(sys, mu0s.data(), cs.data(), h), "readSerialization");
+    }
   }
 
   // calcDensity + calcDensityPost (cudaParticleSPH_NS.cu:136-207)

@@ -5,7 +5,7 @@
 
 class cudaParticleVV : public virtual cudaParticleBase {
 public:
-  void setup(int n) { cudaParticleBase::setup(n); }
+  void setup(int n) override { cudaParticleBase::setup(n); }
   // v(t) = v(t-dt) + dt/2m (F(t)+F(t-dt)); r(t+dt) = r + dt v(t) + dt^2/2m F(t)
   void TimeEvolution(real dt) { pbamd::check(pb_vv_step(sys, dt), "cudaParticleVV::TimeEvolution"); }
   // clears F and Fold (cudaParticleVV.cu:10-15)

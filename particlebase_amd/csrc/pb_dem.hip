@@ -441,6 +441,35 @@ extern "C" int pb_dem_get_contacts(pb_sys *s, uint32_t *rowptr, uint32_t *colidx
   return PB_OK;
 }
 
+extern "C" int pb_dem_set_contacts(pb_sys *s, const uint32_t *rowptr, const uint32_t *colidx,
+                                   const double *val, uint32_t nnz) {
+  if (!s || !rowptr || (nnz && (!colidx || !val))) return PB_ERR_ARG;
+  const uint32_t n = s->n;
+  // a CSR the history merge can read: rows of original indices, columns
+  // ascending and in range
+  if (rowptr[0] != 0 || rowptr[n] != nnz)
+    return pb_fail(s->ctx, PB_ERR_ARG, "pb_dem_set_contacts: rowptr does not span nnz");
+  for (uint32_t i = 0; i < n; ++i) {
+    if (rowptr[i + 1] < rowptr[i])
+      return pb_fail(s->ctx, PB_ERR_ARG, "pb_dem_set_contacts: rowptr decreases");
+    for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+      if (colidx[k] >= n || (k > rowptr[i] && colidx[k] <= colidx[k - 1]))
+        return pb_fail(s->ctx, PB_ERR_ARG, "pb_dem_set_contacts: columns out of range or unsorted");
+  }
+  const int b = s->cm_cur < 0 ? 0 : s->cm_cur;
+  if (int rc = ensure_csr(s, b, nnz)) return rc;
+  hipStream_t st = s->ctx->stream;
+  PB_HIP(s->ctx, hipMemcpyAsync(s->cm_row[b], rowptr, 4 * ((size_t)n + 1), hipMemcpyHostToDevice, st));
+  if (nnz) {
+    PB_HIP(s->ctx, hipMemcpyAsync(s->cm_col[b], colidx, 4 * (size_t)nnz, hipMemcpyHostToDevice, st));
+    PB_HIP(s->ctx, hipMemcpyAsync(s->cm_val[b], val, 8 * (size_t)nnz, hipMemcpyHostToDevice, st));
+  }
+  PB_HIP(s->ctx, hipStreamSynchronize(st));
+  s->cm_nnz[b] = nnz;
+  s->cm_cur = b;
+  return PB_OK;
+}
+
 extern "C" int pb_dem_last_violations(pb_sys *s, uint32_t *count) {
   if (!s || !count) return PB_ERR_ARG;
   *count = s->dem_bad;

@@ -2,7 +2,8 @@
 call sequences through the class layer and the C ABI on the GPU: LJ
 (testMD1.cu order, per-call API and fused; and the testMD1 gas with
 initialAnnealing and the neighbour-list path), liquid argon under the Gaussian
-thermostat (testMD2.cu order), SPH dam break (testSPH5.cu
+thermostat (testMD2.cu order), checkpoint / restart of MD, DEM and SPH
+objects (bit-exact continuation), SPH dam break (testSPH5.cu
 order with AdaptiveTime) and DEM settle (testDEM5.cu order with
 AdaptiveTime).  Each must finish and report a physically sane end state."""
 import os
@@ -71,3 +72,12 @@ def test_md_gas(mode):
     assert float(d["dmin"]) > 2.0
     assert abs(float(d["Tanneal"]) / 353.0 - 1) < 0.05
     assert 300.0 < float(d["Tmin"]) and float(d["Tmax"]) < 1.2 * 353.0
+
+
+@pytest.mark.parametrize("mode", ["md", "dem", "sph"])
+def test_restart_exact(mode, tmp_path):
+    # writeSerialization after k steps, readSerialization into a fresh
+    # object, k more steps: the same end state as 2k uninterrupted steps
+    d = run("restart", mode, 15, str(tmp_path / f"{mode}.ck"))
+    assert int(d["k"]) == 15 and int(d["N"]) > 500
+    assert float(d["maxdiff"]) == 0.0
