@@ -182,7 +182,10 @@ typedef struct {
 
 /* Fused cudaParticleMD loop: nsteps of [force -> VV -> periodic wrap] with a
  * Verlet list re-built (and particles re-sorted into cell order) every
- * rebuild_every steps.  Starts from and ends in the reference VV state
+ * rebuild_every steps, or earlier when a particle may have moved half its
+ * replica's skin.  With several replicas, replica q's skin is
+ * skin * (T_q / T_min)^(1/4) from the temperatures at the start (capped
+ * below half its box), so hot replicas do not force rebuilds of all.  Starts from and ends in the reference VV state
  * (r(t), v(t-dt), Fold = F(t-dt)); call pb_clear_force(sys, 1) first for the
  * reference's start.  energies (may be NULL) receives, for each recorded
  * step s (s % energy_every == 0), nrep x {KE(v(t)), PE(r(t)), W(r(t))}. */

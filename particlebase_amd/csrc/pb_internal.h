@@ -29,6 +29,10 @@ struct RepInfo {
   uint32_t npad;       // slots (multiple of 64)
   uint32_t wave0, nwave;
   uint32_t seg0, nseg;   // pair-kernel work units (segments) of this replica
+  double rlist;          // Verlet list radius of this replica (rc + its skin)
+  double half_skin2;     // (skin / 2)^2: a particle beyond it invalidates the list
+  float rb2;             // FP32 list-build radius^2 (rlist + rounding margin)^2
+  float inv_hs2;         // 1 / (skin / 2)^2 (0 without a skin): displacement scale
 };
 
 struct LJTable {
@@ -110,10 +114,9 @@ struct pb_sys {
   uint8_t *move2 = nullptr;
   double4 *a2buf = nullptr;
   uint32_t list_ncell = 0;
-  double list_rlist = 0.0;
+  double list_rlist = 0.0;         // largest list radius over the replicas
   uint32_t *list_segs = nullptr;   // list-build work units (5 words each)
   uint32_t list_nsegs = 0;
-  float list_rb2 = 0.f;            // FP32 build radius^2 (rlist + rounding margin)
   uint32_t tile_w = 2;             // list-cell columns per tile side (1 or 2)
   double tile_fill = 0.72;         // target fill of the LDS window stage
   uint32_t last_rebuilds = 0, last_maxnn = 0;

@@ -394,11 +394,12 @@ __device__ __forceinline__ void tile_particle(const WinTable &W, uint32_t u, uin
 // load whole words and the 16 particles of a wavefront read adjacent words.
 // Lists are padded with NL_PAD to a multiple of 16 entries (nn holds the
 // padded length).
-__global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint32_t *cstart,
+// six wavefronts per SIMD (<= 80 VGPRs with the granule of 8): with the
+// 38.6 KB stage, four workgroups per CU
+__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_build_list(const double4 *r, const uint32_t *cstart,
                                                    const ListSeg *segs, const RepInfo *reps,
-                                                   uint32_t npad, double rlist2, float rb2,
-                                                   uint32_t maxnb, uint16_t *nl, uint32_t *nn,
-                                                   uint32_t *counter) {
+                                                   uint32_t npad, uint32_t maxnb, uint16_t *nl,
+                                                   uint32_t *nn, uint32_t *counter) {
   __shared__ StageB S;
   const ListSeg G = segs[pb_xcd_block()];
   const RepInfo &R = reps[G.rep];
@@ -414,6 +415,8 @@ __global__ void __launch_bounds__(TB) k_build_list(const double4 *r, const uint3
     return;
   }
   const bool staged = W.total <= WCAPB;
+  const double rlist2 = R.rlist * R.rlist;
+  const float rb2 = R.rb2;
   // window origin (FP64): lower corner of the first window cell
   const double ox = R.lo[0] + ((double)G.ci - 1.0) * R.L[0] / R.nb[0];
   const double oy = R.lo[1] + ((double)G.cj - 1.0) * R.L[1] / R.nb[1];
@@ -561,7 +564,7 @@ struct StepArgs {
   uint32_t *mon;        // host-mapped ring (8 slots x 4 words) or null
   uint32_t step;        // index of this step inside the run
   uint32_t npad, nrep;
-  double dt, rc2, half_skin2;
+  double dt, rc2;
   // single-type parameters
   double s2, e24, e4, shift;
   float s2f, e24f, e4f, shiftf;
@@ -609,6 +612,9 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
   const uint32_t lane = pb_lane(), sl = lane & 3u, grp = lane >> 2;
   const uint32_t wave = threadIdx.x / 64, nwv = blockDim.x / 64;
   const uint32_t PAD = FAST ? (uint32_t)WCAP : NL_PAD;
+  // replica constants read once, before any store (stores could alias them)
+  const double half_skin2 = R.half_skin2;
+  const float inv_hs2 = R.inv_hs2;
   const size_t ld = A.npad;
   // Software pipeline across particle groups: while group u is evaluated,
   // the next group's slot, list length, first list batch and integration
@@ -758,7 +764,7 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
       {
         const float ex = (float)(p.x - (double)rb.x), ey = (float)(p.y - (double)rb.y),
                     ez = (float)(p.z - (double)rb.z);
-        if ((double)(ex * ex + ey * ey + ez * ez) > A.half_skin2) atomicOr(&A.counter[CNT_SKIN], 1u);
+        if ((double)(ex * ex + ey * ey + ez * ez) > half_skin2) atomicOr(&A.counter[CNT_SKIN], 1u);
       }
       // kick-drift form of propagateVelocityVerlet_F4 (kernelfuncs.cu:
       // 347-365): v holds v(t - dt/2) = v_ref(t-dt) + Fold dt/2m
@@ -776,8 +782,9 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
       // displacement of the new positions feeds the rebuild schedule
       const float ddx = (float)(p.x - (double)rb.x), ddy = (float)(p.y - (double)rb.y),
                   ddz = (float)(p.z - (double)rb.z);
-      d2m = fmaxf(d2m, ddx * ddx + ddy * ddy + ddz * ddz);
-      v2m = fmaxf(v2m, (float)(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z));
+      // in units of the replica's half skin (replicas may have different skins)
+      d2m = fmaxf(d2m, (ddx * ddx + ddy * ddy + ddz * ddz) * inv_hs2);
+      v2m = fmaxf(v2m, (float)(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z) * inv_hs2);
     }
     if (WRITE_F) A.Fout[gi] = make_double4(fx, fy, fz, 0.0);
     if (ENERGY) {
@@ -873,21 +880,32 @@ __global__ void __launch_bounds__(TBS) k_lj_step_wide(StepArgs A) {
 }
 
 // ------------------------------------------------------------- host side
-static uint32_t expected_maxnb(const pb_sys *s, double rlist) {
-  double rho = 0.0;
+static uint32_t expected_maxnb(const pb_sys *s) {
+  double mean = 0.0;
   for (uint32_t q = 0; q < s->nrep; ++q) {
     const RepInfo &R = s->reps[q];
-    rho = std::max(rho, (double)R.natoms / (R.L[0] * R.L[1] * R.L[2]));
+    const double rho = (double)R.natoms / (R.L[0] * R.L[1] * R.L[2]);
+    mean = std::max(mean, 4.0 / 3.0 * M_PI * R.rlist * R.rlist * R.rlist * rho);
   }
-  const double mean = 4.0 / 3.0 * M_PI * rlist * rlist * rlist * rho;
   uint32_t m = (uint32_t)std::ceil(1.5 * mean + 48.0);
   return (m + 15u) & ~15u;
 }
 
-static int setup_list_cells(pb_sys *s, double rlist) {
+// List cells, radii and segments for cutoff rc with skin[q] per replica.
+static int setup_list_cells(pb_sys *s, double rc, const std::vector<double> &skin) {
   uint32_t cell0 = 0;
+  double rmax_all = 0.0;
   for (uint32_t q = 0; q < s->nrep; ++q) {
     RepInfo &R = s->reps[q];
+    const double rlist = rc + skin[q];
+    rmax_all = std::max(rmax_all, rlist);
+    R.rlist = rlist;
+    // FP32 build radius: window-relative coordinates stay below ~4 list
+    // cells plus the drift since the last wrap; 1e-4 covers their rounding
+    R.rb2 = (float)((rlist + 1e-4) * (rlist + 1e-4));
+    const double hs = 0.5 * skin[q];
+    R.half_skin2 = hs * hs;
+    R.inv_hs2 = hs > 0.0 ? (float)(1.0 / (hs * hs)) : 0.0f;
     for (int a = 0; a < 3; ++a) {
       if (!(2.0 * rlist < R.L[a]))
         return pb_fail(s->ctx, PB_ERR_CELLS, "list cutoff must be below half the box");
@@ -900,7 +918,7 @@ static int setup_list_cells(pb_sys *s, double rlist) {
     cell0 += R.nb[0] * R.nb[1] * R.nb[2] + 1;   // + pad cell
   }
   s->list_ncell = cell0;
-  s->list_rlist = rlist;
+  s->list_rlist = rmax_all;
   // Tiles (segments) of tx x ty columns x nz cells.  nz is chosen so the
   // expected window (tx+2)(ty+2)(nz+2) cells fills tile_fill of the LDS
   // stage (0.72 unless pb_lj_autotune chose otherwise);
@@ -935,9 +953,6 @@ static int setup_list_cells(pb_sys *s, double rlist) {
   if (int rc = pb_alloc(s->ctx, &s->list_segs, 8 * segs.size())) return rc;
   PB_HIP(s->ctx, hipMemcpyAsync(s->list_segs, segs.data(), sizeof(ListSeg) * segs.size(),
                                 hipMemcpyHostToDevice, s->ctx->stream));
-  // FP32 build radius: window-relative coordinates stay below ~4 list cells
-  // plus the drift since the last wrap; 1e-4 covers their FP32 rounding
-  s->list_rb2 = (float)((rlist + 1e-4) * (rlist + 1e-4));
   // per-segment energy partials
   if (s->list_nsegs > s->nwave) {
     if (int rc = pb_alloc(s->ctx, &s->partials, 4 * (size_t)s->list_nsegs)) return rc;
@@ -992,8 +1007,8 @@ static int rebuild(pb_sys *s) {
   std::swap(s->type, s->type2);
   std::swap(s->move, s->move2);
   hipLaunchKernelGGL(k_build_list, dim3(s->list_nsegs), dim3(TB), 0, st, s->r, s->cstart,
-                     (const ListSeg *)s->list_segs, s->d_reps, n, s->list_rlist * s->list_rlist,
-                     s->list_rb2, s->maxnb, s->nl, s->nn, s->d_counter);
+                     (const ListSeg *)s->list_segs, s->d_reps, n, s->maxnb, s->nl, s->nn,
+                     s->d_counter);
   PB_LAUNCH_CHECK(s->ctx, "k_build_list");
   s->ref_binned = false;
   return PB_OK;
@@ -1015,8 +1030,6 @@ static StepArgs make_args(pb_sys *s, double dt) {
   A.nrep = s->nrep;
   A.dt = dt;
   A.rc2 = s->rc2;
-  const double hs = 0.5 * (s->list_rlist - std::sqrt(s->rc2));
-  A.half_skin2 = hs * hs;
   A.s2 = s->lj.s2[0];
   A.e24 = s->lj.e24[0];
   A.e4 = s->lj.e4[0];
@@ -1100,8 +1113,8 @@ static int reset_counters(pb_sys *s) {
 }
 
 // build (or grow) the list so that no particle overflows it
-static int build_checked(pb_sys *s, double rlist) {
-  uint32_t want = std::max(s->maxnb, expected_maxnb(s, rlist));
+static int build_checked(pb_sys *s) {
+  uint32_t want = std::max(s->maxnb, expected_maxnb(s));
   for (int attempt = 0; attempt < 4; ++attempt) {
     if (int rc = ensure_list_buffers(s, want)) return rc;
     if (int rc = reset_counters(s)) return rc;
@@ -1124,8 +1137,8 @@ extern "C" int pb_lj_force(pb_sys *s, int precision, double *pe, double *virial)
   if (int rc = check_lj_ready(s)) return rc;
   PB_HIP(s->ctx, hipSetDevice(s->ctx->device));
   const double rc = std::sqrt(s->rc2);
-  if (int e = setup_list_cells(s, rc * (1.0 + 1e-12))) return e;
-  if (int e = build_checked(s, rc * (1.0 + 1e-12))) return e;
+  if (int e = setup_list_cells(s, rc, std::vector<double>(s->nrep, rc * 1e-12))) return e;
+  if (int e = build_checked(s)) return e;
   StepArgs A = make_args(s, 0.0);
   A.rin = s->r;
   A.vin = s->v;
@@ -1189,8 +1202,36 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
   PB_HIP(s->ctx, hipSetDevice(s->ctx->device));
   hipStream_t st = s->ctx->stream;
   const bool mixed = o->precision == PB_MIXED;
-  const double rlist = std::sqrt(s->rc2) + o->skin;
-  if (int rc = setup_list_cells(s, rlist)) return rc;
+  // Per-replica skins: replica q's particles move with its thermal speed, so
+  // a hot replica would reach a common skin first and force rebuilds of
+  // all.  Its skin is skin * (T_q / T_min)^(1/4) (T from sum m v^2 per atom
+  // at the start), capped below half its box.  The exponent 1/2 would make
+  // every replica reach its skin after the same number of steps; 1/4 trades
+  // a few more rebuilds for shorter lists of the hot replicas and measured
+  // best on the 256-replica set (2.57e9 atom-steps/s against 2.48e9 with
+  // 1/2, 2.54e9 with 1/3, 2.39e9 with 1/8 and 2.27e9 with one skin).  One
+  // replica keeps the given skin.
+  std::vector<double> skins(s->nrep, o->skin);
+  if (s->nrep > 1 && o->skin > 0) {
+    std::vector<double> mv2(s->nrep);
+    if (int rc = pb_mv2(s, mv2.data())) return rc;
+    double tmin = 0.0;
+    std::vector<double> tq(s->nrep, 0.0);
+    for (uint32_t q = 0; q < s->nrep; ++q) {
+      if (!s->reps[q].natoms) continue;
+      tq[q] = mv2[q] / s->reps[q].natoms;
+      if (tq[q] > 0 && (tmin == 0.0 || tq[q] < tmin)) tmin = tq[q];
+    }
+    const double rc0 = std::sqrt(s->rc2);
+    for (uint32_t q = 0; q < s->nrep && tmin > 0; ++q) {
+      const RepInfo &R = s->reps[q];
+      const double lmin = std::min(R.L[0], std::min(R.L[1], R.L[2]));
+      const double cap = 0.5 * lmin * (1.0 - 1e-9) - rc0;
+      const double sk = o->skin * std::pow(std::max(tq[q], tmin) / tmin, 0.25);
+      skins[q] = std::max(o->skin, std::min(sk, cap));
+    }
+  }
+  if (int rc = setup_list_cells(s, std::sqrt(s->rc2), skins)) return rc;
   const uint32_t ne = o->energy_every;
   const uint32_t nrec = ne ? (o->nsteps + ne - 1) / ne : 0;
   double *ebuf = nullptr;
@@ -1226,14 +1267,13 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
   // reference VV state -> kick-drift state
   hipLaunchKernelGGL(k_to_half, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->v, s->a,
                      s->npad, o->dt);
-  if ((rc = build_checked(s, rlist))) goto done;
+  if ((rc = build_checked(s))) goto done;
   ++rebuilds;
   if ((rc = reset_counters(s))) goto done;
   {
     StepArgs A = make_args(s, o->dt);
     A.energy = ebuf;
     A.mon = s->dmon;
-    const double half_skin = 0.5 * o->skin;
     uint32_t last_build = 0;
     uint32_t step = 0;
     // Graph mode: whole rebuild periods -- R steps followed by the list
@@ -1364,7 +1404,8 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
           float d2, v2;
           std::memcpy(&d2, &ud, 4);
           std::memcpy(&v2, &uv, 4);
-          if (std::sqrt((double)d2) + 1.5 * std::sqrt((double)v2) * o->dt >= half_skin) reb = true;
+          // maxima are in units of each replica's half skin
+          if (std::sqrt((double)d2) + 1.5 * std::sqrt((double)v2) * o->dt >= 1.0) reb = true;
         }
       }
       if (reb) {

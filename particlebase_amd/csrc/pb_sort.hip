@@ -157,57 +157,86 @@ __global__ void k_scatter(const uint32_t *cellid, const uint32_t *rank, const ui
 }
 
 // 256 consecutive cells per workgroup own one contiguous range of perm.
-// The range and its keys are staged in LDS, every thread insertion-sorts
-// its cell there, and the range is written back.  A range larger than the
-// LDS stage (a pathological pile-up) is sorted in global memory instead.
+// The range is sorted in passes: each pass stages the longest run of the
+// remaining cells whose particles fit the LDS stage (at least one cell).
+// Every element then finds its cell and its rank in it -- the number of
+// elements of the cell with a smaller key (equal keys: smaller position) --
+// and is written to cell start + rank.  The rank loops are independent LDS
+// reads, so a crowded cell costs O(n) per element in parallel instead of a
+// serial O(n^2) insertion sort.  A single cell larger than the stage (a
+// pathological pile-up) is insertion-sorted in global memory by its thread.
 #define CS_CELLS 256
 #define CS_CAP 6144
 __global__ void __launch_bounds__(CS_CELLS) k_cell_sort(const uint32_t *start, uint32_t ncell,
                                                         const uint32_t *key, uint32_t *perm) {
   __shared__ uint32_t sv[CS_CAP], sk[CS_CAP];
+  __shared__ uint32_t cst[CS_CELLS + 1];
+  __shared__ uint32_t s_next;
   const uint32_t c0 = blockIdx.x * CS_CELLS;
   const uint32_t c1 = min(c0 + CS_CELLS, ncell);
-  const uint32_t base = start[c0], end = start[c1];
-  const uint32_t len = end - base;
   const uint32_t c = c0 + threadIdx.x;
-  if (len <= CS_CAP) {
-    for (uint32_t q = threadIdx.x; q < len; q += blockDim.x) {
-      const uint32_t p = perm[base + q];
-      sv[q] = p;
-      sk[q] = key[p];
+  // one pass when the whole range fits (the common case); otherwise runs of
+  // cells found by thread 0
+  const bool whole = start[c1] - start[c0] <= CS_CAP;
+  for (uint32_t ca = c0; ca < c1;) {                 // block-uniform
+    if (whole) {
+      if (threadIdx.x == 0) s_next = c1;
+    } else if (threadIdx.x == 0) {
+      // largest cb in [ca + 1, c1] with start[cb] - start[ca] <= CS_CAP
+      const uint32_t lim = start[ca] + CS_CAP;
+      uint32_t lo = ca + 1, hi = c1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= lim) lo = mid;
+        else hi = mid - 1;
+      }
+      s_next = lo;
     }
     __syncthreads();
-    if (c < c1) {
-      const uint32_t b = start[c] - base, e = start[c + 1] - base;
+    const uint32_t cb = s_next;
+    const uint32_t base = start[ca], len = start[cb] - base;
+    if (len <= CS_CAP) {
+      const uint32_t nc = cb - ca;
+      for (uint32_t k = threadIdx.x; k <= nc; k += blockDim.x) cst[k] = start[ca + k] - base;
+      for (uint32_t q = threadIdx.x; q < len; q += blockDim.x) {
+        const uint32_t p = perm[base + q];
+        sv[q] = p;
+        sk[q] = key[p];
+      }
+      __syncthreads();
+      for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
+        uint32_t lo = 0, hi = nc;                    // cst[lo] <= i < cst[hi]
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (cst[mid] <= i) lo = mid;
+          else hi = mid;
+        }
+        const uint32_t b = cst[lo], e = cst[lo + 1];
+        const uint32_t ki = sk[i];
+        uint32_t rank = 0;
+        for (uint32_t j = b; j < e; ++j) {
+          const uint32_t kj = sk[j];
+          rank += (kj < ki || (kj == ki && j < i)) ? 1u : 0u;
+        }
+        perm[base + b + rank] = sv[i];
+      }
+    } else if (c >= ca && c < cb) {
+      const uint32_t b = start[c], e = start[c + 1];
       for (uint32_t i = b + 1; i < e; ++i) {
-        const uint32_t p = sv[i], kp = sk[i];
+        const uint32_t p = perm[i];
+        const uint32_t kp = key[p];
         uint32_t j = i;
-        while (j > b && sk[j - 1] > kp) {
-          sv[j] = sv[j - 1];
-          sk[j] = sk[j - 1];
+        while (j > b) {
+          const uint32_t q = perm[j - 1];
+          if (key[q] <= kp) break;
+          perm[j] = q;
           --j;
         }
-        sv[j] = p;
-        sk[j] = kp;
+        perm[j] = p;
       }
     }
-    __syncthreads();
-    for (uint32_t q = threadIdx.x; q < len; q += blockDim.x) perm[base + q] = sv[q];
-    return;
-  }
-  if (c >= c1) return;
-  const uint32_t b = start[c], e = start[c + 1];
-  for (uint32_t i = b + 1; i < e; ++i) {
-    const uint32_t p = perm[i];
-    const uint32_t kp = key[p];
-    uint32_t j = i;
-    while (j > b) {
-      const uint32_t q = perm[j - 1];
-      if (key[q] <= kp) break;
-      perm[j] = q;
-      --j;
-    }
-    perm[j] = p;
+    __syncthreads();                                 // LDS and s_next are reused
+    ca = cb;
   }
 }
 

@@ -176,6 +176,32 @@ def test_batched_equals_single(ctx):
         off += w.n
 
 
+def test_per_replica_skin(ctx):
+    say("hot and cold replicas in one run: per-replica skins, fewer rebuilds")
+    hot = workloads.lj_fcc(8, 0.8, 2.0, seed=41)
+    cold = workloads.lj_fcc(8, 0.8, 0.7, seed=42)
+    batch = workloads.Batch([cold, hot])
+    r, v, m, t, cells = batch.concat()
+    s = pb.System(ctx, batch.counts)
+    s.set_cell(cells)
+    s.import_particles(r, m, v=v, type=t)
+    s.lj_set_params([1.0, 1.0], 1)
+    s.lj_set_cutoff(RC2, True)
+    s.clear_force(True)
+    s.lj_run(0.005, 200, precision=pb.PB_FP64, skin=0.3, rebuild_every=100, energy_every=0)
+    st = s.lj_run_stats()
+    assert st["skin_violation"] == 0
+    alone = lj_sys(ctx, hot)
+    alone.clear_force(True)
+    alone.lj_run(0.005, 200, precision=pb.PB_FP64, skin=0.3, rebuild_every=100, energy_every=0)
+    # the hot replica got a larger skin in the batch, so the batch rebuilds
+    # less often than the hot replica alone with the base skin
+    assert st["rebuilds"] < alone.lj_run_stats()["rebuilds"]
+    d = s.get("r")[cold.n:] - alone.get("r")
+    d -= hot.L * np.rint(d / hot.L)
+    assert np.abs(d).max() < 1e-9
+
+
 def test_small_boxes_dedup(ctx, orc):
     say("small box (fewer than 3 list cells per axis)")
     w = jittered(workloads.lj_fcc(4, rho=0.9), 0.05, 9)     # N=256, L=6.6 -> 2 cells
