@@ -68,6 +68,18 @@ def timed_run(ctx, s, steps, precision, skin, rebuild_every, energy_every, timin
     return time.perf_counter() - t0, e
 
 
+def choose_tiles(s, a, precision):
+    """The chosen tile shape: the autotuner's (pb_lj_autotune, timed from the
+    warmed-up state, which it restores), or the one given on the command
+    line (profiling runs, so the kernel trace holds only timed launches)."""
+    if a.tile_fill > 0:
+        s.lj_set_tiles(a.tile_w or 2, a.tile_fill)
+        return {"tile_w": a.tile_w or 2, "tile_fill": a.tile_fill, "autotuned": False}
+    t = s.lj_autotune(DT, steps=20, precision=precision, skin=a.skin,
+                      rebuild_every=a.rebuild_every)
+    return {"tile_w": t["tile_w"], "tile_fill": t["tile_fill"], "autotuned": True}
+
+
 def drift_per_1e4(e, n, energy_every):
     et = (e[:, 0, 0] + e[:, 0, 1]) / n
     if et.size < 4:
@@ -198,6 +210,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--skin", type=float, default=0.3)
     ap.add_argument("--rebuild-every", type=int, default=20)
+    ap.add_argument("--tile-w", type=int, default=0,
+                    help="fixed tile shape (with --tile-fill) instead of the autotuner")
+    ap.add_argument("--tile-fill", type=float, default=0.0)
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--plain-steps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
@@ -238,6 +253,8 @@ def main():
     # ---- mixed (chosen) form
     s = make_system(ctx, w)
     timed_run(ctx, s, a.warmup, pb.PB_MIXED, a.skin, a.rebuild_every, 0, timing=False)
+    tiles = choose_tiles(s, a, pb.PB_MIXED)
+    log(f"tiles: {tiles}")
     el, e = timed_run(ctx, s, a.steps, pb.PB_MIXED, a.skin, a.rebuild_every, 10)
     tm = s.lj_run_timing()
     stats = s.lj_run_stats()
@@ -262,6 +279,7 @@ def main():
     # ---- plain FP64 form
     s = make_system(ctx, w)
     timed_run(ctx, s, a.warmup, pb.PB_FP64, a.skin, a.rebuild_every, 0, timing=False)
+    tiles64 = choose_tiles(s, a, pb.PB_FP64)
     el64, e64 = timed_run(ctx, s, a.steps, pb.PB_FP64, a.skin, a.rebuild_every, 10)
     tm64 = s.lj_run_timing()
     value64 = w.n * a.steps / el64
@@ -282,11 +300,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(),
                      "kernel": "k_lj_step (mixed, fused force+VV)",
                      "kernel_us": kern_ms * 1e3, "alg_bytes_per_atom": ALG_BYTES},
-        "plain_fp64": {"value": value64, "ms_per_step": 1e3 * el64 / a.steps,
+        "plain_fp64": {"value": value64, "ms_per_step": 1e3 * el64 / a.steps, "tiles": tiles64,
                        "kernel_us": kern64 * 1e3,
                        "roofline_frac": ALG_BYTES * w.n / (kern64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
                        "energy_drift_per_1e4_steps": drift_per_1e4(e64, w.n, 10)},
         "mixed": {"force_err_max_over_frms": ferr, "energy_drift_per_1e4_steps": drift,
+                  "tiles": tiles,
                   "max_nn": stats["max_nn"], "skin_violation": stats["skin_violation"],
                   "rebuild_ms_avg": tm["rebuild_ms"] / max(tm["nrebuild"], 1)},
     }
@@ -297,6 +316,7 @@ def main():
         ntot = int(reps.counts.sum())
         sb = make_system(ctx, reps)
         timed_run(ctx, sb, 10, pb.PB_MIXED, a.skin, a.rebuild_every, 0, timing=False)
+        tiles_b = choose_tiles(sb, a, pb.PB_MIXED)
         elb, eb = timed_run(ctx, sb, a.batch_steps, pb.PB_MIXED, a.skin, a.rebuild_every, 0)
         vb = ntot * a.batch_steps / elb
         del sb
@@ -315,7 +335,7 @@ def main():
         vp = ntot * a.plain_steps / elp
         del plain_systems
         log(f"plain sequential FP64: {vp:.4g} atom-steps/s")
-        out["batched"] = {"replicas": len(reps.systems), "atoms": ntot, "value": vb,
+        out["batched"] = {"replicas": len(reps.systems), "atoms": ntot, "value": vb, "tiles": tiles_b,
                           "plain_sequential_fp64": vp, "speedup": vb / vp,
                           "steps": a.batch_steps, "plain_steps": a.plain_steps}
 

@@ -203,12 +203,16 @@ int pb_lj_last_run_timing(pb_sys *sys, double *step_ms, uint32_t *nstep,
 /* Tile-shape autotuner of the fused LJ run (no reference counterpart; the
  * reference's launch shapes come from CUDAenv::setThnum, CUDAenv.hh:36-50):
  * times `steps` fused steps with opts for each candidate tile shape (1 or 2
- * list-cell columns per tile side x 4 LDS fill targets) from the same saved
+ * list-cell columns per tile side x 6 LDS fill targets, 0.55 to 1) from the same saved
  * state, keeps the fastest for later runs of this system and restores the
  * state.  Outputs (may be NULL): ms per step of the best candidate, its
  * columns per side and fill. */
 int pb_lj_autotune(pb_sys *sys, const pb_run_opts *opts, uint32_t steps, double *best_ms,
                    uint32_t *best_w, double *best_fill);
+/* Sets the tile shape of later fused runs directly (no reference
+ * counterpart): tile_w = 1 or 2 list-cell columns per tile side, tile_fill
+ * in [0.1, 1] the target fill of the LDS window stage (default 2, 0.72). */
+int pb_lj_set_tiles(pb_sys *sys, uint32_t tile_w, double tile_fill);
 
 /* ------------------------------------------------------------------ g(r) */
 /* cudaCalcGR::setup + makePairInfo + calcgr (cudaCalcGR.hh:25-39,

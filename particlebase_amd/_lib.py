@@ -49,7 +49,7 @@ EXPORTS = [
     "pb_dem_get_contacts", "pb_dem_last_violations",
     "pb_gt_step", "pb_gt_xi", "pb_adjust_velocities", "pb_const_temp",
     "pb_nlist_build", "pb_nlist_get", "pb_lj_force_nlist", "pb_softcore_force", "pb_euler_step",
-    "pb_dem_set_contacts",
+    "pb_dem_set_contacts", "pb_lj_set_tiles",
 ]
 
 
@@ -156,6 +156,7 @@ def _extra_signatures():
         "pb_softcore_force": (i32, [vp, d, d, d]),
         "pb_euler_step": (i32, [vp, d]),
         "pb_dem_set_contacts": (i32, [vp, P(u32), P(u32), P(d), u32]),
+        "pb_lj_set_tiles": (i32, [vp, u32, d]),
         "pb_dem_calc_force": (i32, [vp, d, i32, P(u32)]),
         "pb_dem_get_contacts": (i32, [vp, P(u32), P(u32), P(d), u32, P(u32)]),
         "pb_dem_last_violations": (i32, [vp, P(u32)]),
@@ -466,6 +467,9 @@ class System:
         self._c(self.lib.pb_lj_autotune(self.h, C.byref(o), steps, C.byref(ms), C.byref(w),
                                         C.byref(f)))
         return {"ms_per_step": ms.value, "tile_w": w.value, "tile_fill": f.value}
+
+    def lj_set_tiles(self, tile_w, tile_fill):
+        self._c(self.lib.pb_lj_set_tiles(self.h, int(tile_w), float(tile_fill)))
 
     def lj_run_stats(self):
         rb = C.c_uint32()

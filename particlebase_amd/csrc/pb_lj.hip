@@ -1520,6 +1520,15 @@ extern "C" int pb_lj_last_run_stats(pb_sys *s, uint32_t *rebuilds, uint32_t *max
 // Every candidate runs `steps` fused steps from the same saved state, timed
 // with HIP events on the context's stream; the fastest is kept and the
 // state is restored, so the call leaves the simulation where it was.
+extern "C" int pb_lj_set_tiles(pb_sys *s, uint32_t tile_w, double tile_fill) {
+  if (!s) return PB_ERR_ARG;
+  if ((tile_w != 1 && tile_w != 2) || !(tile_fill >= 0.1 && tile_fill <= 1.0))
+    return pb_fail(s->ctx, PB_ERR_ARG, "pb_lj_set_tiles: tile_w must be 1 or 2, tile_fill in [0.1, 1]");
+  s->tile_w = tile_w;
+  s->tile_fill = tile_fill;
+  return PB_OK;
+}
+
 extern "C" int pb_lj_autotune(pb_sys *s, const pb_run_opts *o, uint32_t steps, double *best_ms,
                               uint32_t *best_w, double *best_fill) {
   if (!s || !o || steps < 2) return PB_ERR_ARG;
@@ -1558,7 +1567,7 @@ extern "C" int pb_lj_autotune(pb_sys *s, const pb_run_opts *o, uint32_t steps, d
     return PB_OK;
   };
   const uint32_t ws[2] = {1, 2};
-  const double fills[4] = {0.4, 0.55, 0.72, 0.9};
+  const double fills[6] = {0.55, 0.72, 0.85, 0.9, 0.95, 1.0};
   const uint32_t w0 = s->tile_w;
   const double f0 = s->tile_fill;
   double best = 1e300;
