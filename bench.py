@@ -109,16 +109,23 @@ def cpu_baseline(w, budget_s=20.0):
                       f"({el:.1f} s, FP64, OpenMP {cores} threads)"}
 
 
-def bench_sph(ctx, a):
-    """SPH dam break (reference physics), the reference driver's step:
-    bin, density, acceleration, gravity, leap-frog, inspectVelocity,
-    absolute boundary."""
-    w = workloads.sph_dam_break(0.08 if a.quick else 0.4 / 63)
+def bench_sph(ctx, a, config=False):
+    """SPH dam break, the reference driver's step: bin, density,
+    acceleration, gravity, leap-frog, inspectVelocity, absolute boundary.
+    Reference physics (Lucy, support h = 1.7 dx) or the config physics
+    (cubic spline of support 2h, h = 1.3 dx, Tait, artificial viscosity)."""
+    dx = 0.08 if a.quick else 0.4 / 63
+    w = workloads.sph_dam_break_config(dx) if config else workloads.sph_dam_break(dx)
     s = pb.System(ctx, [w.n])
     s.set_cell(w.cell)
     s.import_particles(w.r, w.m, is_fixed=w.fixed)
-    s.sph_set_properties(w.mu0, w.c, w.h)
-    s.setup_cells(w.h, periodic=False)
+    if config:
+        s.sph_set_wcsph(w.h, w.rho0, w.c[0], 7.0, 0.1, 0.0)
+        sup = 2 * w.h
+    else:
+        s.sph_set_properties(w.mu0, w.c, w.h)
+        sup = w.h
+    s.setup_cells(sup, periodic=False)
     dt = 0.25 * w.h / w.c[0]
 
     def step():
@@ -127,7 +134,7 @@ def bench_sph(ctx, a):
         s.sph_acceleration()
         s.add_acceleration(2, -9.81)
         s.lf_step(dt)
-        s.inspect_velocity(2 * w.h / dt, 0.04, 0.0025)
+        s.inspect_velocity(2 * sup / dt, 0.04, 0.0025)
         s.treat_absolute()
 
     for _ in range(3):
@@ -139,9 +146,10 @@ def bench_sph(ctx, a):
     ctx.sync()
     el = time.perf_counter() - t0
     v = w.n * a.sph_steps / el
-    log(f"sph: N={w.n} ({int((w.fixed == 0).sum())} fluid), {v:.4g} particle-steps/s, "
+    kind = "cubic spline h=1.3dx, Tait, artificial viscosity" if config else "Lucy h=1.7dx"
+    log(f"sph ({kind}): N={w.n} ({int((w.fixed == 0).sum())} fluid), {v:.4g} particle-steps/s, "
         f"{1e3 * el / a.sph_steps:.3f} ms/step")
-    return {"workload": f"SPH dam break, Lucy h=1.7dx, dx={w.dx:.6g}, {int((w.fixed == 0).sum())} "
+    return {"workload": f"SPH dam break, {kind}, dx={w.dx:.6g}, {int((w.fixed == 0).sum())} "
                         f"fluid + {int(w.fixed.sum())} wall particles, FP64",
             "value": v, "unit": "particle-steps/s", "ms_per_step": 1e3 * el / a.sph_steps,
             "steps": a.sph_steps}
@@ -341,6 +349,7 @@ def main():
 
     if not a.no_sph:
         out["sph"] = bench_sph(ctx, a)
+        out["sph_config"] = bench_sph(ctx, a, config=True)
     if not a.no_dem:
         out["dem"] = bench_dem(ctx, a)
 

@@ -236,6 +236,20 @@ int pb_gr_hist(pb_sys *sys, const float *rstep, uint32_t rnum,
  * w0 = 105 / (16 pi h^7). */
 int pb_sph_set_properties(pb_sys *sys, const double *mu0, const double *c, double h);
 
+/* The dam-break workload's config physics behind the same SPH calls
+ * (pb_sph_density, pb_sph_acceleration, pb_sph_inspect_density).  No
+ * counterpart in the reference's kernels, whose SPH is the Lucy form above;
+ * it follows the published weakly compressible method: M4 cubic spline of
+ * support 2h (Monaghan & Lattanzio 1985), summation density with the self
+ * term, Tait equation of state p = max(B ((rho/rho0)^gamma - 1), pmin) with
+ * B = rho0 c0^2 / gamma, and Monaghan's artificial viscosity (alpha, beta = 0,
+ * eps = 0.01, sound speed c0).  Masses are the particles' own.  Cells:
+ * pb_setup_cells(sys, 2h, 0, 0).  The third component of PB_FIELD_SPH_NUM
+ * is p here (mu under the reference physics).  pb_sph_set_properties
+ * switches back to the reference physics. */
+int pb_sph_set_wcsph(pb_sys *sys, double h, double rho0, double c0, double gamma,
+                     double alpha, double pmin);
+
 /* cudaParticleSPH_NS::calcDensity + calcDensityPost (cudaParticleSPH_NS.cu:
  * 136-207; SPHcalcDensity<SPHKernelLucy>, kerneltemplate.hh:636-656):
  * number density, rho = sum m_j W (self included), 1/rho and the field mu;

@@ -170,6 +170,21 @@ void pbo_sph_density(const double *r, const double *m, const double *mu0, uint32
 void pbo_sph_accel(const double *r, const double *v, const double *m, const double *c2,
                    const double *rhoinv, const double *mu, uint32_t n, double h,
                    double rho0, double *a);
+/* Config physics of the dam-break workload (pb_oracle_sph.c): M4 cubic
+ * spline of support 2h, summation density, Tait EOS clamped at pmin,
+ * Monaghan artificial viscosity with mean sound speed c0; no counterpart in
+ * the reference's kernels.  pinned by: int W 4 pi r^2 dr = 1, W and dW/dr
+ * continuous at q = 1 and 2, dW/dr against a finite difference of W, the
+ * lattice sum, single-particle rho = m/(pi h^3), gamma = 1 reducing to the
+ * reference EOS c0^2 (rho - rho0), sum m_i a_i = 0, sum m_i r_i x a_i = 0,
+ * viscous dissipation <= 0 and none between receding pairs
+ * (tests/test_oracle_sph.py). */
+double pbo_wc_w(double r, double h);
+double pbo_wc_dw(double r, double h);
+void pbo_wc_density(const double *r, const double *m, uint32_t n, double h, double rho0,
+                    double c0, double gamma, double pmin, double *num_out, double *rhoinv);
+void pbo_wc_accel(const double *r, const double *v, const double *m, const double *rho,
+                  const double *p, uint32_t n, double h, double c0, double alpha, double *a);
 /* propagateLeapFrog_F4 (kernelfuncs.cu:270-296), FP64 */
 void pbo_lf_step(double *r, double *v, const double *a, const uint8_t *move, uint32_t n,
                  double dt);

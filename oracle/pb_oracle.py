@@ -72,6 +72,10 @@ def _extra(L):
         ("pbo_sph_lucy_dw", d, [d, d, d]),
         ("pbo_sph_density", None, [P(d), P(d), P(d), u32, d, P(d), P(d)]),
         ("pbo_sph_accel", None, [P(d), P(d), P(d), P(d), P(d), P(d), u32, d, d, P(d)]),
+        ("pbo_wc_w", d, [d, d]),
+        ("pbo_wc_dw", d, [d, d]),
+        ("pbo_wc_density", None, [P(d), P(d), u32, d, d, d, d, d, P(d), P(d)]),
+        ("pbo_wc_accel", None, [P(d), P(d), P(d), P(d), P(d), u32, d, d, d, P(d)]),
         ("pbo_lf_step", None, [P(d), P(d), P(d), P(u8), u32, d]),
         ("pbo_dem_contacts", u32, [P(d), P(d), P(u8), u32, P(u32), P(u32)]),
         ("pbo_dem_carry", None, [P(u32), P(u32), P(d), P(u32), P(u32), P(d), u32]),
@@ -298,6 +302,38 @@ def sph_accel(r, v, m, c2, rhoinv, mu, h, rho0=1.0):
     d = C.c_double
     lib().pbo_sph_accel(_p(r, d), _p(v, d), _p(m, d), _p(c2, d), _p(rhoinv, d), _p(mu, d), n,
                         h, rho0, _p(a, d))
+    return a
+
+
+def wc_w(r, h):
+    return lib().pbo_wc_w(r, h)
+
+
+def wc_dw(r, h):
+    return lib().pbo_wc_dw(r, h)
+
+
+def wc_density(r, m, h, rho0, c0, gamma=7.0, pmin=0.0):
+    """Config-physics density: -> num (n, 3: sum W, rho, p), rhoinv (n)."""
+    r, m = f64(r), f64(np.broadcast_to(np.asarray(m, dtype=np.float64), (len(r),)))
+    n = r.shape[0]
+    num = np.zeros((n, 3))
+    rhoinv = np.zeros(n)
+    d = C.c_double
+    lib().pbo_wc_density(_p(r, d), _p(m, d), n, h, rho0, c0, gamma, pmin, _p(num, d),
+                         _p(rhoinv, d))
+    return num, rhoinv
+
+
+def wc_accel(r, v, m, rho, p, h, c0, alpha=0.1):
+    """Config-physics pressure + artificial-viscosity acceleration."""
+    r, v, rho, p = map(f64, (r, v, rho, p))
+    m = f64(np.broadcast_to(np.asarray(m, dtype=np.float64), (len(r),)))
+    n = r.shape[0]
+    a = np.zeros((n, 3))
+    d = C.c_double
+    lib().pbo_wc_accel(_p(r, d), _p(v, d), _p(m, d), _p(rho, d), _p(p, d), n, h, c0, alpha,
+                       _p(a, d))
     return a
 
 

@@ -110,3 +110,104 @@ void pbo_lf_step(double *r, double *v, const double *a, const uint8_t *move, uin
     }
   }
 }
+
+/* ---------------------------------------------------------------------------
+ * Config physics of the dam-break workload (weakly compressible SPH).  The
+ * reference's kernels hold no such form; this follows the published method:
+ *   kernel   M4 cubic spline (Monaghan & Lattanzio 1985), 3-D, support 2h:
+ *            W = 1/(pi h^3) * { 1 - 3/2 q^2 + 3/4 q^3      0 <= q < 1
+ *                             { 1/4 (2 - q)^3              1 <= q < 2
+ *                             { 0                          q >= 2,  q = r/h
+ *   density  rho_i = sum_j m_j W_ij, j = i included (summation density)
+ *   EOS      Tait (Monaghan 1994): p = B ((rho/rho0)^gamma - 1),
+ *            B = rho0 c0^2 / gamma, then max(p, pmin)
+ *   momentum a_i = -sum_{j != i} m_j (p_i/rho_i^2 + p_j/rho_j^2 + Pi_ij)
+ *                  grad_i W_ij
+ *   viscosity Pi_ij = -alpha c0 mu_ij / rho_bar_ij when v_ij . r_ij < 0,
+ *            else 0; mu_ij = h v_ij . r_ij / (r_ij^2 + 0.01 h^2),
+ *            rho_bar_ij = (rho_i + rho_j)/2 (Monaghan 1992, beta = 0, the
+ *            mean sound speed taken as c0).
+ * Non-periodic distances, O(N^2) pair search.
+ * ------------------------------------------------------------------------- */
+
+/* W(r) of the M4 spline, written piecewise */
+double pbo_wc_w(double r, double h) {
+  const double q = r / h, s = 1.0 / (M_PI * h * h * h);
+  if (q < 1.0) return s * (1.0 - 1.5 * q * q + 0.75 * q * q * q);
+  if (q < 2.0) {
+    const double t = 2.0 - q;
+    return s * 0.25 * t * t * t;
+  }
+  return 0.0;
+}
+
+/* (dW/dr)/r of the M4 spline: 1/(pi h^5) * (-3 + 9/4 q) for q < 1,
+ * 1/(pi h^5) * (-3/4 (2 - q)^2 / q) for 1 <= q < 2, 0 beyond */
+double pbo_wc_dw(double r, double h) {
+  const double q = r / h, s = 1.0 / (M_PI * h * h * h * h * h);
+  if (q < 1.0) return s * (-3.0 + 2.25 * q);
+  if (q < 2.0) {
+    const double t = 2.0 - q;
+    return s * (-0.75 * t * t / q);
+  }
+  return 0.0;
+}
+
+/* summation density and Tait pressure; num_out 3 per particle
+ * (sum W, rho, p), rhoinv = 1/rho */
+void pbo_wc_density(const double *r, const double *m, uint32_t n, double h, double rho0,
+                    double c0, double gamma, double pmin, double *num_out, double *rhoinv) {
+  const double B = rho0 * c0 * c0 / gamma, sup = 2.0 * h;
+#pragma omp parallel for schedule(static)
+  for (uint32_t i = 0; i < n; ++i) {
+    double s0 = 0.0, s1 = 0.0;
+    for (uint32_t j = 0; j < n; ++j) {
+      const double dx = r[3 * i] - r[3 * j], dy = r[3 * i + 1] - r[3 * j + 1],
+                   dz = r[3 * i + 2] - r[3 * j + 2];
+      const double R = sqrt(dx * dx + dy * dy + dz * dz);
+      if (!(R < sup)) continue;
+      const double W = pbo_wc_w(R, h);
+      s0 += W;
+      s1 += m[j] * W;
+    }
+    double p = B * (pow(s1 / rho0, gamma) - 1.0);
+    if (p < pmin) p = pmin;
+    num_out[3 * i] = s0;
+    num_out[3 * i + 1] = s1;
+    num_out[3 * i + 2] = p;
+    rhoinv[i] = 1.0 / s1;
+  }
+}
+
+/* pressure + artificial-viscosity acceleration (gravity excluded);
+ * rho and p from pbo_wc_density; a overwritten */
+void pbo_wc_accel(const double *r, const double *v, const double *m, const double *rho,
+                  const double *p, uint32_t n, double h, double c0, double alpha, double *a) {
+  const double sup = 2.0 * h, eh2 = 0.01 * h * h;
+#pragma omp parallel for schedule(static)
+  for (uint32_t i = 0; i < n; ++i) {
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    const double Pi = p[i] / (rho[i] * rho[i]);
+    for (uint32_t j = 0; j < n; ++j) {
+      if (j == i) continue;
+      const double dx = r[3 * i] - r[3 * j], dy = r[3 * i + 1] - r[3 * j + 1],
+                   dz = r[3 * i + 2] - r[3 * j + 2];
+      const double R2 = dx * dx + dy * dy + dz * dz, R = sqrt(R2);
+      if (!(R < sup)) continue;
+      const double vr = (v[3 * i] - v[3 * j]) * dx + (v[3 * i + 1] - v[3 * j + 1]) * dy +
+                        (v[3 * i + 2] - v[3 * j + 2]) * dz;
+      double visc = 0.0;
+      if (vr < 0.0) {
+        const double mu = h * vr / (R2 + eh2);
+        visc = -alpha * c0 * mu / (0.5 * (rho[i] + rho[j]));
+      }
+      const double f = -m[j] * (Pi + p[j] / (rho[j] * rho[j]) + visc) * pbo_wc_dw(R, h);
+      ax += f * dx;
+      ay += f * dy;
+      az += f * dz;
+    }
+    a[3 * i] = ax;
+    a[3 * i + 1] = ay;
+    a[3 * i + 2] = az;
+  }
+}

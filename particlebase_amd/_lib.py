@@ -44,7 +44,8 @@ EXPORTS = [
     "pb_bin", "pb_get_bins", "pb_get_neighbor_table", "pb_lj_set_params", "pb_lj_set_cutoff",
     "pb_lj_force", "pb_coord_counts", "pb_lj_run", "pb_lj_last_run_stats",
     "pb_lj_last_run_timing", "pb_lj_autotune", "pb_gr_hist",
-    "pb_sph_set_properties", "pb_sph_density", "pb_sph_acceleration", "pb_sph_inspect_density",
+    "pb_sph_set_properties", "pb_sph_set_wcsph", "pb_sph_density", "pb_sph_acceleration",
+    "pb_sph_inspect_density",
     "pb_dem_set_properties", "pb_rot_set_inertia", "pb_rot_step", "pb_dem_calc_force",
     "pb_dem_get_contacts", "pb_dem_last_violations",
     "pb_gt_step", "pb_gt_xi", "pb_adjust_velocities", "pb_const_temp",
@@ -140,6 +141,7 @@ def _extra_signatures():
     P = C.POINTER
     return {
         "pb_sph_set_properties": (i32, [vp, P(d), P(d), d]),
+        "pb_sph_set_wcsph": (i32, [vp, d, d, d, d, d, d]),
         "pb_sph_density": (i32, [vp]),
         "pb_sph_acceleration": (i32, [vp]),
         "pb_sph_inspect_density": (i32, [vp, P(d)]),
@@ -492,6 +494,12 @@ class System:
         c = np.ascontiguousarray(np.broadcast_to(np.asarray(c, dtype=np.float64), (self.n,)))
         self._c(self.lib.pb_sph_set_properties(self.h, _p(mu0, C.c_double), _p(c, C.c_double),
                                                float(h)))
+
+    def sph_set_wcsph(self, h, rho0, c0, gamma=7.0, alpha=0.1, pmin=0.0):
+        """Config physics (cubic spline, Tait, artificial viscosity) for the
+        following sph_density / sph_acceleration calls."""
+        self._c(self.lib.pb_sph_set_wcsph(self.h, float(h), float(rho0), float(c0), float(gamma),
+                                          float(alpha), float(pmin)))
 
     def sph_density(self):
         self._c(self.lib.pb_sph_density(self.h))

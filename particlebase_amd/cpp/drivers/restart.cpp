@@ -6,8 +6,10 @@
 //   md  : liquid argon under GaussianThermo (testMD2 set-up, 864 atoms)
 //   dem : spheres settling with rotation and tangential history (testDEM5)
 //   sph : a small dam break (testSPH5)
+//   sphc: the same dam break with the config physics (cubic spline, Tait,
+//         artificial viscosity)
 //
-//   restart <md|dem|sph> [k=20] [file=restart.ck]
+//   restart <md|dem|sph|sphc> [k=20] [file=restart.ck]
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -122,15 +124,17 @@ static void dem_steps(cudaParticleDEM &p, uint32_t k) {
 }
 
 // ---------------------------------------------------------------- SPH
+template <bool config>
 static void sph_setup(cudaParticleSPH_NS &p) {
-  const double dx = 0.05, h = 1.7 * dx, c0 = 10.0 * std::sqrt(2.0 * 9.81 * 0.4);
+  const double dx = 0.05, h = (config ? 1.3 : 1.7) * dx, c0 = 10.0 * std::sqrt(2.0 * 9.81 * 0.4);
+  const double mass = (config ? 1000.0 : 1.0) * dx * dx * dx;
   std::vector<ParticleBase> G;
   auto add = [&](double x, double y, double z, bool fixed) {
     ParticleBase q{};
     q.r[0] = x;
     q.r[1] = y;
     q.r[2] = z;
-    q.m = dx * dx * dx;
+    q.m = mass;
     q.isFixed = fixed;
     q.type = fixed ? 1 : 0;
     G.push_back(q);
@@ -152,13 +156,19 @@ static void sph_setup(cudaParticleSPH_NS &p) {
   p.setup((int)G.size());
   p.setCell(cell);
   p.import(G);
-  p.setSPHProperties(mu, c1, (real)h);
-  p.setupCutoffBlock((real)h, false);
+  if (config) {
+    p.setWCSPHProperties(h, 1000.0, c0, 7.0, 0.1, 0.0);
+    p.setupCutoffBlock((real)(2 * h), false);
+  } else {
+    p.setSPHProperties(mu, c1, (real)h);
+    p.setupCutoffBlock((real)h, false);
+  }
   p.calcBlockID();
   p.calcVinit((real)(0.25 * h / c0));
 }
+template <bool config>
 static void sph_steps(cudaParticleSPH_NS &p, uint32_t k) {
-  const double dx = 0.05, h = 1.7 * dx, c0 = 10.0 * std::sqrt(2.0 * 9.81 * 0.4);
+  const double dx = 0.05, h = (config ? 1.3 : 1.7) * dx, c0 = 10.0 * std::sqrt(2.0 * 9.81 * 0.4);
   const real dt = (real)(0.25 * h / c0);
   for (uint32_t j = 0; j < k; ++j) {
     p.calcBlockID();
@@ -213,8 +223,11 @@ int main(int argc, char **argv) {
     return check<cudaParticleDEM>("dem", k, fn, dem_setup, [](cudaParticleDEM &) {}, dem_steps,
                                   {PB_FIELD_R, PB_FIELD_V, PB_FIELD_W, PB_FIELD_L});
   if (mode == "sph")
-    return check<cudaParticleSPH_NS>("sph", k, fn, sph_setup, [](cudaParticleSPH_NS &) {}, sph_steps,
-                                     {PB_FIELD_R, PB_FIELD_V});
-  std::cerr << "mode: md | dem | sph" << std::endl;
+    return check<cudaParticleSPH_NS>("sph", k, fn, sph_setup<false>, [](cudaParticleSPH_NS &) {},
+                                     sph_steps<false>, {PB_FIELD_R, PB_FIELD_V});
+  if (mode == "sphc")
+    return check<cudaParticleSPH_NS>("sphc", k, fn, sph_setup<true>, [](cudaParticleSPH_NS &) {},
+                                     sph_steps<true>, {PB_FIELD_R, PB_FIELD_V});
+  std::cerr << "mode: md | dem | sph | sphc" << std::endl;
   return 2;
 }

@@ -1,6 +1,9 @@
 // cudaParticleSPH_NS.hh -- SPH Navier-Stokes particles (reference
 // cudaParticleSPHBase.hh/.cu + cudaParticleSPH_NS.hh/.cu) over the engine's
 // SPH kernels: Lucy kernel of support h, p = c^2 (rho - 1), viscosity mu.
+// setWCSPHProperties switches the same class to the weakly compressible
+// form (cubic spline of support 2h, Tait, artificial viscosity), which the
+// reference does not have; setSPHProperties switches back.
 // The sortedOutput flags of the reference exist for its multi-GPU exchange;
 // one GPU owns every particle here, so they are accepted and have no effect.
 #pragma once
@@ -14,6 +17,9 @@ protected:
   real h = 0;
   double meanNum = 0.0;
   std::vector<double> mu0s, cs;    // setSPHProperties arguments, kept for restarts
+  // setWCSPHProperties arguments (h, rho0, c0, gamma, alpha, pmin); empty
+  // under the reference physics
+  std::vector<double> wcsph;
 
 public:
   void setup(int n) override {
@@ -36,7 +42,18 @@ public:
     h = _h;
     mu0s = m;
     cs = c;
+    wcsph.clear();
     pbamd::check(pb_sph_set_properties(sys, m.data(), c.data(), _h), "setSPHProperties");
+  }
+
+  // weakly compressible form: M4 cubic spline of support 2h, summation
+  // density, p = max(B ((rho/rho0)^gamma - 1), pmin), B = rho0 c0^2/gamma,
+  // Monaghan artificial viscosity alpha; the cell length must be >= 2h
+  void setWCSPHProperties(double _h, double rho0, double c0, double gamma = 7.0,
+                          double alpha = 0.1, double pmin = 0.0) {
+    h = (real)_h;
+    wcsph = {_h, rho0, c0, gamma, alpha, pmin};
+    pbamd::check(pb_sph_set_wcsph(sys, _h, rho0, c0, gamma, alpha, pmin), "setWCSPHProperties");
   }
 
   // cudaParticleSPHBase / cudaParticleSPH_NS fields (cudaParticleSPHBase.hh:
@@ -52,6 +69,7 @@ public:
     for (size_t i = 0; i < cs.size(); ++i) c2[i] = cs[i] * cs[i];
     a.put("cudaParticleSPH_NS.soundVelocityC2", c2);
     a.put("cudaParticleSPH_NS.soundVelocityC", cs);
+    if (!wcsph.empty()) a.put("cudaParticleSPH_NS.wcsph", wcsph);
   }
   void load(const pbamd::IArchive &a) override {
     loadBase(a);
@@ -59,8 +77,13 @@ public:
     h = a.scalar<real>("cudaParticleSPHBase.h");
     mu0s = a.get<double>("cudaParticleSPH_NS.viscosityMu");
     cs = a.get<double>("cudaParticleSPH_NS.soundVelocityC");
-    if (!mu0s.empty()) {
+    if (a.has("cudaParticleSPH_NS.wcsph")) {
+      const std::vector<double> p = a.get<double>("cudaParticleSPH_NS.wcsph");
+      if (p.size() != 6) throw std::runtime_error("readSerialization: SPH wcsph parameters");
+      setWCSPHProperties(p[0], p[1], p[2], p[3], p[4], p[5]);
+    } else if (!mu0s.empty()) {
       if (mu0s.size() != N || cs.size() != N) throw std::runtime_error("readSerialization: SPH arrays");
+      wcsph.clear();
       pbamd::check(pb_sph_set_properties(sys, mu0s.data(), cs.data(), h), "readSerialization");
     }
   }

@@ -143,10 +143,14 @@ struct pb_sys {
 
   // SPH (cudaParticleSPH_NS), storage order unless noted
   double *sph_mu0 = nullptr, *sph_c2 = nullptr, *sph_rhoinv = nullptr;
-  double4 *sph_num = nullptr;                  // (number density, rho, mu, 0)
+  double4 *sph_num = nullptr;                  // (number density, rho, mu or p, 0)
   double4 *sph_rs = nullptr, *sph_vs = nullptr, *sph_ds = nullptr;   // cell-sorted views
   double sph_h = 0.0, sph_rho0 = 1.0;
   bool sph_set = false, sph_dens = false;
+  // 0: reference physics (Lucy, linear EOS, mu); 1: weakly compressible
+  // config physics (cubic spline, Tait, artificial viscosity)
+  int sph_kind = 0;
+  double sph_c0 = 0.0, sph_gamma = 7.0, sph_alpha = 0.0, sph_pmin = 0.0;
 
   // DEM (cudaParticleDEM, cudaParticleRotation)
   double *m0inv = nullptr;                     // minv / (0.4 r0^2), storage order

@@ -15,6 +15,12 @@
 // buffers, no reductions.  Distances are non-periodic (distance2,
 // kernelinline.hh:19-29); the density sum includes the particle itself, the
 // acceleration sum excludes it; rho0 = 1 (cudaParticleSPH_NS.cu:91).
+//
+// The same calls also run the dam-break workload's config physics, which the
+// reference's kernels do not hold (weakly compressible SPH): the M4 cubic
+// spline of support 2h, summation density, the Tait equation of state
+// clamped at pmin, and Monaghan's artificial viscosity; selected by
+// pb_sph_set_wcsph, with the same gathered views, cell grid and launches.
 #include "pb_internal.h"
 #include <cmath>
 #include <vector>
@@ -29,10 +35,14 @@ struct SphArgs {
   const uint32_t *cstart;    // cell -> first sorted slot (cstart[c+1] ends it)
   const double4 *rs;         // sorted (x, y, z, m)
   const double4 *vs;         // sorted (vx, vy, vz, mu0 or mu)
-  double4 *ds;               // sorted (1/rho, P, mu, 0) from the density pass
+  double4 *ds;               // sorted (1/rho, P, mu, 0) from the density pass;
+                             // (rho, p/rho^2, 0, 0) under the config physics
   SphGrid g;
   uint32_t n;
   double h, w0, w1, rho0;
+  // config physics: support^2, 1/h, 1/(pi h^3), 1/(pi h^5), Tait B, 1/rho0,
+  // gamma, pmin, alpha c0 h, 0.01 h^2
+  double sup2, hinv, ws, wf, B, rho0inv, gamma, pmin, avh, eh2;
 };
 
 // positions (mass in w), velocities, mu0 in cell order
@@ -135,6 +145,67 @@ __global__ void __launch_bounds__(256) k_sph_accel(SphArgs A, double4 *acc) {
   acc[s] = make_double4(ax, ay, az, 0.0);
 }
 
+// Config physics density: inside the support W = ws (t2^3/4 - t1^3) with
+// t1 = max(1 - q, 0), t2 = 2 - q, q = r/h (the M4 spline in B-spline form),
+// rho = sum m_j W (self included), p = max(B ((rho/rho0)^gamma - 1), pmin).
+// num = (sum W, rho, p), ds = (rho, p/rho^2) for the acceleration pass.
+__global__ void __launch_bounds__(256) k_wc_density(SphArgs A, double4 *num, double *rhoinv) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= A.n) return;
+  const uint32_t s = A.perm[k];
+  const double4 pi = A.rs[k];
+  double n0 = 0.0, rho = 0.0;
+  for_neighbors(A, A.cellid[s], [&](uint32_t q) {
+    const double4 pj = A.rs[q];
+    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const double R2 = dx * dx + dy * dy + dz * dz;
+    if (R2 < A.sup2) {
+      const double qq = sqrt(R2) * A.hinv;
+      const double t1 = fmax(1.0 - qq, 0.0), t2 = 2.0 - qq;
+      const double W = 0.25 * (t2 * t2 * t2) - t1 * t1 * t1;
+      n0 += W;
+      rho = fma(pj.w, W, rho);
+    }
+  });
+  n0 *= A.ws;
+  rho *= A.ws;
+  const double p = fmax(A.B * (pow(rho * A.rho0inv, A.gamma) - 1.0), A.pmin);
+  const double ri = 1.0 / rho;
+  num[s] = make_double4(n0, rho, p, 0.0);
+  rhoinv[s] = ri;
+  A.ds[k] = make_double4(rho, p * ri * ri, 0.0, 0.0);
+}
+
+// Config physics acceleration (gravity excluded):
+//   a_i = -sum_{j != i} m_j (p_i/rho_i^2 + p_j/rho_j^2 + Pi_ij) F(r) r_ij,
+// F = (dW/dr)/r = wf (-3 + 9/4 q) for q < 1, wf (-3/4 (2 - q)^2 / q) below 2;
+// Pi_ij = -alpha c0 h (v_ij . r_ij) / ((r^2 + 0.01 h^2) (rho_i + rho_j)/2)
+// for approaching pairs, 0 otherwise.  a is overwritten.
+__global__ void __launch_bounds__(256) k_wc_accel(SphArgs A, double4 *acc) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= A.n) return;
+  const uint32_t s = A.perm[k];
+  const double4 pi = A.rs[k], vi = A.vs[k], di = A.ds[k];
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for_neighbors(A, A.cellid[s], [&](uint32_t q) {
+    const double4 pj = A.rs[q];
+    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const double R2 = dx * dx + dy * dy + dz * dz;
+    if (R2 < A.sup2 && q != k) {
+      const double4 vj = A.vs[q], dj = A.ds[q];
+      const double qq = sqrt(R2) * A.hinv, t2 = 2.0 - qq;
+      const double g = qq < 1.0 ? fma(2.25, qq, -3.0) : -0.75 * t2 * t2 / qq;
+      const double vr = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
+      const double visc = vr < 0.0 ? -A.avh * vr / ((R2 + A.eh2) * (0.5 * (di.x + dj.x))) : 0.0;
+      const double f = -pj.w * (di.y + dj.y + visc) * g;
+      ax = fma(f, dx, ax);
+      ay = fma(f, dy, ay);
+      az = fma(f, dz, az);
+    }
+  });
+  acc[s] = make_double4(ax * A.wf, ay * A.wf, az * A.wf, 0.0);
+}
+
 // inspectDense_x (kernelfuncs.cu:772-801): per-wave partial sums of the
 // number density and the count of movable particles
 __global__ void k_sph_inspect(const double4 *num, const uint8_t *move, const uint32_t *orig,
@@ -158,9 +229,9 @@ static int sph_ready(pb_sys *s) {
   if (!s->sph_set) return pb_fail(s->ctx, PB_ERR_STATE, "SPH properties not set");
   if (!s->ref_binned) return pb_fail(s->ctx, PB_ERR_STATE, "SPH: pb_bin (calcBlockID) first");
   if (s->ref_periodic) return pb_fail(s->ctx, PB_ERR_CELLS, "SPH: cells must be non-periodic");
-  const float hf = (float)s->sph_h;
+  const float hf = (float)(s->sph_kind == 1 ? 2.0 * s->sph_h : s->sph_h);
   if (s->blocklen[0] < hf || s->blocklen[1] < hf || s->blocklen[2] < hf)
-    return pb_fail(s->ctx, PB_ERR_CELLS, "SPH: cell length below the kernel radius h");
+    return pb_fail(s->ctx, PB_ERR_CELLS, "SPH: cell length below the kernel support");
   return PB_OK;
 }
 
@@ -178,7 +249,34 @@ static SphArgs sph_args(pb_sys *s) {
   A.w0 = 105.0 / (16.0 * M_PI * std::pow(s->sph_h, 7));   // cudaParticleSPHBase.cu:121
   A.w1 = -12.0 * A.w0;
   A.rho0 = s->sph_rho0;
+  const double h = s->sph_h;
+  A.sup2 = 4.0 * h * h;
+  A.hinv = 1.0 / h;
+  A.ws = 1.0 / (M_PI * h * h * h);
+  A.wf = A.ws / (h * h);
+  A.B = s->sph_rho0 * s->sph_c0 * s->sph_c0 / s->sph_gamma;
+  A.rho0inv = 1.0 / s->sph_rho0;
+  A.gamma = s->sph_gamma;
+  A.pmin = s->sph_pmin;
+  A.avh = s->sph_alpha * s->sph_c0 * h;
+  A.eh2 = 0.01 * h * h;
   return A;
+}
+
+// density, 1/rho and the cell-order views, shared by both physics
+static int sph_alloc_views(pb_sys *s) {
+  const size_t np = s->npad;
+  int rc = PB_OK;
+  if (!s->sph_rs) {
+    if ((rc = pb_alloc(s->ctx, &s->sph_rhoinv, np)) || (rc = pb_alloc(s->ctx, &s->sph_num, np)) ||
+        (rc = pb_alloc(s->ctx, &s->sph_rs, np)) || (rc = pb_alloc(s->ctx, &s->sph_vs, np)) ||
+        (rc = pb_alloc(s->ctx, &s->sph_ds, np)))
+      return rc;
+  }
+  hipStream_t st = s->ctx->stream;
+  PB_HIP(s->ctx, hipMemsetAsync(s->sph_num, 0, sizeof(double4) * np, st));
+  PB_HIP(s->ctx, hipMemsetAsync(s->sph_rhoinv, 0, 8 * np, st));
+  return PB_OK;
 }
 
 extern "C" int pb_sph_set_properties(pb_sys *s, const double *mu0, const double *c, double h) {
@@ -196,20 +294,38 @@ extern "C" int pb_sph_set_properties(pb_sys *s, const double *mu0, const double 
       hc[k] = c[o[k]] * c[o[k]];                    // c^2, cudaParticleSPH_NS.cu:63-64
     }
   if (!s->sph_mu0) {
-    if ((rc = pb_alloc(s->ctx, &s->sph_mu0, np)) || (rc = pb_alloc(s->ctx, &s->sph_c2, np)) ||
-        (rc = pb_alloc(s->ctx, &s->sph_rhoinv, np)) || (rc = pb_alloc(s->ctx, &s->sph_num, np)) ||
-        (rc = pb_alloc(s->ctx, &s->sph_rs, np)) || (rc = pb_alloc(s->ctx, &s->sph_vs, np)) ||
-        (rc = pb_alloc(s->ctx, &s->sph_ds, np)))
+    if ((rc = pb_alloc(s->ctx, &s->sph_mu0, np)) || (rc = pb_alloc(s->ctx, &s->sph_c2, np)))
       return rc;
   }
+  if ((rc = sph_alloc_views(s))) return rc;
   hipStream_t st = s->ctx->stream;
   PB_HIP(s->ctx, hipMemcpyAsync(s->sph_mu0, hm.data(), 8 * np, hipMemcpyHostToDevice, st));
   PB_HIP(s->ctx, hipMemcpyAsync(s->sph_c2, hc.data(), 8 * np, hipMemcpyHostToDevice, st));
-  PB_HIP(s->ctx, hipMemsetAsync(s->sph_num, 0, sizeof(double4) * np, st));
-  PB_HIP(s->ctx, hipMemsetAsync(s->sph_rhoinv, 0, 8 * np, st));
   PB_HIP(s->ctx, hipStreamSynchronize(st));
   s->sph_h = h;
   s->sph_rho0 = 1.0;
+  s->sph_kind = 0;
+  s->sph_set = true;
+  s->sph_dens = false;
+  return PB_OK;
+}
+
+extern "C" int pb_sph_set_wcsph(pb_sys *s, double h, double rho0, double c0, double gamma,
+                                double alpha, double pmin) {
+  if (!s || !(h > 0) || !(rho0 > 0) || !(c0 > 0) || !(gamma > 0) || !(alpha >= 0) ||
+      std::isnan(pmin) || !std::isfinite(h * rho0 * c0 * gamma * alpha))
+    return PB_ERR_ARG;
+  if (s->nrep != 1) return pb_fail(s->ctx, PB_ERR_ARG, "SPH: single replica only");
+  PB_HIP(s->ctx, hipSetDevice(s->ctx->device));
+  if (int rc = sph_alloc_views(s)) return rc;
+  PB_HIP(s->ctx, hipStreamSynchronize(s->ctx->stream));
+  s->sph_h = h;
+  s->sph_rho0 = rho0;
+  s->sph_c0 = c0;
+  s->sph_gamma = gamma;
+  s->sph_alpha = alpha;
+  s->sph_pmin = pmin;
+  s->sph_kind = 1;
   s->sph_set = true;
   s->sph_dens = false;
   return PB_OK;
@@ -223,10 +339,15 @@ extern "C" int pb_sph_density(pb_sys *s) {
   const uint32_t n = s->n;
   if (n == 0) return PB_OK;
   SphArgs A = sph_args(s);
+  const bool wc = s->sph_kind == 1;
   hipLaunchKernelGGL(k_sph_gather, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->perm, s->r, s->v,
-                     s->sph_mu0, n, s->sph_rs, s->sph_vs);
-  hipLaunchKernelGGL(k_sph_density, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A, s->sph_num,
-                     s->sph_rhoinv);
+                     wc ? (const double *)nullptr : s->sph_mu0, n, s->sph_rs, s->sph_vs);
+  if (wc)
+    hipLaunchKernelGGL(k_wc_density, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A, s->sph_num,
+                       s->sph_rhoinv);
+  else
+    hipLaunchKernelGGL(k_sph_density, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A, s->sph_num,
+                       s->sph_rhoinv);
   PB_LAUNCH_CHECK(s->ctx, "k_sph_density");
   s->sph_dens = true;
   return PB_OK;
@@ -245,8 +366,12 @@ extern "C" int pb_sph_acceleration(pb_sys *s) {
   // density pass are already in ds)
   hipLaunchKernelGGL(k_sph_gather, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->perm, s->r, s->v,
                      (const double *)nullptr, n, s->sph_rs, s->sph_vs);
-  hipLaunchKernelGGL(k_sph_pressure, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A, s->sph_c2);
-  hipLaunchKernelGGL(k_sph_accel, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A, s->a);
+  if (s->sph_kind == 1) {
+    hipLaunchKernelGGL(k_wc_accel, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A, s->a);
+  } else {
+    hipLaunchKernelGGL(k_sph_pressure, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A, s->sph_c2);
+    hipLaunchKernelGGL(k_sph_accel, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A, s->a);
+  }
   PB_LAUNCH_CHECK(s->ctx, "k_sph_accel");
   return PB_OK;
 }

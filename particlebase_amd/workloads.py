@@ -154,6 +154,7 @@ class SPHSystem:
     h: float
     cell: np.ndarray     # (6,) box
     dx: float
+    rho0: float = 1.0    # rest density (config physics: Tait reference density)
 
     @property
     def n(self):
@@ -199,6 +200,35 @@ def sph_dam_break(dx: float = 0.4 / 63, box=(1.6, 0.4, 0.8), column=(0.4, 0.4, 0
     cell = np.array([lo, box[0] - lo, lo, box[1] - lo, lo, box[2] + layers * dx])
     return SPHSystem(r=r, m=m, fixed=fixed, mu0=mu0, c=np.full(n, c0), h=hfac * dx, cell=cell,
                      dx=dx)
+
+
+def m4_lattice_sum(hfac: float) -> float:
+    """sum over the cubic lattice Z^3 of W_M4(|n| dx) dx^3 for h = hfac dx
+    (the M4 cubic spline of support 2h, 3-D normalisation 1/(pi h^3))."""
+    k = int(math.ceil(2 * hfac))
+    g = np.arange(-k, k + 1, dtype=np.float64)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    q = np.sqrt(X * X + Y * Y + Z * Z).ravel() / hfac
+    t1 = np.maximum(1.0 - q, 0.0)
+    t2 = np.maximum(2.0 - q, 0.0)
+    return float((0.25 * t2 ** 3 - t1 ** 3).sum() / (math.pi * hfac ** 3))
+
+
+def sph_dam_break_config(dx: float = 0.4 / 63, box=(1.6, 0.4, 0.8), column=(0.4, 0.4, 0.8),
+                         layers: int = 3, hfac: float = 1.3, g: float = 9.81,
+                         rho0: float = 1000.0, seed: int = 3, jitter: float = 0.0) -> SPHSystem:
+    """The dam break with the config physics (cubic spline of support 2h,
+    Tait, artificial viscosity) in SI units: the same lattice, walls and
+    box as sph_dam_break, h = 1.3 dx, rest density 1000 kg/m^3 and
+    c0 = 10 sqrt(2 g H).  The particle mass is rho0 dx^3 divided by the
+    kernel's lattice sum, so a particle with full support on the initial
+    lattice has density rho0 exactly (p = 0 at rest).  Three wall layers
+    (3 dx) cover the support 2.6 dx."""
+    w = sph_dam_break(dx, box, column, layers, hfac, g, seed, jitter)
+    w.m = np.full(w.n, rho0 * dx ** 3 / m4_lattice_sum(hfac))
+    w.rho0 = rho0
+    w.mu0 = np.zeros(w.n)
+    return w
 
 
 # ------------------------------------------------------------------ DEM

@@ -4,7 +4,7 @@ call sequences through the class layer and the C ABI on the GPU: LJ
 initialAnnealing and the neighbour-list path), liquid argon under the Gaussian
 thermostat (testMD2.cu order), checkpoint / restart of MD, DEM and SPH
 objects (bit-exact continuation), SPH dam break (testSPH5.cu
-order with AdaptiveTime) and DEM settle (testDEM5.cu order with
+order with AdaptiveTime; reference and config physics) and DEM settle (testDEM5.cu order with
 AdaptiveTime).  Each must finish and report a physically sane end state."""
 import os
 import re
@@ -44,6 +44,14 @@ def test_sph_dam():
     assert float(d["mean_num"]) > 0
 
 
+def test_sph_dam_config():
+    # the config physics (cubic spline, Tait, artificial viscosity)
+    d = run("sph_dam", 0.08, 60, "config")
+    assert int(d["steps"]) == 60
+    assert 0.3 < float(d["front_x"]) < 1.7
+    assert float(d["mean_num"]) > 0
+
+
 def test_dem_settle():
     d = run("dem_settle", 3000, 200)
     assert int(d["steps"]) == 200
@@ -74,7 +82,7 @@ def test_md_gas(mode):
     assert 300.0 < float(d["Tmin"]) and float(d["Tmax"]) < 1.2 * 353.0
 
 
-@pytest.mark.parametrize("mode", ["md", "dem", "sph"])
+@pytest.mark.parametrize("mode", ["md", "dem", "sph", "sphc"])
 def test_restart_exact(mode, tmp_path):
     # writeSerialization after k steps, readSerialization into a fresh
     # object, k more steps: the same end state as 2k uninterrupted steps
