@@ -9,10 +9,11 @@
 // particles are gathered into cell order once per call (positions with the
 // mass in w, velocities, per-particle material terms); one kernel computes
 // the number density, rho, 1/rho and the field mu in a single pass over
-// the 27 cells, and one kernel computes the acceleration.  Both read the
-// cell-sorted views (neighbour cells are contiguous ranges, coalesced in
-// HBM) and write straight to the particle's storage slot: no partial
-// buffers, no reductions.  Distances are non-periodic (distance2,
+// the 27 cells, and one kernel computes the acceleration.  Both stage the
+// cell-sorted views through LDS column by column (neighbour cells of a
+// column are one contiguous range) and write straight to the particle's
+// storage slot: no partial buffers, no reductions.  Distances are
+// non-periodic (distance2,
 // kernelinline.hh:19-29); the density sum includes the particle itself, the
 // acceleration sum excludes it; rho0 = 1 (cudaParticleSPH_NS.cu:91).
 //
@@ -45,7 +46,7 @@ struct SphArgs {
   double sup2, hinv, ws, wf, B, rho0inv, gamma, pmin, avh, eh2;
 };
 
-// positions (mass in w), velocities, mu0 in cell order
+// positions (mass in w), velocities, mu0 in cell order (density call)
 __global__ void k_sph_gather(const uint32_t *perm, const double4 *r, const double4 *v,
                              const double *mu0, uint32_t n, double4 *rs, double4 *vs) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -56,154 +57,269 @@ __global__ void k_sph_gather(const uint32_t *perm, const double4 *r, const doubl
   vs[k] = make_double4(u.x, u.y, u.z, mu0 ? mu0[s] : 0.0);
 }
 
-// visit the sorted slots of the 27 cells around cell c (non-periodic)
-template <typename F>
-__device__ __forceinline__ void for_neighbors(const SphArgs &A, uint32_t c, F &&f) {
-  const uint32_t nz = A.g.nb[2], nyz = A.g.nb[1] * nz;
-  const int i0 = (int)(c / nyz), j0 = (int)((c / nz) % A.g.nb[1]), k0 = (int)(c % nz);
-  for (int a = -1; a <= 1; ++a) {
-    const int i2 = i0 + a;
-    if (i2 < 0 || i2 >= (int)A.g.nb[0]) continue;
-    for (int b = -1; b <= 1; ++b) {
-      const int j2 = j0 + b;
-      if (j2 < 0 || j2 >= (int)A.g.nb[1]) continue;
-      // the three z cells of a column are adjacent in the sorted order
-      const int kl = k0 > 0 ? k0 - 1 : 0, kh = k0 + 1 < (int)nz ? k0 + 1 : k0;
-      const uint32_t cb = (uint32_t)i2 * nyz + (uint32_t)j2 * nz;
-      const uint32_t q0 = A.cstart[cb + (uint32_t)kl], q1 = A.cstart[cb + (uint32_t)kh + 1];
-      // four candidates per iteration so their loads are in flight together
-      uint32_t q = q0;
-      for (; q + 4 <= q1; q += 4) {
-        f(q);
-        f(q + 1);
-        f(q + 2);
-        f(q + 3);
+// Acceleration call: positions, velocities and the per-particle terms of the
+// last density call, read from the storage-order outputs, so a pb_bin
+// between the two calls is harmless.  Reference physics: ds = (1/rho,
+// c2/rho (1 - rho0/rho), mu); config physics: ds = (rho, p/rho^2).
+__global__ void k_sph_gather_acc(const uint32_t *perm, const double4 *r, const double4 *v,
+                                 const double4 *num, const double *rhoinv, const double *c2,
+                                 double rho0, int wc, uint32_t n, double4 *rs, double4 *vs,
+                                 double4 *ds) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t s = perm[k];
+  const double4 p = r[s], u = v[s], d = num[s];
+  const double ri = rhoinv[s];
+  rs[k] = make_double4(p.x, p.y, p.z, 1.0 / u.w);
+  vs[k] = make_double4(u.x, u.y, u.z, 0.0);
+  ds[k] = wc ? make_double4(d.y, d.z * ri * ri, 0.0, 0.0)
+             : make_double4(ri, c2[s] * ri * (1.0 - rho0 * ri), d.z, 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// Pair kernels.  One workgroup per segment of zseg cells of a cell column
+// (i, j): the segment's sorted slots are taken 256 at a time, one thread
+// each.  For each of the 9 neighbour columns the candidates of cells
+// z-1 .. z+1 of every thread in the chunk form one contiguous sorted range;
+// it is staged into LDS (coalesced loads, sph_cap<M>() at a time) and each
+// thread scans its own 3-cell window of it:
+//   * a conservative FP32 filter, 32 candidates per step into a bit mask:
+//     |ri - rj|^2 = |ri|^2 + |rj|^2 - 2 ri.rj in coordinates relative to
+//     the segment's first particle, against r2max plus a margin that covers
+//     the FP32 rounding of that form;
+//   * the FP64 pair body per set bit, from the staged FP64 fields, which
+//     applies the exact support test itself (so the margin changes nothing).
+// Pairs are visited in the order of a per-particle 27-cell loop (neighbour
+// columns in (a, b) order, slots ascending).
+constexpr uint32_t SC_T = 256;     // threads per workgroup, one sorted slot each
+
+enum { SPH_LUCY_DENS = 0, SPH_LUCY_ACC = 1, SPH_WC_DENS = 2, SPH_WC_ACC = 3 };
+
+// staged FP64 fields per candidate: x, y, z, m, then
+//   Lucy density: mu0 | Lucy accel: vx, vy, vz, 1/rho, P, mu |
+//   config density: - | config accel: vx, vy, vz, rho, P
+template <int M>
+constexpr int sph_nf() {
+  return M == SPH_LUCY_DENS ? 5 : M == SPH_LUCY_ACC ? 10 : M == SPH_WC_DENS ? 4 : 9;
+}
+// candidates per batch: LDS of a workgroup stays below 40 KB (4 per CU)
+template <int M>
+constexpr uint32_t sph_cap() {
+  return (M == SPH_LUCY_ACC || M == SPH_WC_ACC) ? 384u : 512u;
+}
+
+// 1/sqrt(x), x > 0: the hardware estimate and two Newton steps (about one
+// ulp); and 1/x the same way
+__device__ __forceinline__ double sph_rsqrt(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  y = y * fma(-hx * y, y, 1.5);
+  y = y * fma(-hx * y, y, 1.5);
+  return y;
+}
+__device__ __forceinline__ double sph_rcp(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  y = fma(y, fma(-x, y, 1.0), y);
+  y = fma(y, fma(-x, y, 1.0), y);
+  return y;
+}
+// Tait pressure max(B ((rho/rho0)^gamma - 1), pmin); out of line, once per
+// particle, so pow's registers do not weigh on the pair loop
+__device__ __attribute__((noinline)) double sph_tait(double rho, double rho0inv, double gamma,
+                                                     double B, double pmin) {
+  return fmax(B * (pow(rho * rho0inv, gamma) - 1.0), pmin);
+}
+
+// |r| from r^2 (0 at 0)
+__device__ __forceinline__ double sph_norm(double r2, double *rinv) {
+  const double y = r2 > 0.0 ? sph_rsqrt(r2) : 0.0;
+  *rinv = y;
+  return r2 * y;
+}
+
+template <int M>
+__global__ void __launch_bounds__(SC_T) __attribute__((amdgpu_waves_per_eu(6)))
+k_sph_pairs(SphArgs A, uint32_t zseg, float r2f, double4 *out4, double *out1) {
+  constexpr int NF = sph_nf<M>();
+  constexpr uint32_t CAP = sph_cap<M>();
+  __shared__ double L[NF][CAP];
+  __shared__ float4 Fq[CAP + 32];             // filter view; 32 spare for the last step
+  const uint32_t nz = A.g.nb[2], ny = A.g.nb[1];
+  const uint32_t col = blockIdx.x;
+  const int ci = (int)(col / ny), cj = (int)(col % ny);
+  // this workgroup's cells: z in [zs, ze) of column col
+  const uint32_t zs = blockIdx.y * zseg, ze = zs + zseg < nz ? zs + zseg : nz;
+  const uint32_t s0 = A.cstart[col * nz + zs], s1 = A.cstart[col * nz + ze];
+  if (s0 >= s1) return;
+  const double4 O = A.rs[s0];                 // origin of the filter coordinates
+  const double r2max = (M == SPH_LUCY_DENS || M == SPH_LUCY_ACC) ? A.h * A.h : A.sup2;
+  for (uint32_t c0 = s0; c0 < s1; c0 += SC_T) {
+    const uint32_t c1 = c0 + SC_T < s1 ? c0 + SC_T : s1;
+    const uint32_t k = c0 + threadIdx.x;
+    const bool act = k < c1;
+    uint32_t s = 0, kz = 0;
+    double4 pi = O, vi = make_double4(0.0, 0.0, 0.0, 0.0), di = vi;
+    if (act) {
+      s = A.perm[k];
+      kz = A.cellid[s] % nz;
+      pi = A.rs[k];
+      if constexpr (M == SPH_LUCY_ACC || M == SPH_WC_ACC) {
+        vi = A.vs[k];
+        di = A.ds[k];
       }
-      for (; q < q1; ++q) f(q);
+    }
+    // filter constants: -2 ri and r2 margin - |ri|^2, FP32, relative to O
+    const float xf = (float)(pi.x - O.x), yf = (float)(pi.y - O.y), zf = (float)(pi.z - O.z);
+    const float fx = -2.0f * xf, fy = -2.0f * yf, fz = -2.0f * zf;
+    const float lim = r2f - (xf * xf + yf * yf + zf * zf);
+    // z range of the chunk (slots are sorted by cell, so by z in a column)
+    const uint32_t za = A.cellid[A.perm[c0]] % nz, zb = A.cellid[A.perm[c1 - 1]] % nz;
+    const uint32_t zlo = za > 0 ? za - 1 : 0, zhi = zb + 1 < nz ? zb + 1 : zb;
+    const uint32_t klo = kz > 0 ? kz - 1 : 0, khi = kz + 1 < nz ? kz + 1 : kz;
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0;
+    for (int a = -1; a <= 1; ++a) {
+      const int i2 = ci + a;
+      if (i2 < 0 || i2 >= (int)A.g.nb[0]) continue;
+      for (int b = -1; b <= 1; ++b) {
+        const int j2 = cj + b;
+        if (j2 < 0 || j2 >= (int)ny) continue;
+        const uint32_t cb = ((uint32_t)i2 * ny + (uint32_t)j2) * nz;
+        const uint32_t r0 = A.cstart[cb + zlo], r1 = A.cstart[cb + zhi + 1];
+        uint32_t w0 = 0, w1 = 0;          // this thread's window (empty if inactive)
+        if (act) {
+          w0 = A.cstart[cb + klo];
+          w1 = A.cstart[cb + khi + 1];
+        }
+        for (uint32_t b0 = r0; b0 < r1; b0 += CAP) {
+          const uint32_t b1 = b0 + CAP < r1 ? b0 + CAP : r1;
+          __syncthreads();                // the previous batch is consumed
+          for (uint32_t t = threadIdx.x; t < b1 - b0; t += SC_T) {
+            const double4 pj = A.rs[b0 + t];
+            L[0][t] = pj.x;
+            L[1][t] = pj.y;
+            L[2][t] = pj.z;
+            L[3][t] = pj.w;
+            const float xr = (float)(pj.x - O.x), yr = (float)(pj.y - O.y),
+                        zr = (float)(pj.z - O.z);
+            Fq[t] = make_float4(xr, yr, zr, xr * xr + yr * yr + zr * zr);
+            if constexpr (M == SPH_LUCY_DENS) {
+              L[4][t] = A.vs[b0 + t].w;
+            } else if constexpr (M == SPH_LUCY_ACC || M == SPH_WC_ACC) {
+              const double4 vj = A.vs[b0 + t], dj = A.ds[b0 + t];
+              L[4][t] = vj.x;
+              L[5][t] = vj.y;
+              L[6][t] = vj.z;
+              L[7][t] = dj.x;
+              L[8][t] = dj.y;
+              if constexpr (M == SPH_LUCY_ACC) L[9][t] = dj.z;
+            }
+          }
+          __syncthreads();
+          const uint32_t q0 = w0 > b0 ? w0 : b0, q1 = w1 < b1 ? w1 : b1;
+          for (uint32_t base = q0; base < q1; base += 32) {
+            const uint32_t t0 = base - b0;
+            uint32_t mask = 0;
+            // 32 tests (entries past the window are masked off below; Fq
+            // has 32 spare entries so the reads stay inside the array)
+#pragma unroll 1
+            for (uint32_t g = 0; g < 32; g += 8) {
+#pragma unroll
+              for (uint32_t u = 0; u < 8; ++u) {
+                const float4 f = Fq[t0 + g + u];
+                if (fmaf(fx, f.x, fmaf(fy, f.y, fmaf(fz, f.z, f.w))) < lim) mask |= 1u << (g + u);
+              }
+            }
+            if (q1 - base < 32u) mask &= (1u << (q1 - base)) - 1u;
+            while (mask) {
+              const uint32_t u = (uint32_t)__builtin_ctz(mask);
+              mask &= mask - 1;
+              const uint32_t j = t0 + u, q = base + u;
+              const double xj = L[0][j], yj = L[1][j], zj = L[2][j], mj = L[3][j];
+              if constexpr (M == SPH_LUCY_DENS) {
+                // SPHcalcDensity<SPHKernelLucy> (kerneltemplate.hh:636-656):
+                // W = w0 (h + 3R)(h - R)^3, 0 beyond h (kernelinline.hh:182-188)
+                const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+                double ri;
+                const double R = sph_norm(dx * dx + dy * dy + dz * dz, &ri);
+                const double tt = fmax(A.h - R, 0.0);
+                const double W = A.w0 * (A.h + 3.0 * R) * (tt * tt * tt);
+                const double mW = mj * W;
+                acc0 += W;
+                acc1 += mW;
+                acc2 = fma(L[4][j], mW, acc2);
+              } else if constexpr (M == SPH_LUCY_ACC) {
+                // SPHNavierStokes<SPHKernelLucyDW> (kerneltemplate.hh:666-709)
+                const double dx = xj - pi.x, dy = yj - pi.y, dz = zj - pi.z;
+                double ri;
+                const double R = sph_norm(dx * dx + dy * dy + dz * dz, &ri);
+                const double tt = A.h - R;
+                if (tt > 0.0 && q != k) {                 // no self term, 0 beyond h
+                  const double mw = mj * (A.w1 * tt * tt);  // m_j (dW/dr)/r, kernelinline.hh:209-214
+                  const double p1 = di.y + L[8][j];
+                  const double v1 = (di.z + L[9][j]) * di.x * L[7][j];
+                  acc0 += mw * (dx * p1 - (L[4][j] - vi.x) * v1);
+                  acc1 += mw * (dy * p1 - (L[5][j] - vi.y) * v1);
+                  acc2 += mw * (dz * p1 - (L[6][j] - vi.z) * v1);
+                }
+              } else if constexpr (M == SPH_WC_DENS) {
+                // M4 spline inside the support: W = ws (t2^3/4 - t1^3),
+                // t1 = max(1 - q, 0), t2 = 2 - q, q = r/h
+                const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+                const double R2 = dx * dx + dy * dy + dz * dz;
+                if (R2 < A.sup2) {
+                  double ri;
+                  const double qq = sph_norm(R2, &ri) * A.hinv;
+                  const double t1 = fmax(1.0 - qq, 0.0), t2 = 2.0 - qq;
+                  const double W = 0.25 * (t2 * t2 * t2) - t1 * t1 * t1;
+                  acc0 += W;
+                  acc1 = fma(mj, W, acc1);
+                }
+              } else {
+                // a_i = -sum m_j (P_i + P_j + Pi_ij) F(r) r_ij, F = (dW/dr)/r
+                // = wf (-3 + 9/4 q) for q < 1, wf (-3/4 (2 - q)^2 / q) below 2;
+                // Pi_ij = -alpha c0 h (v_ij . r_ij) / ((r^2 + 0.01 h^2) rho_bar)
+                // for approaching pairs
+                const double dx = pi.x - xj, dy = pi.y - yj, dz = pi.z - zj;
+                const double R2 = dx * dx + dy * dy + dz * dz;
+                if (R2 < A.sup2 && q != k) {
+                  double ri;
+                  const double qq = sph_norm(R2, &ri) * A.hinv, t2 = 2.0 - qq;
+                  const double g = qq < 1.0 ? fma(2.25, qq, -3.0) : -0.75 * t2 * t2 * (A.h * ri);
+                  const double vr =
+                      (vi.x - L[4][j]) * dx + (vi.y - L[5][j]) * dy + (vi.z - L[6][j]) * dz;
+                  const double visc =
+                      vr < 0.0 ? -A.avh * vr * sph_rcp((R2 + A.eh2) * (0.5 * (di.x + L[7][j])))
+                               : 0.0;
+                  const double f = -mj * (di.y + L[8][j] + visc) * g;
+                  acc0 = fma(f, dx, acc0);
+                  acc1 = fma(f, dy, acc1);
+                  acc2 = fma(f, dz, acc2);
+                }
+              }
+            }
+          }
+        }
+      }
+    }
+    if (act) {
+      if constexpr (M == SPH_LUCY_DENS) {
+        // calcDensityPost (cudaParticleSPH_NS.cu:184-207)
+        const double ri = 1.0 / acc1;
+        out4[s] = make_double4(acc0, acc1, acc2 * ri, 0.0);
+        out1[s] = ri;
+      } else if constexpr (M == SPH_WC_DENS) {
+        // Tait: p = max(B ((rho/rho0)^gamma - 1), pmin)
+        const double n0 = acc0 * A.ws, rho = acc1 * A.ws;
+        const double p = sph_tait(rho, A.rho0inv, A.gamma, A.B, A.pmin);
+        out4[s] = make_double4(n0, rho, p, 0.0);
+        out1[s] = 1.0 / rho;
+      } else if constexpr (M == SPH_LUCY_ACC) {
+        out4[s] = make_double4(acc0, acc1, acc2, 0.0);
+      } else {
+        out4[s] = make_double4(acc0 * A.wf, acc1 * A.wf, acc2 * A.wf, 0.0);
+      }
     }
   }
-}
-
-// SPHcalcDensity<SPHKernelLucy> (kerneltemplate.hh:636-656) + calcDensityPost
-// (cudaParticleSPH_NS.cu:184-207), one thread per sorted slot.
-__global__ void __launch_bounds__(256) k_sph_density(SphArgs A, double4 *num, double *rhoinv) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= A.n) return;
-  const uint32_t s = A.perm[k];
-  const double4 pi = A.rs[k];
-  double n0 = 0.0, rho = 0.0, mr = 0.0;
-  for_neighbors(A, A.cellid[s], [&](uint32_t q) {
-    const double4 pj = A.rs[q];
-    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-    const double R = sqrt(dx * dx + dy * dy + dz * dz);
-    const double t = fmax(A.h - R, 0.0);            // W = 0 beyond h, kernelinline.hh:182-188
-    const double W = A.w0 * (A.h + 3.0 * R) * (t * t * t);
-    const double mW = pj.w * W;
-    n0 += W;
-    rho += mW;
-    mr = fma(A.vs[q].w, mW, mr);
-  });
-  const double ri = 1.0 / rho;
-  const double mu = mr * ri;
-  num[s] = make_double4(n0, rho, mu, 0.0);
-  rhoinv[s] = ri;
-  A.ds[k] = make_double4(ri, 0.0, mu, 0.0);   // P filled by the acceleration call
-}
-
-// per-particle pressure term c2/rho (1 - rho0/rho) in cell order
-__global__ void k_sph_pressure(SphArgs A, const double *c2) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= A.n) return;
-  double4 d = A.ds[k];
-  d.y = c2[A.perm[k]] * d.x * (1.0 - A.rho0 * d.x);
-  A.ds[k] = d;
-}
-
-// SPHNavierStokes<SPHKernelLucyDW> (kerneltemplate.hh:666-709), one thread
-// per sorted slot; a is overwritten (the reference clears it first).
-__global__ void __launch_bounds__(256) k_sph_accel(SphArgs A, double4 *acc) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= A.n) return;
-  const uint32_t s = A.perm[k];
-  const double4 pi = A.rs[k], vi = A.vs[k], di = A.ds[k];
-  double ax = 0.0, ay = 0.0, az = 0.0;
-  for_neighbors(A, A.cellid[s], [&](uint32_t q) {
-    const double4 pj = A.rs[q];
-    const double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-    const double R = sqrt(dx * dx + dy * dy + dz * dz);
-    const double t = A.h - R;
-    if (t > 0.0 && q != k) {                      // no self term, 0 beyond h
-      const double4 vj = A.vs[q], dj = A.ds[q];
-      const double mj = pj.w * (A.w1 * t * t);    // m_j (dW/dr)/r, kernelinline.hh:209-214
-      const double p1 = di.y + dj.y;
-      const double v1 = (di.z + dj.z) * di.x * dj.x;
-      ax += mj * (dx * p1 - (vj.x - vi.x) * v1);
-      ay += mj * (dy * p1 - (vj.y - vi.y) * v1);
-      az += mj * (dz * p1 - (vj.z - vi.z) * v1);
-    }
-  });
-  acc[s] = make_double4(ax, ay, az, 0.0);
-}
-
-// Config physics density: inside the support W = ws (t2^3/4 - t1^3) with
-// t1 = max(1 - q, 0), t2 = 2 - q, q = r/h (the M4 spline in B-spline form),
-// rho = sum m_j W (self included), p = max(B ((rho/rho0)^gamma - 1), pmin).
-// num = (sum W, rho, p), ds = (rho, p/rho^2) for the acceleration pass.
-__global__ void __launch_bounds__(256) k_wc_density(SphArgs A, double4 *num, double *rhoinv) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= A.n) return;
-  const uint32_t s = A.perm[k];
-  const double4 pi = A.rs[k];
-  double n0 = 0.0, rho = 0.0;
-  for_neighbors(A, A.cellid[s], [&](uint32_t q) {
-    const double4 pj = A.rs[q];
-    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-    const double R2 = dx * dx + dy * dy + dz * dz;
-    if (R2 < A.sup2) {
-      const double qq = sqrt(R2) * A.hinv;
-      const double t1 = fmax(1.0 - qq, 0.0), t2 = 2.0 - qq;
-      const double W = 0.25 * (t2 * t2 * t2) - t1 * t1 * t1;
-      n0 += W;
-      rho = fma(pj.w, W, rho);
-    }
-  });
-  n0 *= A.ws;
-  rho *= A.ws;
-  const double p = fmax(A.B * (pow(rho * A.rho0inv, A.gamma) - 1.0), A.pmin);
-  const double ri = 1.0 / rho;
-  num[s] = make_double4(n0, rho, p, 0.0);
-  rhoinv[s] = ri;
-  A.ds[k] = make_double4(rho, p * ri * ri, 0.0, 0.0);
-}
-
-// Config physics acceleration (gravity excluded):
-//   a_i = -sum_{j != i} m_j (p_i/rho_i^2 + p_j/rho_j^2 + Pi_ij) F(r) r_ij,
-// F = (dW/dr)/r = wf (-3 + 9/4 q) for q < 1, wf (-3/4 (2 - q)^2 / q) below 2;
-// Pi_ij = -alpha c0 h (v_ij . r_ij) / ((r^2 + 0.01 h^2) (rho_i + rho_j)/2)
-// for approaching pairs, 0 otherwise.  a is overwritten.
-__global__ void __launch_bounds__(256) k_wc_accel(SphArgs A, double4 *acc) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= A.n) return;
-  const uint32_t s = A.perm[k];
-  const double4 pi = A.rs[k], vi = A.vs[k], di = A.ds[k];
-  double ax = 0.0, ay = 0.0, az = 0.0;
-  for_neighbors(A, A.cellid[s], [&](uint32_t q) {
-    const double4 pj = A.rs[q];
-    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-    const double R2 = dx * dx + dy * dy + dz * dz;
-    if (R2 < A.sup2 && q != k) {
-      const double4 vj = A.vs[q], dj = A.ds[q];
-      const double qq = sqrt(R2) * A.hinv, t2 = 2.0 - qq;
-      const double g = qq < 1.0 ? fma(2.25, qq, -3.0) : -0.75 * t2 * t2 / qq;
-      const double vr = (vi.x - vj.x) * dx + (vi.y - vj.y) * dy + (vi.z - vj.z) * dz;
-      const double visc = vr < 0.0 ? -A.avh * vr / ((R2 + A.eh2) * (0.5 * (di.x + dj.x))) : 0.0;
-      const double f = -pj.w * (di.y + dj.y + visc) * g;
-      ax = fma(f, dx, ax);
-      ay = fma(f, dy, ay);
-      az = fma(f, dz, az);
-    }
-  });
-  acc[s] = make_double4(ax * A.wf, ay * A.wf, az * A.wf, 0.0);
 }
 
 // inspectDense_x (kernelfuncs.cu:772-801): per-wave partial sums of the
@@ -261,6 +377,30 @@ static SphArgs sph_args(pb_sys *s) {
   A.avh = s->sph_alpha * s->sph_c0 * h;
   A.eh2 = 0.01 * h * h;
   return A;
+}
+
+// Pair-kernel grid: (columns, z segments).  A segment of zseg cells holds
+// 64 particles at the mean cell occupancy, and two to three times that in
+// the filled part of a partly filled box, so dense columns are spread over
+// several workgroups of one or two 256-slot chunks each.
+static dim3 sph_pair_grid(pb_sys *s, uint32_t *zseg) {
+  const uint32_t nz = s->blocknum[2];
+  const double ncell = (double)s->blocknum[0] * s->blocknum[1] * nz;
+  const double occ = s->n / (ncell > 0 ? ncell : 1.0);
+  uint32_t z = (uint32_t)std::lround(64.0 / (occ > 1e-9 ? occ : 1e-9));
+  z = z < 1 ? 1 : (z > nz ? nz : z);
+  *zseg = z;
+  return dim3(s->blocknum[0] * s->blocknum[1], (nz + z - 1) / z);
+}
+
+// FP32 filter bound: r2max plus 1e-6 Lmax^2, where Lmax bounds the filter
+// coordinates' magnitude (two cells across x and y, the segment and one cell
+// along z); the rounding of the expanded |ri - rj|^2 in FP32 stays below
+// ten units of 2^-24 Lmax^2, so no pair within r2max is filtered out.
+static float sph_filter_r2(pb_sys *s, uint32_t zseg, double r2max) {
+  const double cx = s->blocklen[0], cy = s->blocklen[1], cz = s->blocklen[2];
+  const double l2 = 4.0 * cx * cx + 4.0 * cy * cy + (zseg + 1.0) * (zseg + 1.0) * cz * cz;
+  return (float)(r2max * (1.0 + 1e-6) + 1e-6 * l2);
 }
 
 // density, 1/rho and the cell-order views, shared by both physics
@@ -340,15 +480,17 @@ extern "C" int pb_sph_density(pb_sys *s) {
   if (n == 0) return PB_OK;
   SphArgs A = sph_args(s);
   const bool wc = s->sph_kind == 1;
+  uint32_t zseg = 0;
+  const dim3 grid = sph_pair_grid(s, &zseg);
   hipLaunchKernelGGL(k_sph_gather, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->perm, s->r, s->v,
                      wc ? (const double *)nullptr : s->sph_mu0, n, s->sph_rs, s->sph_vs);
   if (wc)
-    hipLaunchKernelGGL(k_wc_density, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A, s->sph_num,
-                       s->sph_rhoinv);
+    hipLaunchKernelGGL(k_sph_pairs<SPH_WC_DENS>, grid, dim3(SC_T), 0, st, A, zseg,
+                       sph_filter_r2(s, zseg, A.sup2), s->sph_num, s->sph_rhoinv);
   else
-    hipLaunchKernelGGL(k_sph_density, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A, s->sph_num,
-                       s->sph_rhoinv);
-  PB_LAUNCH_CHECK(s->ctx, "k_sph_density");
+    hipLaunchKernelGGL(k_sph_pairs<SPH_LUCY_DENS>, grid, dim3(SC_T), 0, st, A, zseg,
+                       sph_filter_r2(s, zseg, A.h * A.h), s->sph_num, s->sph_rhoinv);
+  PB_LAUNCH_CHECK(s->ctx, "k_sph_pairs (density)");
   s->sph_dens = true;
   return PB_OK;
 }
@@ -362,17 +504,21 @@ extern "C" int pb_sph_acceleration(pb_sys *s) {
   const uint32_t n = s->n;
   if (n == 0) return PB_OK;
   SphArgs A = sph_args(s);
-  // current positions and velocities in cell order (1/rho and mu of the
-  // density pass are already in ds)
-  hipLaunchKernelGGL(k_sph_gather, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->perm, s->r, s->v,
-                     (const double *)nullptr, n, s->sph_rs, s->sph_vs);
-  if (s->sph_kind == 1) {
-    hipLaunchKernelGGL(k_wc_accel, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A, s->a);
-  } else {
-    hipLaunchKernelGGL(k_sph_pressure, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A, s->sph_c2);
-    hipLaunchKernelGGL(k_sph_accel, dim3(pb_blocks(n, 256)), dim3(256), 0, st, A, s->a);
-  }
-  PB_LAUNCH_CHECK(s->ctx, "k_sph_accel");
+  const bool wc = s->sph_kind == 1;
+  uint32_t zseg = 0;
+  const dim3 grid = sph_pair_grid(s, &zseg);
+  // current positions and velocities, and the density call's rho, 1/rho,
+  // mu or p, in the current cell order
+  hipLaunchKernelGGL(k_sph_gather_acc, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->perm, s->r,
+                     s->v, s->sph_num, s->sph_rhoinv, s->sph_c2, s->sph_rho0, wc ? 1 : 0, n,
+                     s->sph_rs, s->sph_vs, s->sph_ds);
+  if (wc)
+    hipLaunchKernelGGL(k_sph_pairs<SPH_WC_ACC>, grid, dim3(SC_T), 0, st, A, zseg,
+                       sph_filter_r2(s, zseg, A.sup2), s->a, (double *)nullptr);
+  else
+    hipLaunchKernelGGL(k_sph_pairs<SPH_LUCY_ACC>, grid, dim3(SC_T), 0, st, A, zseg,
+                       sph_filter_r2(s, zseg, A.h * A.h), s->a, (double *)nullptr);
+  PB_LAUNCH_CHECK(s->ctx, "k_sph_pairs (acceleration)");
   return PB_OK;
 }
 
