@@ -143,7 +143,7 @@ k_sph_pairs(SphArgs A, uint32_t zseg, float r2f, double4 *out4, double *out1) {
   constexpr int NF = sph_nf<M>();
   constexpr uint32_t CAP = sph_cap<M>();
   __shared__ double L[NF][CAP];
-  __shared__ float4 Fq[CAP + 32];             // filter view; 32 spare for the last step
+  __shared__ float4 Fq[CAP + 64];             // filter view; 64 spare for the last step
   const uint32_t nz = A.g.nb[2], ny = A.g.nb[1];
   const uint32_t col = blockIdx.x;
   const int ci = (int)(col / ny), cj = (int)(col % ny);
@@ -216,22 +216,36 @@ k_sph_pairs(SphArgs A, uint32_t zseg, float r2f, double4 *out4, double *out1) {
           }
           __syncthreads();
           const uint32_t q0 = w0 > b0 ? w0 : b0, q1 = w1 < b1 ? w1 : b1;
-          for (uint32_t base = q0; base < q1; base += 32) {
+          for (uint32_t base = q0; base < q1; base += 64) {
             const uint32_t t0 = base - b0;
-            uint32_t mask = 0;
-            // 32 tests (entries past the window are masked off below; Fq
-            // has 32 spare entries so the reads stay inside the array)
+            // 64 tests, two 32-bit halves (entries past the window are
+            // masked off below; Fq has 64 spare entries so the reads stay
+            // inside the array).  A window of three cells usually fits in
+            // one step, so the pair loop below runs the largest per-lane hit
+            // count of the window once.
+            uint32_t lo = 0, hi = 0;
 #pragma unroll 1
             for (uint32_t g = 0; g < 32; g += 8) {
 #pragma unroll
               for (uint32_t u = 0; u < 8; ++u) {
                 const float4 f = Fq[t0 + g + u];
-                if (fmaf(fx, f.x, fmaf(fy, f.y, fmaf(fz, f.z, f.w))) < lim) mask |= 1u << (g + u);
+                if (fmaf(fx, f.x, fmaf(fy, f.y, fmaf(fz, f.z, f.w))) < lim) lo |= 1u << (g + u);
               }
             }
-            if (q1 - base < 32u) mask &= (1u << (q1 - base)) - 1u;
+            if (q1 - base > 32u) {
+#pragma unroll 1
+              for (uint32_t g = 0; g < 32; g += 8) {
+#pragma unroll
+                for (uint32_t u = 0; u < 8; ++u) {
+                  const float4 f = Fq[t0 + 32 + g + u];
+                  if (fmaf(fx, f.x, fmaf(fy, f.y, fmaf(fz, f.z, f.w))) < lim) hi |= 1u << (g + u);
+                }
+              }
+            }
+            uint64_t mask = ((uint64_t)hi << 32) | lo;
+            if (q1 - base < 64u) mask &= (1ull << (q1 - base)) - 1ull;
             while (mask) {
-              const uint32_t u = (uint32_t)__builtin_ctz(mask);
+              const uint32_t u = (uint32_t)__builtin_ctzll(mask);
               mask &= mask - 1;
               const uint32_t j = t0 + u, q = base + u;
               const double xj = L[0][j], yj = L[1][j], zj = L[2][j], mj = L[3][j];

@@ -129,3 +129,19 @@ def test_switch_physics(ctx, orc):
     assert np.allclose(s.get("sph_num"), no, rtol=1e-11, atol=0)
     with pytest.raises(pb.PBError):
         s.sph_set_wcsph(w.h, -1.0, w.c[0])   # rho0 must be positive
+
+
+def test_rebin_between_density_and_acceleration(ctx, orc):
+    say("SPH config physics: a new cell grid and pb_bin between density and acceleration")
+    w = workloads.sph_dam_break_config(0.04, jitter=0.3, seed=12)
+    s = make(ctx, w)
+    rng = np.random.default_rng(6)
+    v = rng.normal(scale=0.5, size=(w.n, 3)) * (1 - w.fixed)[:, None]
+    s.set("v", v)
+    s.bin()
+    s.sph_density()
+    s.setup_cells(2.6 * w.h, periodic=False)   # another grid: another cell order
+    s.bin()
+    s.sph_acceleration()
+    _, _, ao = oracle_state(orc, w, w.r, v)
+    assert np.abs(s.get("a") - ao).max() < 1e-9 * np.abs(ao).max()
