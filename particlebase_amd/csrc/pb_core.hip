@@ -170,7 +170,7 @@ extern "C" void pb_sys_destroy(pb_sys *s) {
   pb_free(s->orig2); pb_free(s->type2); pb_free(s->move2); pb_free(s->a2buf);
   pb_free(s->sph_mu0); pb_free(s->sph_c2); pb_free(s->sph_rhoinv); pb_free(s->sph_num);
   pb_free(s->sph_rs); pb_free(s->sph_vs); pb_free(s->sph_ds); pb_free(s->m0inv);
-  pb_free(s->dem_inv);
+  pb_free(s->dem_inv); pb_free(s->dem_rs); pb_free(s->dem_so);
   for (int b = 0; b < 2; ++b) {
     pb_free(s->cm_row[b]); pb_free(s->cm_col[b]); pb_free(s->cm_val[b]);
   }
@@ -668,21 +668,36 @@ __global__ void k_inspect_partial(const double4 *v, const uint32_t *orig, uint32
   }
 }
 
-__global__ void k_inspect_final(const double *partials, uint32_t nwave, double *out) {
-  // one block: max over waves, ties -> lowest index (fixed order)
-  __shared__ double sm[256], si[256];
+__global__ void __launch_bounds__(1024) k_inspect_final(const double *partials, uint32_t nwave,
+                                                       double *out) {
+  // one block: max over waves, ties -> lowest index (a total order, so the
+  // result does not depend on the visiting order)
+  __shared__ double sm[1024], si[1024];
   double m = -1.0, id = 1e300;
-  for (uint32_t w = threadIdx.x; w < nwave; w += blockDim.x) {
-    const double x = partials[4 * w], i = partials[4 * w + 1];
+  auto take = [&](double x, double i) {
     if (x > m || (x == m && i < id)) {
       m = x;
       id = i;
     }
+  };
+  const uint32_t T = blockDim.x;
+  uint32_t w = threadIdx.x;
+  // four partials per step, all loads issued before the compares
+  for (; w + 3 * T < nwave; w += 4 * T) {
+    double x[4], i[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      x[u] = partials[4 * (size_t)(w + u * T)];
+      i[u] = partials[4 * (size_t)(w + u * T) + 1];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) take(x[u], i[u]);
   }
+  for (; w < nwave; w += T) take(partials[4 * (size_t)w], partials[4 * (size_t)w + 1]);
   sm[threadIdx.x] = m;
   si[threadIdx.x] = id;
   __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+  for (int o = T / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o) {
       const double x = sm[threadIdx.x + o], i = si[threadIdx.x + o];
       if (x > sm[threadIdx.x] || (x == sm[threadIdx.x] && i < si[threadIdx.x])) {
@@ -839,7 +854,7 @@ extern "C" int pb_inspect_velocity(pb_sys *s, double vlim, double lim_u, double 
   const double vl = (double)(uint32_t)(vlim > 0 ? vlim : 0);
   if (vl <= 0) return pb_fail(s->ctx, PB_ERR_ARG, "pb_inspect_velocity: vlim truncates to 0");
   hipLaunchKernelGGL(k_inspect_partial, PB_GRID(s->npad), s->v, s->orig, s->npad, vl, s->partials);
-  hipLaunchKernelGGL(k_inspect_final, dim3(1), dim3(256), 0, s->ctx->stream, s->partials,
+  hipLaunchKernelGGL(k_inspect_final, dim3(1), dim3(1024), 0, s->ctx->stream, s->partials,
                      s->nwave, s->d_out);
   PB_LAUNCH_CHECK(s->ctx, "pb_inspect_velocity");
   double h[2];

@@ -9,11 +9,16 @@
 // pointers, a host read of nnz, a column-index pass, a per-row sort, the
 // history carry-over from the previous matrix and the contact-force pass.
 //
-// Here the same stages run as: count (one thread per particle in cell
-// order, 27 or 125 neighbour cells as contiguous sorted ranges) -> LDS-tiled
-// exclusive scan -> fill + in-row insertion sort (rows are short) -> merge
-// carry of the tangential history (both rows sorted) -> contact force and
-// torque.  Rows are indexed by the original particle index and hold
+// Here the same stages run as: a gather of (x, y, z, radius) and the
+// original index into cell order -> count (one thread per particle in cell
+// order, 27 or 125 neighbour cells as contiguous ranges of that view) ->
+// LDS-tiled exclusive scan -> fill + in-row insertion sort (rows are short)
+// -> merge carry of the tangential history (both rows sorted) -> contact
+// force and torque.  The column arrays are sized from the previous step's
+// count; the stages after the scan stop at that capacity and the host reads
+// nnz and the time-step check together at the end, repeating the stages
+// after the scan with larger arrays in the rare step where nnz outgrew them
+// (one host synchronisation per call).  Rows are indexed by the original particle index and hold
 // original indices in ascending order.  Contacts use the strict
 // |r_ij| < r_i + r_j for both the count and the list (the reference lists
 // with <=, which overruns a row on an exact touch, kernelfuncs.cu:1158).
@@ -25,14 +30,26 @@
 __global__ void k_fill_u32(uint32_t *x, uint32_t n, uint32_t val);
 
 struct DemGrid {
-  const uint32_t *perm, *cellid, *cstart, *orig;
-  const double4 *r;
-  const double *rad;
+  const uint32_t *perm, *cellid, *cstart;
+  const double4 *rs;        // cell order: (x, y, z, radius)
+  const uint32_t *so;       // cell order: original index
   const uint8_t *move;
   uint32_t nb[3];
   int w;            // neighbour half-width: 1 (27 cells) or 2 (125, single-particle cells)
   uint32_t n;
+  uint32_t cap;     // capacity of the column arrays
 };
+
+// positions with the radius in w, and original indices, in cell order
+__global__ void k_dem_gather(const uint32_t *perm, const double4 *r, const double *rad,
+                             const uint32_t *orig, uint32_t n, double4 *rs, uint32_t *so) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint32_t s = perm[k];
+  const double4 p = r[s];
+  rs[k] = make_double4(p.x, p.y, p.z, rad[s]);
+  so[k] = orig[s];
+}
 
 // visit the sorted slots of the (2w+1)^3 cells around cell c (non-periodic:
 // DEM distances never use an image, kernelinline.hh:248)
@@ -49,16 +66,16 @@ __device__ __forceinline__ void dem_cells(const DemGrid &G, uint32_t c, F &&f) {
       if (j2 < 0 || j2 >= (int)G.nb[1]) continue;
       const uint32_t cb = (uint32_t)i2 * nyz + (uint32_t)j2 * nz;
       const uint32_t q0 = G.cstart[cb + (uint32_t)kl], q1 = G.cstart[cb + (uint32_t)kh + 1];
-      for (uint32_t q = q0; q < q1; ++q) f(G.perm[q]);
+      for (uint32_t q = q0; q < q1; ++q) f(q);
     }
   }
 }
 
-__device__ __forceinline__ bool in_contact(const double4 &pi, double ri, const double4 &pj,
-                                           double rj) {
+// (x, y, z, radius) of both particles
+__device__ __forceinline__ bool in_contact(const double4 &pi, const double4 &pj) {
   const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
   const double R2 = dx * dx + dy * dy + dz * dz;
-  const double rc = ri + rj;
+  const double rc = pi.w + pj.w;
   return R2 < rc * rc;
 }
 
@@ -67,16 +84,14 @@ __global__ void k_dem_count(DemGrid G, uint32_t *cnt) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= G.n) return;
   const uint32_t s = G.perm[k];
-  const uint32_t i = G.orig[s];
   uint32_t c = 0;
   if (G.move[s]) {
-    const double4 pi = G.r[s];
-    const double ri = G.rad[s];
-    dem_cells(G, G.cellid[s], [&](uint32_t sj) {
-      if (sj != s && in_contact(pi, ri, G.r[sj], G.rad[sj])) ++c;
+    const double4 pi = G.rs[k];
+    dem_cells(G, G.cellid[s], [&](uint32_t q) {
+      if (q != k && in_contact(pi, G.rs[q])) ++c;
     });
   }
-  cnt[i] = c;
+  cnt[G.so[k]] = c;
 }
 
 // makeJlist_WithBlock2/4_F4 (kernelfuncs.cu:1120-1216) + sortColIdx
@@ -86,13 +101,13 @@ __global__ void k_dem_fill(DemGrid G, const uint32_t *row, uint32_t *col) {
   if (k >= G.n) return;
   const uint32_t s = G.perm[k];
   if (!G.move[s]) return;
-  const uint32_t i = G.orig[s];
-  const uint32_t b = row[i], e = row[i + 1];
+  const uint32_t i = G.so[k];
+  const uint32_t b = row[i], e = min(row[i + 1], G.cap);   // rows past the capacity: redone
+  if (b >= e) return;
   uint32_t p = b;
-  const double4 pi = G.r[s];
-  const double ri = G.rad[s];
-  dem_cells(G, G.cellid[s], [&](uint32_t sj) {
-    if (sj != s && in_contact(pi, ri, G.r[sj], G.rad[sj]) && p < e) col[p++] = G.orig[sj];
+  const double4 pi = G.rs[k];
+  dem_cells(G, G.cellid[s], [&](uint32_t q) {
+    if (q != k && in_contact(pi, G.rs[q]) && p < e) col[p++] = G.so[q];
   });
   for (uint32_t x = b + 1; x < e; ++x) {
     const uint32_t v = col[x];
@@ -109,12 +124,13 @@ __global__ void k_dem_fill(DemGrid G, const uint32_t *row, uint32_t *col) {
 // merge finds the contacts present in both matrices; new contacts start at 0
 __global__ void k_dem_carry(const uint32_t *prow, const uint32_t *pcol, const double *pval,
                             const uint32_t *crow, const uint32_t *ccol, double *cval,
-                            uint32_t n) {
+                            uint32_t n, uint32_t cap) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t p = prow ? prow[i] : 0u;
   const uint32_t pe = prow ? prow[i + 1] : 0u;
-  for (uint32_t q = crow[i]; q < crow[i + 1]; ++q) {
+  const uint32_t qe = min(crow[i + 1], cap);
+  for (uint32_t q = crow[i]; q < qe; ++q) {
     const uint32_t j = ccol[q];
     while (p < pe && pcol[p] < j) ++p;
     cval[q] = (p < pe && pcol[p] == j) ? pval[p] : 0.0;
@@ -131,6 +147,7 @@ struct DemForceArgs {
   double4 *F, *T;
   uint32_t *bad;                         // Cundall-Strack violations
   uint32_t n;
+  uint32_t cap;                          // capacity of col / val
   double E, mu, s2, gamma2, mu_r, dt;
 };
 
@@ -147,7 +164,8 @@ __global__ void __launch_bounds__(256) k_dem_force(DemForceArgs A) {
   double f[3] = {0, 0, 0}, t[3] = {0, 0, 0};
   uint32_t bad = 0;
   const bool wnz = wi.x != 0 && wi.y != 0 && wi.z != 0;
-  for (uint32_t q = A.row[i]; q < A.row[i + 1]; ++q) {
+  const uint32_t qe = min(A.row[i + 1], A.cap);
+  for (uint32_t q = A.row[i]; q < qe; ++q) {
     const uint32_t sj = A.inv[A.col[q]];
     const double4 pj = A.r[sj], vj = A.v[sj], wj = A.w[sj];
     const double rj = A.rad[sj];
@@ -344,19 +362,26 @@ extern "C" int pb_dem_calc_force(pb_sys *s, double dt, int autotune, uint32_t *n
   }
   hipLaunchKernelGGL(k_dem_inv, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->orig, s->npad,
                      s->dem_inv);
+  if (!s->dem_rs) {
+    if ((rc = pb_alloc(s->ctx, &s->dem_rs, s->npad)) || (rc = pb_alloc(s->ctx, &s->dem_so, s->npad)))
+      return rc;
+  }
+  if (n)
+    hipLaunchKernelGGL(k_dem_gather, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->perm, s->r,
+                       s->radius, s->orig, n, s->dem_rs, s->dem_so);
   DemGrid G;
   G.perm = s->perm;
   G.cellid = s->cellid;
   G.cstart = s->cstart;
-  G.orig = s->orig;
-  G.r = s->r;
-  G.rad = s->radius;
+  G.rs = s->dem_rs;
+  G.so = s->dem_so;
   G.move = s->move;
   for (int a = 0; a < 3; ++a) G.nb[a] = s->blocknum[a];
   G.w = s->ref_single ? 2 : 1;
   G.n = n;
   const int b = (s->cm_cur + 1) & 1;         // the other buffer keeps the previous matrix
-  if ((rc = ensure_csr(s, b, 0))) return rc;
+  // capacity from the previous matrix (ensure_csr adds a quarter)
+  if ((rc = ensure_csr(s, b, s->cm_cur >= 0 ? s->cm_nnz[s->cm_cur] : 0))) return rc;
   // per-row counts go to the sort workspace's rank array (free once pb_bin
   // has produced perm / cstart), their exclusive scan to the new row array
   uint32_t *cnt = s->rank;
@@ -366,31 +391,10 @@ extern "C" int pb_dem_calc_force(pb_sys *s, double dt, int autotune, uint32_t *n
     PB_LAUNCH_CHECK(s->ctx, "k_dem_count");
   }
   if ((rc = pb_exclusive_scan(s, cnt, s->cm_row[b], n))) return rc;
-  uint32_t nnz = 0;
-  PB_HIP(s->ctx, hipMemcpyAsync(&nnz, s->cm_row[b] + n, 4, hipMemcpyDeviceToHost, st));
-  PB_HIP(s->ctx, hipStreamSynchronize(st));
-  if ((rc = ensure_csr(s, b, nnz))) return rc;
-  if (n) {
-    hipLaunchKernelGGL(k_dem_fill, dim3(grid), dim3(256), 0, st, G, s->cm_row[b], s->cm_col[b]);
-    const int p = s->cm_cur;
-    hipLaunchKernelGGL(k_dem_carry, dim3(grid), dim3(256), 0, st,
-                       p >= 0 ? s->cm_row[p] : (const uint32_t *)nullptr,
-                       p >= 0 ? s->cm_col[p] : (const uint32_t *)nullptr,
-                       p >= 0 ? s->cm_val[p] : (const double *)nullptr, s->cm_row[b], s->cm_col[b],
-                       s->cm_val[b], n);
-    PB_LAUNCH_CHECK(s->ctx, "k_dem_fill/carry");
-  }
-  s->cm_nnz[b] = nnz;
-  s->cm_cur = b;
-  // contact forces and torques
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s->d_counter + 8, 1u, 0u);
   DemForceArgs A;
   A.perm = s->perm;
   A.orig = s->orig;
   A.inv = s->dem_inv;
-  A.row = s->cm_row[b];
-  A.col = s->cm_col[b];
-  A.val = s->cm_val[b];
   A.r = s->r;
   A.v = s->v;
   A.w = s->w;
@@ -406,13 +410,42 @@ extern "C" int pb_dem_calc_force(pb_sys *s, double dt, int autotune, uint32_t *n
   A.gamma2 = s->dem_gamma2;
   A.mu_r = s->dem_mu_r;
   A.dt = dt;
-  // every real particle's F and T is written (empty rows give zero); padding
-  // slots are never written and stay zero
-  if (n) hipLaunchKernelGGL(k_dem_force, dim3(grid), dim3(256), 0, st, A);
-  PB_LAUNCH_CHECK(s->ctx, "k_dem_force");
-  uint32_t bad = 0;
+  const int p = s->cm_cur;
+  // fill + sort, history carry, contact forces and torques up to the
+  // capacity; then nnz and the time-step check in one read
+  auto contacts_and_force = [&]() -> int {
+    G.cap = A.cap = s->cm_cap[b];
+    A.row = s->cm_row[b];
+    A.col = s->cm_col[b];
+    A.val = s->cm_val[b];
+    if (n) {
+      hipLaunchKernelGGL(k_dem_fill, dim3(grid), dim3(256), 0, st, G, s->cm_row[b], s->cm_col[b]);
+      hipLaunchKernelGGL(k_dem_carry, dim3(grid), dim3(256), 0, st,
+                         p >= 0 ? s->cm_row[p] : (const uint32_t *)nullptr,
+                         p >= 0 ? s->cm_col[p] : (const uint32_t *)nullptr,
+                         p >= 0 ? s->cm_val[p] : (const double *)nullptr, s->cm_row[b],
+                         s->cm_col[b], s->cm_val[b], n, s->cm_cap[b]);
+      PB_LAUNCH_CHECK(s->ctx, "k_dem_fill/carry");
+    }
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s->d_counter + 8, 1u, 0u);
+    // every real particle's F and T is written (empty rows give zero);
+    // padding slots are never written and stay zero
+    if (n) hipLaunchKernelGGL(k_dem_force, dim3(grid), dim3(256), 0, st, A);
+    PB_LAUNCH_CHECK(s->ctx, "k_dem_force");
+    return PB_OK;
+  };
+  if ((rc = contacts_and_force())) return rc;
+  uint32_t nnz = 0, bad = 0;
+  PB_HIP(s->ctx, hipMemcpyAsync(&nnz, s->cm_row[b] + n, 4, hipMemcpyDeviceToHost, st));
   PB_HIP(s->ctx, hipMemcpyAsync(&bad, s->d_counter + 8, 4, hipMemcpyDeviceToHost, st));
   PB_HIP(s->ctx, hipStreamSynchronize(st));
+  if (nnz > s->cm_cap[b]) {                  // the matrix outgrew the arrays: again, larger
+    if ((rc = ensure_csr(s, b, nnz)) || (rc = contacts_and_force())) return rc;
+    PB_HIP(s->ctx, hipMemcpyAsync(&bad, s->d_counter + 8, 4, hipMemcpyDeviceToHost, st));
+    PB_HIP(s->ctx, hipStreamSynchronize(st));
+  }
+  s->cm_nnz[b] = nnz;
+  s->cm_cur = b;
   s->dem_bad = bad;
   if (nnz_out) *nnz_out = nnz;
   if (bad && !autotune)

@@ -162,6 +162,8 @@ struct pb_sys {
   uint32_t cm_cap[2] = {0, 0}, cm_nnz[2] = {0, 0};
   int cm_cur = -1;                             // buffer of the latest CSR, -1 before any
   uint32_t *dem_inv = nullptr;                 // original index -> storage slot
+  double4 *dem_rs = nullptr;                   // cell order: (x, y, z, radius)
+  uint32_t *dem_so = nullptr;                  // cell order: original index
   uint32_t dem_bad = 0;                        // Cundall-Strack violations of the last call
 };
 
