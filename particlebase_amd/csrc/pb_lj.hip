@@ -193,8 +193,8 @@ struct Stage {              // pair kernel: FP64 positions with shifts applied
 };
 #define WCAPB WCAP          // build kernel stage (FP32, 16 B a particle): three workgroups per CU
 struct StageB {             // list build: FP32 window-relative positions, 12 B a
-  float2 xy[WCAPB + 4];      // particle (four workgroups per CU); 4 slack
-  float z[WCAPB + 4];        // entries read by the last group of tests
+  float2 xy[WCAPB + 32];     // particle (four workgroups per CU); 32 slack
+  float z[WCAPB + 32];       // entries read by the last group of 32 tests
   WinTable W;
 };
 
@@ -450,9 +450,11 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
         word = 0;
       }
     };
-    // staged candidates [q0, q1): 32 distance tests into a bit mask, then
-    // the hits appended in order (the append runs once per hit, not once
-    // per candidate)
+    // staged candidates [q0, q1) except li: 32 distance tests into a bit
+    // mask, then the hits appended in order (the append runs once per hit,
+    // not once per candidate).  The last group reads up to 31 entries past
+    // q1 (the stage's slack) and li's own test is kept; their bits are
+    // cleared, so every group runs the same unrolled tests.
     auto scan_staged = [&](uint32_t q0, uint32_t q1, bool mi) {
       const float2 pixy = S.xy[li];
       const float piz = S.z[li];
@@ -468,18 +470,10 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
       };
       for (uint32_t c0 = q0; c0 < q1; c0 += 32) {
         uint32_t m = 0;
-        if (c0 + 32 <= q1) {
 #pragma unroll 4
-          for (uint32_t k = 0; k < 32; ++k) m |= (near(c0 + k) ? 1u : 0u) << k;
-        } else {
-          // tail: groups of 4 tests; a group may read up to 3 entries past
-          // q1 (the stage's slack), whose bits are cleared
-          for (uint32_t j = 0; j < q1 - c0; j += 4) {
-#pragma unroll
-            for (uint32_t k = 0; k < 4; ++k) m |= (near(c0 + j + k) ? 1u : 0u) << (j + k);
-          }
-          m &= (1u << (q1 - c0)) - 1u;
-        }
+        for (uint32_t k = 0; k < 32; ++k) m |= (near(c0 + k) ? 1u : 0u) << k;
+        if (q1 - c0 < 32u) m &= (1u << (q1 - c0)) - 1u;
+        if (li - c0 < 32u) m &= ~(1u << (li - c0));   // li < c0 wraps past 32
         while (m) {
           append(c0 + (uint32_t)__builtin_ctz(m));
           m &= m - 1u;
@@ -502,16 +496,11 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
     };
     // candidates [q0, q1) except the particle itself
     auto scan = [&](uint32_t q0, uint32_t q1, bool mi) {
-      if (li >= q0 && li < q1) {
-        if (staged) {
-          scan_staged(q0, li, mi);
-          scan_staged(li + 1, q1, mi);
-        } else {
-          scan_global(q0, li, mi);
-          scan_global(li + 1, q1, mi);
-        }
-      } else if (staged) {
+      if (staged) {
         scan_staged(q0, q1, mi);
+      } else if (li >= q0 && li < q1) {
+        scan_global(q0, li, mi);
+        scan_global(li + 1, q1, mi);
       } else {
         scan_global(q0, q1, mi);
       }

@@ -524,7 +524,9 @@ extern "C" int pb_sph_acceleration(pb_sys *s) {
   // current positions and velocities, and the density call's rho, 1/rho,
   // mu or p, in the current cell order
   hipLaunchKernelGGL(k_sph_gather_acc, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->perm, s->r,
-                     s->v, s->sph_num, s->sph_rhoinv, s->sph_c2, s->sph_rho0, wc ? 1 : 0, n,
+                     s->v, s->sph_num, s->sph_rhoinv,
+                     s->sph_c2 ? s->sph_c2 : s->sph_rhoinv,   // not read under the config physics
+                     s->sph_rho0, wc ? 1 : 0, n,
                      s->sph_rs, s->sph_vs, s->sph_ds);
   if (wc)
     hipLaunchKernelGGL(k_sph_pairs<SPH_WC_ACC>, grid, dim3(SC_T), 0, st, A, zseg,
