@@ -89,6 +89,23 @@ def drift_per_1e4(e, n, energy_every):
     return float(slope * 1e4)
 
 
+def drift_direct(ctx, precision, skin, rebuild_every, steps=10000, warm=1000):
+    """The direct energy-drift figure: the N=4000 LJ fluid (FCC 10^3 x 4,
+    rho 0.8442, T 0.72, rc 2.5 shifted energy, dt 0.005) melted for `warm`
+    fused steps, then `steps` more with the total energy recorded every 100
+    steps: (E_end - E_start) / N and the fitted slope per 10^4 steps."""
+    w = workloads.lj_config1()
+    s = make_system(ctx, w)
+    s.lj_run(DT, warm, precision=precision, skin=skin, rebuild_every=rebuild_every,
+             energy_every=0)
+    e, _ = s.lj_run(DT, steps, precision=precision, skin=skin, rebuild_every=rebuild_every,
+                    energy_every=100)
+    et = (e[:, 0, 0] + e[:, 0, 1]) / w.n
+    del s
+    return {"delta_e_per_atom": float(et[-1] - et[0]), "fit_per_1e4": drift_per_1e4(e, w.n, 100),
+            "records": int(et.size), "steps": steps, "e_per_atom_start": float(et[0])}
+
+
 def cpu_baseline(w, budget_s=20.0):
     """The oracle's restatement of the reference loop (rebin, 27-cell forces,
     VV, wrap; FP64, OpenMP) on this host, on a bounded sample of steps."""
@@ -311,8 +328,12 @@ def main():
         "plain_fp64": {"value": value64, "ms_per_step": 1e3 * el64 / a.steps, "tiles": tiles64,
                        "kernel_us": kern64 * 1e3,
                        "roofline_frac": ALG_BYTES * w.n / (kern64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                       "energy_drift_per_1e4_steps": drift_per_1e4(e64, w.n, 10)},
+                       "energy_drift_per_1e4_steps": drift_per_1e4(e64, w.n, 10),
+                       "drift_n4000_1e4_steps": drift_direct(ctx, pb.PB_FP64, a.skin,
+                                                             a.rebuild_every)},
         "mixed": {"force_err_max_over_frms": ferr, "energy_drift_per_1e4_steps": drift,
+                  "drift_n4000_1e4_steps": drift_direct(ctx, pb.PB_MIXED, a.skin,
+                                                        a.rebuild_every),
                   "tiles": tiles,
                   "max_nn": stats["max_nn"], "skin_violation": stats["skin_violation"],
                   "rebuild_ms_avg": tm["rebuild_ms"] / max(tm["nrebuild"], 1)},

@@ -291,3 +291,26 @@ def test_force_1M_vs_oracle(ctx, orc):
     for prec, tol in ((pb.PB_FP64, 1e-12), (pb.PB_MIXED, 3 * MIXED_TOL)):
         s.lj_force(prec, energies=False)
         assert np.abs(s.get("F") - Fo).max() / frms < tol, prec
+
+
+@pytest.mark.parametrize("precision", [pb.PB_FP64, pb.PB_MIXED])
+def test_energy_drift_1e4_steps_n4000(ctx, precision):
+    print(f"\n[case] 10^4-step energy drift, N=4000, precision={precision}", flush=True)
+    # the LJ fluid melted from the lattice, then 10^4 fused steps: the total
+    # energy per atom moves by far less than kT (0.72) and the FP32 pair
+    # forces drift no more than the FP64 ones by a wide margin
+    w = workloads.lj_config1()
+    s = pb.System(ctx, [w.n])
+    s.set_cell(w.cell)
+    s.import_particles(w.r, w.m, v=w.v, type=w.type)
+    s.lj_set_params([1.0, 1.0], 1)
+    s.lj_set_cutoff(6.25, shifted=True)
+    s.clear_force(True)
+    s.lj_run(0.005, 1000, precision=precision, skin=0.3, rebuild_every=20, energy_every=0)
+    e, _ = s.lj_run(0.005, 10000, precision=precision, skin=0.3, rebuild_every=20,
+                    energy_every=100)
+    et = (e[:, 0, 0] + e[:, 0, 1]) / w.n
+    assert et.size == 100 and np.all(np.isfinite(et))
+    assert abs(et[-1] - et[0]) < 2e-3      # measured: about 4e-5 (mixed), 2e-5 (FP64)
+    assert np.ptp(et) < 1e-2
+    assert s.lj_run_stats()["skin_violation"] == 0
