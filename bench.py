@@ -106,12 +106,13 @@ def drift_direct(ctx, precision, skin, rebuild_every, steps=10000, warm=1000):
             "records": int(et.size), "steps": steps, "e_per_atom_start": float(et[0])}
 
 
-def cpu_baseline(w, budget_s=20.0):
+def cpu_baseline(w, budget_s=20.0, cores=0):
     """The oracle's restatement of the reference loop (rebin, 27-cell forces,
-    VV, wrap; FP64, OpenMP) on this host, on a bounded sample of steps."""
+    VV, wrap; FP64, OpenMP) on this host, on a bounded sample of steps;
+    `cores` threads (0: the host's share, at most 16)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pb_oracle as orc
-    cores = min(os.cpu_count() or 1, 16)
+    cores = cores or min(os.cpu_count() or 1, 16)
     p = orc.lj_params([1.0, 1.0], 1, RC2, True)
     minv = 1.0 / w.m
     t0 = time.perf_counter()
@@ -377,6 +378,8 @@ def main():
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(w, budget_s=5.0 if a.quick else 20.0)
+        c1 = cpu_baseline(w, budget_s=3.0 if a.quick else 8.0, cores=1)
+        out["cpu_baseline"]["single_core"] = {"value": c1["value"], "sample": c1["sample"]}
         log(f"cpu: {out['cpu_baseline']}")
     print(json.dumps(out), flush=True)
 
