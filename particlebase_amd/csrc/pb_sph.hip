@@ -93,6 +93,12 @@ __global__ void k_sph_gather_acc(const uint32_t *perm, const double4 *r, const d
 // columns in (a, b) order, slots ascending).
 constexpr uint32_t SC_T = 256;     // threads per workgroup, one sorted slot each
 
+// FP32 pre-filter bound r2 (r2max plus the rounding margin) and the squared
+// coordinate range l2 that the margin covers
+struct SphFilter {
+  float r2, l2;
+};
+
 enum { SPH_LUCY_DENS = 0, SPH_LUCY_ACC = 1, SPH_WC_DENS = 2, SPH_WC_ACC = 3 };
 
 // staged FP64 fields per candidate: x, y, z, m, then
@@ -139,7 +145,7 @@ __device__ __forceinline__ double sph_norm(double r2, double *rinv) {
 
 template <int M>
 __global__ void __launch_bounds__(SC_T) __attribute__((amdgpu_waves_per_eu(6)))
-k_sph_pairs(SphArgs A, uint32_t zseg, float r2f, double4 *out4, double *out1) {
+k_sph_pairs(SphArgs A, uint32_t zseg, SphFilter FB, double4 *out4, double *out1) {
   constexpr int NF = sph_nf<M>();
   constexpr uint32_t CAP = sph_cap<M>();
   __shared__ double L[NF][CAP];
@@ -168,10 +174,13 @@ k_sph_pairs(SphArgs A, uint32_t zseg, float r2f, double4 *out4, double *out1) {
         di = A.ds[k];
       }
     }
-    // filter constants: -2 ri and r2 margin - |ri|^2, FP32, relative to O
+    // filter constants: -2 ri and r2 margin - |ri|^2, FP32, relative to O;
+    // a particle farther from O than the margin covers (outside the cell
+    // grid) passes every candidate to the exact test
     const float xf = (float)(pi.x - O.x), yf = (float)(pi.y - O.y), zf = (float)(pi.z - O.z);
     const float fx = -2.0f * xf, fy = -2.0f * yf, fz = -2.0f * zf;
-    const float lim = r2f - (xf * xf + yf * yf + zf * zf);
+    const float bi = xf * xf + yf * yf + zf * zf;
+    const float lim = bi <= FB.l2 ? FB.r2 - bi : INFINITY;
     // z range of the chunk (slots are sorted by cell, so by z in a column)
     const uint32_t za = A.cellid[A.perm[c0]] % nz, zb = A.cellid[A.perm[c1 - 1]] % nz;
     const uint32_t zlo = za > 0 ? za - 1 : 0, zhi = zb + 1 < nz ? zb + 1 : zb;
@@ -201,7 +210,9 @@ k_sph_pairs(SphArgs A, uint32_t zseg, float r2f, double4 *out4, double *out1) {
             L[3][t] = pj.w;
             const float xr = (float)(pj.x - O.x), yr = (float)(pj.y - O.y),
                         zr = (float)(pj.z - O.z);
-            Fq[t] = make_float4(xr, yr, zr, xr * xr + yr * yr + zr * zr);
+            const float bj = xr * xr + yr * yr + zr * zr;
+            // beyond the margin's range: a filter entry every thread passes
+            Fq[t] = bj <= FB.l2 ? make_float4(xr, yr, zr, bj) : make_float4(0.f, 0.f, 0.f, -INFINITY);
             if constexpr (M == SPH_LUCY_DENS) {
               L[4][t] = A.vs[b0 + t].w;
             } else if constexpr (M == SPH_LUCY_ACC || M == SPH_WC_ACC) {
@@ -408,13 +419,18 @@ static dim3 sph_pair_grid(pb_sys *s, uint32_t *zseg) {
 }
 
 // FP32 filter bound: r2max plus 1e-6 Lmax^2, where Lmax bounds the filter
-// coordinates' magnitude (two cells across x and y, the segment and one cell
-// along z); the rounding of the expanded |ri - rj|^2 in FP32 stays below
-// ten units of 2^-24 Lmax^2, so no pair within r2max is filtered out.
-static float sph_filter_r2(pb_sys *s, uint32_t zseg, double r2max) {
+// coordinates' magnitude for particles inside the cell grid (two cells
+// across x and y, the segment and one cell along z); the rounding of the
+// expanded |ri - rj|^2 in FP32 stays below ten units of 2^-24 Lmax^2, so no
+// pair within r2max is filtered out.  Coordinates beyond Lmax (particles
+// outside the grid, binned into its edge cells) bypass the filter.
+static SphFilter sph_filter(pb_sys *s, uint32_t zseg, double r2max) {
   const double cx = s->blocklen[0], cy = s->blocklen[1], cz = s->blocklen[2];
   const double l2 = 4.0 * cx * cx + 4.0 * cy * cy + (zseg + 1.0) * (zseg + 1.0) * cz * cz;
-  return (float)(r2max * (1.0 + 1e-6) + 1e-6 * l2);
+  SphFilter F;
+  F.r2 = (float)(r2max * (1.0 + 1e-6) + 1e-6 * l2);
+  F.l2 = (float)l2;
+  return F;
 }
 
 // density, 1/rho and the cell-order views, shared by both physics
@@ -500,10 +516,10 @@ extern "C" int pb_sph_density(pb_sys *s) {
                      wc ? (const double *)nullptr : s->sph_mu0, n, s->sph_rs, s->sph_vs);
   if (wc)
     hipLaunchKernelGGL(k_sph_pairs<SPH_WC_DENS>, grid, dim3(SC_T), 0, st, A, zseg,
-                       sph_filter_r2(s, zseg, A.sup2), s->sph_num, s->sph_rhoinv);
+                       sph_filter(s, zseg, A.sup2), s->sph_num, s->sph_rhoinv);
   else
     hipLaunchKernelGGL(k_sph_pairs<SPH_LUCY_DENS>, grid, dim3(SC_T), 0, st, A, zseg,
-                       sph_filter_r2(s, zseg, A.h * A.h), s->sph_num, s->sph_rhoinv);
+                       sph_filter(s, zseg, A.h * A.h), s->sph_num, s->sph_rhoinv);
   PB_LAUNCH_CHECK(s->ctx, "k_sph_pairs (density)");
   s->sph_dens = true;
   return PB_OK;
@@ -530,10 +546,10 @@ extern "C" int pb_sph_acceleration(pb_sys *s) {
                      s->sph_rs, s->sph_vs, s->sph_ds);
   if (wc)
     hipLaunchKernelGGL(k_sph_pairs<SPH_WC_ACC>, grid, dim3(SC_T), 0, st, A, zseg,
-                       sph_filter_r2(s, zseg, A.sup2), s->a, (double *)nullptr);
+                       sph_filter(s, zseg, A.sup2), s->a, (double *)nullptr);
   else
     hipLaunchKernelGGL(k_sph_pairs<SPH_LUCY_ACC>, grid, dim3(SC_T), 0, st, A, zseg,
-                       sph_filter_r2(s, zseg, A.h * A.h), s->a, (double *)nullptr);
+                       sph_filter(s, zseg, A.h * A.h), s->a, (double *)nullptr);
   PB_LAUNCH_CHECK(s->ctx, "k_sph_pairs (acceleration)");
   return PB_OK;
 }

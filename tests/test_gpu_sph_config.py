@@ -145,3 +145,39 @@ def test_rebin_between_density_and_acceleration(ctx, orc):
     s.sph_acceleration()
     _, _, ao = oracle_state(orc, w, w.r, v)
     assert np.abs(s.get("a") - ao).max() < 1e-9 * np.abs(ao).max()
+
+
+@pytest.mark.parametrize("physics", ["reference", "config"])
+def test_particles_outside_the_grid(ctx, orc, physics):
+    say(f"SPH {physics} physics with particles just outside the cell grid (edge cells)")
+    if physics == "config":
+        w = workloads.sph_dam_break_config(0.04, jitter=0.2, seed=21)
+        sup = 2 * w.h
+    else:
+        w = workloads.sph_dam_break(0.04, jitter=0.2, seed=21)
+        sup = w.h
+    r = w.r.copy()
+    fluid = np.nonzero(w.fixed == 0)[0]
+    top = fluid[np.argsort(r[fluid, 2])[-40:]]
+    r[top, 2] = w.cell[5] + 0.3 * sup * np.linspace(0.1, 1.0, top.size)   # above the grid
+    s = pb.System(ctx, [w.n])
+    s.set_cell(w.cell)
+    s.import_particles(r, w.m, is_fixed=w.fixed)
+    if physics == "config":
+        s.sph_set_wcsph(w.h, w.rho0, w.c[0], 7.0, ALPHA, 0.0)
+    else:
+        s.sph_set_properties(w.mu0, w.c, w.h)
+    s.setup_cells(sup, periodic=False)
+    s.bin()
+    s.sph_density()
+    num = s.get("sph_num")
+    if physics == "config":
+        no, _ = orc.wc_density(r, w.m, w.h, w.rho0, w.c[0], 7.0, 0.0)
+        assert np.allclose(num[:, :2], no[:, :2], rtol=1e-11, atol=0)
+    else:
+        no, _ = orc.sph_density(r, w.m, w.mu0, w.h)
+        assert np.allclose(num, no, rtol=1e-11, atol=0)
+    # most moved particles have neighbours besides themselves (pairs across
+    # the grid's top face and among the moved particles were found)
+    w_self = 1 / (np.pi * w.h ** 3) if physics == "config" else 105 / (16 * np.pi * w.h ** 3)
+    assert (no[top, 0] > 1.05 * w_self).sum() > top.size // 2
