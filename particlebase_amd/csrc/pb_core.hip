@@ -170,7 +170,7 @@ extern "C" void pb_sys_destroy(pb_sys *s) {
   pb_free(s->orig2); pb_free(s->type2); pb_free(s->move2); pb_free(s->a2buf);
   pb_free(s->sph_mu0); pb_free(s->sph_c2); pb_free(s->sph_rhoinv); pb_free(s->sph_num);
   pb_free(s->sph_rs); pb_free(s->sph_vs); pb_free(s->sph_ds); pb_free(s->m0inv);
-  pb_free(s->dem_inv); pb_free(s->dem_rs); pb_free(s->dem_so);
+  pb_free(s->dem_inv); pb_free(s->dem_rs); pb_free(s->dem_so); pb_free(s->dem_rowbuf);
   for (int b = 0; b < 2; ++b) {
     pb_free(s->cm_row[b]); pb_free(s->cm_col[b]); pb_free(s->cm_val[b]);
   }

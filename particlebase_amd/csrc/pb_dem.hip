@@ -38,7 +38,12 @@ struct DemGrid {
   int w;            // neighbour half-width: 1 (27 cells) or 2 (125, single-particle cells)
   uint32_t n;
   uint32_t cap;     // capacity of the column arrays
+  uint32_t *rowbuf; // DEM_ROWBUF contacts per particle, by original index
 };
+
+// contacts a particle's row keeps from the counting scan; longer rows (more
+// touching neighbours than any sphere packing gives) are scanned again
+#define DEM_ROWBUF 16u
 
 // positions with the radius in w, and original indices, in cell order
 __global__ void k_dem_gather(const uint32_t *perm, const double4 *r, const double *rad,
@@ -80,22 +85,31 @@ __device__ __forceinline__ bool in_contact(const double4 &pi, const double4 &pj)
 }
 
 // calcCoord_F4 (kernelinline.hh:371-382): contacts of movable particles
+// The same scan keeps the first DEM_ROWBUF contacts (original indices, scan
+// order) in the row buffer for the fill.
 __global__ void k_dem_count(DemGrid G, uint32_t *cnt) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= G.n) return;
   const uint32_t s = G.perm[k];
+  const uint32_t i = G.so[k];
   uint32_t c = 0;
   if (G.move[s]) {
     const double4 pi = G.rs[k];
+    uint32_t *buf = G.rowbuf + (size_t)i * DEM_ROWBUF;
     dem_cells(G, G.cellid[s], [&](uint32_t q) {
-      if (q != k && in_contact(pi, G.rs[q])) ++c;
+      if (q != k && in_contact(pi, G.rs[q])) {
+        if (c < DEM_ROWBUF) buf[c] = G.so[q];
+        ++c;
+      }
     });
   }
-  cnt[G.so[k]] = c;
+  cnt[i] = c;
 }
 
 // makeJlist_WithBlock2/4_F4 (kernelfuncs.cu:1120-1216) + sortColIdx
-// (:1262-1290): the thread that owns a row fills it and sorts it.
+// (:1262-1290): the thread that owns a row fills it (from the counting
+// scan's row buffer, or by scanning again when the row is longer) and
+// sorts it.
 __global__ void k_dem_fill(DemGrid G, const uint32_t *row, uint32_t *col) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= G.n) return;
@@ -105,10 +119,15 @@ __global__ void k_dem_fill(DemGrid G, const uint32_t *row, uint32_t *col) {
   const uint32_t b = row[i], e = min(row[i + 1], G.cap);   // rows past the capacity: redone
   if (b >= e) return;
   uint32_t p = b;
-  const double4 pi = G.rs[k];
-  dem_cells(G, G.cellid[s], [&](uint32_t q) {
-    if (q != k && in_contact(pi, G.rs[q]) && p < e) col[p++] = G.so[q];
-  });
+  if (row[i + 1] - b <= DEM_ROWBUF) {     // the counting scan kept the whole row
+    const uint32_t *buf = G.rowbuf + (size_t)i * DEM_ROWBUF;
+    for (; p < e; ++p) col[p] = buf[p - b];
+  } else {
+    const double4 pi = G.rs[k];
+    dem_cells(G, G.cellid[s], [&](uint32_t q) {
+      if (q != k && in_contact(pi, G.rs[q]) && p < e) col[p++] = G.so[q];
+    });
+  }
   for (uint32_t x = b + 1; x < e; ++x) {
     const uint32_t v = col[x];
     uint32_t y = x;
@@ -363,7 +382,8 @@ extern "C" int pb_dem_calc_force(pb_sys *s, double dt, int autotune, uint32_t *n
   hipLaunchKernelGGL(k_dem_inv, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->orig, s->npad,
                      s->dem_inv);
   if (!s->dem_rs) {
-    if ((rc = pb_alloc(s->ctx, &s->dem_rs, s->npad)) || (rc = pb_alloc(s->ctx, &s->dem_so, s->npad)))
+    if ((rc = pb_alloc(s->ctx, &s->dem_rs, s->npad)) || (rc = pb_alloc(s->ctx, &s->dem_so, s->npad)) ||
+        (rc = pb_alloc(s->ctx, &s->dem_rowbuf, (size_t)s->npad * DEM_ROWBUF)))
       return rc;
   }
   if (n)
@@ -375,6 +395,7 @@ extern "C" int pb_dem_calc_force(pb_sys *s, double dt, int autotune, uint32_t *n
   G.cstart = s->cstart;
   G.rs = s->dem_rs;
   G.so = s->dem_so;
+  G.rowbuf = s->dem_rowbuf;
   G.move = s->move;
   for (int a = 0; a < 3; ++a) G.nb[a] = s->blocknum[a];
   G.w = s->ref_single ? 2 : 1;

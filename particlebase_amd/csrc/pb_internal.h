@@ -164,6 +164,7 @@ struct pb_sys {
   uint32_t *dem_inv = nullptr;                 // original index -> storage slot
   double4 *dem_rs = nullptr;                   // cell order: (x, y, z, radius)
   uint32_t *dem_so = nullptr;                  // cell order: original index
+  uint32_t *dem_rowbuf = nullptr;              // contacts kept by the counting scan
   uint32_t dem_bad = 0;                        // Cundall-Strack violations of the last call
 };
 

@@ -180,3 +180,17 @@ def test_cundall_strack_error(ctx):
     assert e.value.code == pb._lib.PB_ERR_DT
     s.dem_calc_force(10.0, autotune=True)
     assert s.dem_violations() > 0
+
+
+def test_long_rows(ctx, orc):
+    say("DEM contacts with rows longer than the count kernel's row buffer (16), N=600")
+    r, rad, fixed, m, v, w, cell = packing(600, 9, box=4.0)
+    s = make(ctx, r, rad, fixed, m, v, cell, 2 * rad.max())
+    s.bin()
+    nnz = s.dem_calc_force(1e-9, autotune=True)
+    row, col, _ = s.dem_contacts()
+    orow, ocol = orc.dem_contacts(r, rad, 1 - fixed)
+    lens = np.diff(orow)
+    assert (lens > 16).sum() > 100 and (lens[lens > 0] <= 16).sum() > 0
+    assert nnz == len(ocol)
+    assert np.array_equal(row, orow) and np.array_equal(col, ocol)
