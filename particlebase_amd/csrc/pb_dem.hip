@@ -157,7 +157,7 @@ __global__ void k_dem_carry(const uint32_t *prow, const uint32_t *pcol, const do
 }
 
 struct DemForceArgs {
-  const uint32_t *perm, *orig, *inv;   // sorted -> storage, storage -> original, original -> storage
+  const uint32_t *orig, *inv;          // storage -> original, original -> storage
   const uint32_t *row, *col;
   double *val;
   const double4 *r, *v, *w;
@@ -165,7 +165,7 @@ struct DemForceArgs {
   const uint8_t *move;
   double4 *F, *T;
   uint32_t *bad;                         // Cundall-Strack violations
-  uint32_t n;
+  uint32_t n, npad;
   uint32_t cap;                          // capacity of col / val
   double E, mu, s2, gamma2, mu_r, dt;
 };
@@ -173,11 +173,13 @@ struct DemForceArgs {
 // DEMContact_F4 (kernelinline.hh:235-363) over the CSR rows
 // (calcF_IJpairWithList_F4, kerneltemplate.hh:135-168), FP64.  F and T are
 // overwritten (the reference clears both first, cudaParticleDEM.cu:168-169).
+// One thread per storage slot, so the particle's own state and its F and T
+// are coalesced; padding slots are skipped (they stay zero).
 __global__ void __launch_bounds__(256) k_dem_force(DemForceArgs A) {
-  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= A.n) return;
-  const uint32_t s = A.perm[k];
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= A.npad) return;
   const uint32_t i = A.orig[s];
+  if (i == PB_NONE) return;
   const double4 pi = A.r[s], vi = A.v[s], wi = A.w[s];
   const double ri = A.rad[s];
   double f[3] = {0, 0, 0}, t[3] = {0, 0, 0};
@@ -413,7 +415,6 @@ extern "C" int pb_dem_calc_force(pb_sys *s, double dt, int autotune, uint32_t *n
   }
   if ((rc = pb_exclusive_scan(s, cnt, s->cm_row[b], n))) return rc;
   DemForceArgs A;
-  A.perm = s->perm;
   A.orig = s->orig;
   A.inv = s->dem_inv;
   A.r = s->r;
@@ -425,6 +426,7 @@ extern "C" int pb_dem_calc_force(pb_sys *s, double dt, int autotune, uint32_t *n
   A.T = s->T;
   A.bad = s->d_counter + 8;
   A.n = n;
+  A.npad = s->npad;
   A.E = s->dem_E;
   A.mu = s->dem_mu;
   A.s2 = s->dem_s2;
@@ -451,7 +453,7 @@ extern "C" int pb_dem_calc_force(pb_sys *s, double dt, int autotune, uint32_t *n
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s->d_counter + 8, 1u, 0u);
     // every real particle's F and T is written (empty rows give zero);
     // padding slots are never written and stay zero
-    if (n) hipLaunchKernelGGL(k_dem_force, dim3(grid), dim3(256), 0, st, A);
+    if (n) hipLaunchKernelGGL(k_dem_force, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, A);
     PB_LAUNCH_CHECK(s->ctx, "k_dem_force");
     return PB_OK;
   };
