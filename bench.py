@@ -245,6 +245,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-sph", action="store_true")
     ap.add_argument("--no-dem", action="store_true")
+    ap.add_argument("--no-drift", action="store_true",
+                    help="skip the 10^4-step N=4000 energy-drift runs")
     ap.add_argument("--sph-steps", type=int, default=50)
     ap.add_argument("--dem-steps", type=int, default=50)
     ap.add_argument("--quick", action="store_true", help="small sizes (debug)")
@@ -330,11 +332,11 @@ def main():
                        "kernel_us": kern64 * 1e3,
                        "roofline_frac": ALG_BYTES * w.n / (kern64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
                        "energy_drift_per_1e4_steps": drift_per_1e4(e64, w.n, 10),
-                       "drift_n4000_1e4_steps": drift_direct(ctx, pb.PB_FP64, a.skin,
-                                                             a.rebuild_every)},
+                       "drift_n4000_1e4_steps": None if a.no_drift else
+                       drift_direct(ctx, pb.PB_FP64, a.skin, a.rebuild_every)},
         "mixed": {"force_err_max_over_frms": ferr, "energy_drift_per_1e4_steps": drift,
-                  "drift_n4000_1e4_steps": drift_direct(ctx, pb.PB_MIXED, a.skin,
-                                                        a.rebuild_every),
+                  "drift_n4000_1e4_steps": None if a.no_drift else
+                  drift_direct(ctx, pb.PB_MIXED, a.skin, a.rebuild_every),
                   "tiles": tiles,
                   "max_nn": stats["max_nn"], "skin_violation": stats["skin_violation"],
                   "rebuild_ms_avg": tm["rebuild_ms"] / max(tm["nrebuild"], 1)},
