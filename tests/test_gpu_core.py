@@ -176,3 +176,35 @@ def test_inspect_velocity(ctx, rng):
     v[7] = [3.0, 0, 0]
     s.set("v", v)
     assert s.inspect_velocity(2.7, 0.9, 0.5)[0] == 1
+
+
+def test_argument_and_state_errors(ctx):
+    say("argument and call-order errors of the core calls (codes and no side effects)")
+    with pytest.raises(pb.PBError) as e:
+        pb.System(ctx, [0])                          # empty replica
+    assert e.value.code == pb.PB_ERR_ARG
+    w = workloads.lj_fcc(4, 0.8442, 0.72, seed=3)     # N = 256, L ~ 6.7
+    s = pb.System(ctx, [w.n])
+    with pytest.raises(pb.PBError) as e:
+        s.setup_cells(2.5)                           # no cell yet
+    assert e.value.code == pb.PB_ERR_STATE
+    s.set_cell(w.cell)
+    s.import_particles(w.r, w.m, v=w.v, type=w.type)
+    with pytest.raises(pb.PBError) as e:
+        s.bin()                                      # no cell grid yet
+    assert e.value.code == pb.PB_ERR_STATE
+    with pytest.raises(pb.PBError) as e:
+        s.setup_cells(-1.0)
+    assert e.value.code == pb.PB_ERR_ARG
+    with pytest.raises(pb.PBError) as e:
+        s.setup_cells(3.0)                           # floor(6.7 / 3) = 2 cells: fewer than 3
+    assert e.value.code == pb.PB_ERR_CELLS
+    with pytest.raises(pb.PBError) as e:
+        s.lj_force(pb.PB_FP64)                       # no LJ parameters
+    assert e.value.code == pb.PB_ERR_STATE
+    # the failed calls left the state alone: a valid sequence still works
+    s.setup_cells(2.2)
+    s.bin()
+    pid, bid, bindex = s.get_bins()
+    assert np.array_equal(np.sort(pid), np.arange(w.n))
+    assert np.allclose(s.get("r"), w.r % w.L, rtol=0, atol=1e-12)
