@@ -32,6 +32,10 @@ import particlebase_amd as pb  # noqa: E402
 from particlebase_amd import workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# VALU issue rate: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane-instructions/s
+# (FP32 FMA = one instruction, no packed FP32)
+VALU_LANE_OPS = 256 * 64 * 2.4e9
+GR_OPS_PER_PAIR = 15.0
 ALG_BYTES = 98.0          # FP64 r, v read + written (96 B) + u16 type read
 RC2 = 6.25
 DT = 0.005
@@ -351,8 +355,21 @@ def main():
         tiles_b = choose_tiles(sb, a, pb.PB_MIXED)
         elb, eb = timed_run(ctx, sb, a.batch_steps, pb.PB_MIXED, a.skin, a.rebuild_every, 0)
         vb = ntot * a.batch_steps / elb
-        del sb
         log(f"batched: {len(reps.systems)} replicas, {ntot} atoms, {vb:.4g} atom-steps/s")
+        # g(r) of every replica on the final snapshot: 200 bins to L_k/2
+        rsteps = np.array([np.float32(x.L / 400.0) for x in reps.systems])
+        sb.gr_hist(rsteps, 200)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            hist = sb.gr_hist(rsteps, 200)
+        elg = (time.perf_counter() - t0) / 3
+        pairs = int(sum(int(n) * (int(n) - 1) // 2 for n in reps.counts))
+        assert int(hist.sum()) == pairs
+        gr = {"pairs": pairs, "ms": 1e3 * elg, "pairs_per_s": pairs / elg,
+              "valu_frac": pairs / elg * GR_OPS_PER_PAIR / VALU_LANE_OPS,
+              "ops_per_pair": GR_OPS_PER_PAIR, "bins": 200}
+        log(f"g(r): {pairs} pairs in {1e3 * elg:.2f} ms, {pairs / elg:.4g} pairs/s")
+        del sb
         plain_systems = [make_system(ctx, x) for x in reps.systems]
         for ps in plain_systems:
             ps.lj_run(DT, 2, precision=pb.PB_FP64, skin=a.skin, rebuild_every=a.rebuild_every,
@@ -369,7 +386,7 @@ def main():
         log(f"plain sequential FP64: {vp:.4g} atom-steps/s")
         out["batched"] = {"replicas": len(reps.systems), "atoms": ntot, "value": vb, "tiles": tiles_b,
                           "plain_sequential_fp64": vp, "speedup": vb / vp,
-                          "steps": a.batch_steps, "plain_steps": a.plain_steps}
+                          "steps": a.batch_steps, "plain_steps": a.plain_steps, "gr": gr}
 
     if not a.no_sph:
         out["sph"] = bench_sph(ctx, a)

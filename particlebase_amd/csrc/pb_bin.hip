@@ -312,7 +312,10 @@ extern "C" int pb_coord_counts(pb_sys *s, float rmax2t, uint32_t *counts) {
 // 440-448).  Work = triangular tile pairs of every replica, found by a
 // binary search over the per-replica tile-pair prefix.  Each workgroup bins
 // into an LDS histogram and flushes it with 64-bit atomics: integer sums, so
-// the result is independent of the order.
+// the result is independent of the order.  Bin 0 (the pairs the reference
+// drops: r < dr/2 and r beyond the last bin, about half of all pairs in a
+// periodic box) is counted in registers and added once per tile pair, so
+// it is not a serialising LDS atomic hot spot.
 #define GR_T 256
 struct GrRep {
   uint32_t atom0, natoms, ntile;
@@ -343,6 +346,7 @@ __global__ void __launch_bounds__(GR_T) k_gr(const double4 *r, const GrRep *reps
       --row;
     }
     const uint32_t tjx = ti + (uint32_t)t;
+    uint32_t zero = 0;                 // this thread's bin-0 pairs of the tile pair
     __syncthreads();
     const uint32_t j = tjx * GR_T + threadIdx.x;
     if (j < R.natoms) {
@@ -362,8 +366,14 @@ __global__ void __launch_bounds__(GR_T) k_gr(const double4 *r, const GrRep *reps
         const float r0 = sqrtf(R2) * R.rstepinv;
         const double tt = (double)r0 + 0.5;
         const uint32_t bin = (tt < (double)rnum) ? (uint32_t)tt : 0u;
-        atomicAdd(&h[bin + lo * 0], 1u);
+        if (bin == 0u) ++zero;
+        else atomicAdd(&h[bin], 1u);
       }
+    }
+    {
+      const uint32_t wz = pb_wave_sum_u32(zero);
+      if (pb_lane() == 0 && wz) atomicAdd(&h[0], wz);
+      zero = 0;
     }
     // flush when the replica changes or at the end (histograms are per replica)
     const uint64_t pn = p + gridDim.x;

@@ -226,6 +226,11 @@ __device__ __forceinline__ double pb_wave_sum(double x) {
   for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
   return x;
 }
+__device__ __forceinline__ uint32_t pb_wave_sum_u32(uint32_t x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += (uint32_t)__shfl_xor((int)x, o, 64);
+  return x;
+}
 __device__ __forceinline__ double pb_wave_max(double x) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
