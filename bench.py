@@ -32,10 +32,6 @@ import particlebase_amd as pb  # noqa: E402
 from particlebase_amd import workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# VALU issue rate: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz lane-instructions/s
-# (FP32 FMA = one instruction, no packed FP32)
-VALU_LANE_OPS = 256 * 64 * 2.4e9
-GR_OPS_PER_PAIR = 15.0
 ALG_BYTES = 98.0          # FP64 r, v read + written (96 B) + u16 type read
 RC2 = 6.25
 DT = 0.005
@@ -108,6 +104,20 @@ def drift_direct(ctx, precision, skin, rebuild_every, steps=10000, warm=1000):
     del s
     return {"delta_e_per_atom": float(et[-1] - et[0]), "fit_per_1e4": drift_per_1e4(e, w.n, 100),
             "records": int(et.size), "steps": steps, "e_per_atom_start": float(et[0])}
+
+
+def bench_gr(sb, reps):
+    """g(r) of every replica (200 bins to L_k/2, the batched config's
+    histogram), 3 timed calls after one warm-up; integer totals checked."""
+    rsteps = np.array([np.float32(x.L / 400.0) for x in reps.systems])
+    sb.gr_hist(rsteps, 200)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        hist = sb.gr_hist(rsteps, 200)
+    elg = (time.perf_counter() - t0) / 3
+    pairs = int(sum(int(n) * (int(n) - 1) // 2 for n in reps.counts))
+    assert int(hist.sum()) == pairs
+    return {"pairs": pairs, "ms": 1e3 * elg, "pairs_per_s": pairs / elg, "bins": 200}
 
 
 def cpu_baseline(w, budget_s=20.0, cores=0):
@@ -258,12 +268,21 @@ def main():
                     help="only the mixed 1M run (for rocprofv3 kernel traces / counters)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only a device copy of 256 MiB, 5 times (HBM counter calibration)")
+    ap.add_argument("--gr-only", action="store_true",
+                    help="only g(r) of the 256 batched replicas, 3 times (for counter runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {a.gpus} is not supported "
               f"(the multi-GPU decomposition is a design, not part of this build)",
               file=sys.stderr)
         sys.exit(2)
+
+    if a.gr_only:
+        ctx = pb.Context(0)
+        reps = workloads.lj_replicas(256, 500, 20000, seed=5)
+        sb = make_system(ctx, reps)
+        log(f"g(r) only: {bench_gr(sb, reps)}")
+        return
 
     if a.copy_probe:
         import torch
@@ -356,19 +375,8 @@ def main():
         elb, eb = timed_run(ctx, sb, a.batch_steps, pb.PB_MIXED, a.skin, a.rebuild_every, 0)
         vb = ntot * a.batch_steps / elb
         log(f"batched: {len(reps.systems)} replicas, {ntot} atoms, {vb:.4g} atom-steps/s")
-        # g(r) of every replica on the final snapshot: 200 bins to L_k/2
-        rsteps = np.array([np.float32(x.L / 400.0) for x in reps.systems])
-        sb.gr_hist(rsteps, 200)
-        t0 = time.perf_counter()
-        for _ in range(3):
-            hist = sb.gr_hist(rsteps, 200)
-        elg = (time.perf_counter() - t0) / 3
-        pairs = int(sum(int(n) * (int(n) - 1) // 2 for n in reps.counts))
-        assert int(hist.sum()) == pairs
-        gr = {"pairs": pairs, "ms": 1e3 * elg, "pairs_per_s": pairs / elg,
-              "valu_frac": pairs / elg * GR_OPS_PER_PAIR / VALU_LANE_OPS,
-              "ops_per_pair": GR_OPS_PER_PAIR, "bins": 200}
-        log(f"g(r): {pairs} pairs in {1e3 * elg:.2f} ms, {pairs / elg:.4g} pairs/s")
+        gr = bench_gr(sb, reps)
+        log(f"g(r): {gr['pairs']} pairs in {gr['ms']:.2f} ms, {gr['pairs_per_s']:.4g} pairs/s")
         del sb
         plain_systems = [make_system(ctx, x) for x in reps.systems]
         for ps in plain_systems:
