@@ -314,3 +314,46 @@ def test_energy_drift_1e4_steps_n4000(ctx, precision):
     assert abs(et[-1] - et[0]) < 2e-3      # measured: about 4e-5 (mixed), 2e-5 (FP64)
     assert np.ptp(et) < 1e-2
     assert s.lj_run_stats()["skin_violation"] == 0
+
+
+@pytest.mark.parametrize("prec", [pb.PB_FP64, pb.PB_MIXED])
+def test_ragged_tiny_replicas(ctx, orc, prec):
+    print(f"\n[case] ragged batch with 1-, 2- and 7-atom replicas beside a 500-atom one, "
+          f"precision={prec}", flush=True)
+    # replicas of 1, 2, 7 and 500 atoms in boxes of different sizes: the
+    # one-atom replica feels no force, the others match the brute force
+    rng = np.random.default_rng(17)
+    sizes = [(1, 6.0), (2, 7.0), (7, 6.5), (500, 8.4)]
+    rs, cells = [], []
+    for n, L in sizes:
+        if n == 500:
+            w = workloads.lj_fcc(5, rho=0.9, T=0.7, seed=4)      # 500 atoms
+            x = (w.r * (L / w.L) + rng.normal(scale=0.03, size=w.r.shape)) % L
+        else:
+            x = rng.uniform(0, L, (n, 3))
+            if n == 2:
+                x[1] = (x[0] + [1.1, 0.2, -0.1]) % L
+        rs.append(x)
+        cells.append([0, L, 0, L, 0, L])
+    counts = [n for n, _ in sizes]
+    r = np.concatenate(rs)
+    s = pb.System(ctx, counts)
+    s.set_cell(np.concatenate(cells))
+    s.import_particles(r, np.ones(r.shape[0]))
+    s.lj_set_params([1.0, 1.0], 1)
+    s.lj_set_cutoff(RC2, True)
+    pe, _ = s.lj_force(prec)
+    F = s.get("F")
+    p = orc.lj_params([1.0, 1.0], 1, RC2, True)
+    off = 0
+    tol = 1e-11 if prec == pb.PB_FP64 else 2e-5
+    for k, (n, L) in enumerate(sizes):
+        x = rs[k]
+        Fo, peo, _ = orc.lj_forces_brute(x, np.zeros(n, np.uint16), np.array(cells[k], float),
+                                         RC2, p)
+        Fk = F[off:off + n]
+        scale = max(np.abs(Fo).max(), 1.0)
+        assert np.abs(Fk - Fo).max() < tol * scale, k
+        assert pe[k] == pytest.approx(peo, rel=1e-5 if prec == pb.PB_MIXED else 1e-11, abs=1e-9)
+        off += n
+    assert np.all(F[:1] == 0)
