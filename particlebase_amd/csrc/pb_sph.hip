@@ -144,8 +144,8 @@ __device__ __forceinline__ double sph_norm(double r2, double *rinv) {
 }
 
 template <int M>
-__global__ void __launch_bounds__(SC_T) __attribute__((amdgpu_waves_per_eu(6)))
-k_sph_pairs(SphArgs A, uint32_t zseg, SphFilter FB, double4 *out4, double *out1) {
+__device__ __forceinline__ void sph_pairs(const SphArgs &A, uint32_t zseg, SphFilter FB,
+                                          double4 *out4, double *out1) {
   constexpr int NF = sph_nf<M>();
   constexpr uint32_t CAP = sph_cap<M>();
   __shared__ double L[NF][CAP];
@@ -158,7 +158,6 @@ k_sph_pairs(SphArgs A, uint32_t zseg, SphFilter FB, double4 *out4, double *out1)
   const uint32_t s0 = A.cstart[col * nz + zs], s1 = A.cstart[col * nz + ze];
   if (s0 >= s1) return;
   const double4 O = A.rs[s0];                 // origin of the filter coordinates
-  const double r2max = (M == SPH_LUCY_DENS || M == SPH_LUCY_ACC) ? A.h * A.h : A.sup2;
   for (uint32_t c0 = s0; c0 < s1; c0 += SC_T) {
     const uint32_t c1 = c0 + SC_T < s1 ? c0 + SC_T : s1;
     const uint32_t k = c0 + threadIdx.x;
@@ -347,6 +346,18 @@ k_sph_pairs(SphArgs A, uint32_t zseg, SphFilter FB, double4 *out4, double *out1)
   }
 }
 
+// The config density fits six wavefronts per SIMD (80 VGPRs, 25 KB of
+// LDS); the other passes need more registers and run at four or five.
+template <int M>
+__global__ void __launch_bounds__(SC_T) k_sph_pairs(SphArgs A, uint32_t zseg, SphFilter FB,
+                                                    double4 *out4, double *out1) {
+  sph_pairs<M>(A, zseg, FB, out4, out1);
+}
+__global__ void __launch_bounds__(SC_T) __attribute__((amdgpu_waves_per_eu(6)))
+k_sph_pairs_wc_density(SphArgs A, uint32_t zseg, SphFilter FB, double4 *out4, double *out1) {
+  sph_pairs<SPH_WC_DENS>(A, zseg, FB, out4, out1);
+}
+
 // inspectDense_x (kernelfuncs.cu:772-801): per-wave partial sums of the
 // number density and the count of movable particles
 __global__ void k_sph_inspect(const double4 *num, const uint8_t *move, const uint32_t *orig,
@@ -515,7 +526,7 @@ extern "C" int pb_sph_density(pb_sys *s) {
   hipLaunchKernelGGL(k_sph_gather, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->perm, s->r, s->v,
                      wc ? (const double *)nullptr : s->sph_mu0, n, s->sph_rs, s->sph_vs);
   if (wc)
-    hipLaunchKernelGGL(k_sph_pairs<SPH_WC_DENS>, grid, dim3(SC_T), 0, st, A, zseg,
+    hipLaunchKernelGGL(k_sph_pairs_wc_density, grid, dim3(SC_T), 0, st, A, zseg,
                        sph_filter(s, zseg, A.sup2), s->sph_num, s->sph_rhoinv);
   else
     hipLaunchKernelGGL(k_sph_pairs<SPH_LUCY_DENS>, grid, dim3(SC_T), 0, st, A, zseg,
