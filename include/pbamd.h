@@ -133,6 +133,8 @@ int pb_inspect_velocity(pb_sys *sys, double vlim, double lim_u,
  * rules of the reference, single replica only.  single_particle selects the
  * 125-neighbour single-occupancy mode (:115-138). */
 int pb_setup_cells(pb_sys *sys, float rmax, int periodic, int single_particle);
+/* the blocknum / blocklen fields that setupCutoffBlock fills
+ * (cudaCutoffBlock.hh:30-32). */
 int pb_get_cell_info(pb_sys *sys, uint32_t blocknum[3], float blocklen[6]);
 /* calcBID_F4 + sort + makeBindex, then cudaSortedPositions::calcBlockID
  * (cudaCutoffBlock.cu:233-258, kernelfuncs.cu:807-932,
@@ -140,7 +142,9 @@ int pb_get_cell_info(pb_sys *sys, uint32_t blocknum[3], float blocklen[6]);
  * the reference computes them; canonical stable order (ascending original
  * index inside a cell). */
 int pb_bin(pb_sys *sys);
-/* download pid[n], bid (sorted)[n], bindex[ncell+1] (UINT32_MAX = empty). */
+/* download of the pid / bid / bindex arrays that calcBlockID leaves
+ * (cudaCutoffBlock.cu:233-258): pid[n], bid (sorted)[n], bindex[ncell+1]
+ * (UINT32_MAX = empty). */
 int pb_get_bins(pb_sys *sys, uint32_t *pid, uint32_t *bid_sorted,
                 uint32_t *bindex);
 /* calc_BlockNeighbor table (cudaCutoffBlock.cu:17-65): ncell*27 or *125. */
@@ -180,22 +184,27 @@ typedef struct {
                               context's stream (pb_lj_last_run_timing) */
 } pb_run_opts;
 
-/* Fused cudaParticleMD loop: nsteps of [force -> VV -> periodic wrap] with a
+/* Fused cudaParticleMD loop (the testMD1.cu:135-169 step: calcBlockID,
+ * calcForce, TimeEvolution, treatPeriodicCondition; no single reference
+ * function): nsteps of [force -> VV -> periodic wrap] with a
  * Verlet list re-built (and particles re-sorted into cell order) every
  * rebuild_every steps, or earlier when a particle may have moved half its
  * replica's skin.  With several replicas, replica q's skin is
  * skin * (T_q / T_min)^(1/4) from the temperatures at the start (capped
- * below half its box), so hot replicas do not force rebuilds of all.  Starts from and ends in the reference VV state
- * (r(t), v(t-dt), Fold = F(t-dt)); call pb_clear_force(sys, 1) first for the
- * reference's start.  energies (may be NULL) receives, for each recorded
+ * below half its box), so hot replicas do not force rebuilds of all.
+ * Starts from and ends in the reference VV state (r(t), v(t-dt),
+ * Fold = F(t-dt)); call pb_clear_force(sys, 1) first for the reference's
+ * start.  energies (may be NULL) receives, for each recorded
  * step s (s % energy_every == 0), nrep x {KE(v(t)), PE(r(t)), W(r(t))}. */
 int pb_lj_run(pb_sys *sys, const pb_run_opts *opts, double *energies);
-/* statistics of the last pb_lj_run: rebuild count, max list length, and
- * whether a skin violation occurred (1) -- never expected for sane skins. */
+/* (no reference counterpart) statistics of the last pb_lj_run: rebuild
+ * count, max list length, and whether a skin violation occurred (1) --
+ * never expected for sane skins. */
 int pb_lj_last_run_stats(pb_sys *sys, uint32_t *rebuilds, uint32_t *max_nn,
                          int *skin_violation);
 
-/* GPU time of the last pb_lj_run with opts.timing = 1: summed HIP-event time
+/* (no reference counterpart) GPU time of the last pb_lj_run with
+ * opts.timing = 1: summed HIP-event time
  * of the fused step kernels and of the list rebuilds, and their counts. */
 int pb_lj_last_run_timing(pb_sys *sys, double *step_ms, uint32_t *nstep,
                           double *rebuild_ms, uint32_t *nrebuild);
