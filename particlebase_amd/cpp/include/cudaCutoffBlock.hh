@@ -1,8 +1,12 @@
 // cudaCutoffBlock.hh -- cell grid ("cutoff blocks") of the reference
 // (cudaCutoffBlock.hh:241-240, cudaCutoffBlock.cu:75-307).  One GPU holds the
-// whole domain, so the multi-GPU block ranges and force imports are refused.
+// whole domain, so only the full block range is accepted; the force and
+// acceleration imports merge another object's arrays as the reference does
+// (its peer copy becomes a read of the other object on the same GPU).
 #pragma once
 #include <cstdint>
+#include <stdexcept>
+#include <string>
 #include <vector>
 #include "cudaParticleBase.hh"
 
@@ -39,6 +43,21 @@ protected:
       throw std::runtime_error("readSerialization: cell grid differs");
   }
 
+  void importField(const cudaCutoffBlock &A, int field, bool directAccess, const char *what) {
+    if (A.N != N) throw std::runtime_error(std::string(what) + ": particle counts differ");
+    std::vector<double> mine(3 * (size_t)N), other(3 * (size_t)N);
+    pbamd::check(pb_sys_get(sys, field, mine.data()), what);
+    if (directAccess) {
+      pbamd::check(pb_sys_get(A.sys, field, other.data()), what);
+    } else {
+      if (A.TMP.size() < 4 * (size_t)N) throw std::runtime_error(std::string(what) + ": TMP");
+      for (size_t i = 0; i < N; ++i)
+        for (int c = 0; c < 3; ++c) other[3 * i + c] = (double)A.TMP[4 * i + c];
+    }
+    for (size_t k = 0; k < mine.size(); ++k) mine[k] += other[k];
+    pbamd::check(pb_sys_set(sys, field, mine.data()), what);
+  }
+
 public:
   // cudaCutoffBlock::setupCutoffBlock, cudaCutoffBlock.cu:75-231
   void setupCutoffBlock(real rmax, bool periodic = true) {
@@ -66,6 +85,23 @@ public:
     if (uninitialized) SingleParticleBlock = (t == ParticleBlockType::single);
     else std::cerr << "cudaCutoffBlock::switchBlockAlgorithm() rejects" << std::endl;
   }
+  // importAcceleration / importForce (cudaCutoffBlock.cu:260-286): add the
+  // other object's a or F (directAccess: its device array; otherwise its
+  // TMP host array, float4 per particle) to this object's a or F.
+  // idMe / idPeer name GPUs of a multi-GPU run and have no effect here.
+  void importAcceleration(const cudaCutoffBlock &A, bool directAccess = false, int idMe = 0,
+                          int idPeer = 0) {
+    importField(A, PB_FIELD_A, directAccess, "importAcceleration");
+    (void)idMe;
+    (void)idPeer;
+  }
+  void importForce(const cudaCutoffBlock &A, bool directAccess = false, int idMe = 0,
+                   int idPeer = 0) {
+    importField(A, PB_FIELD_F, directAccess, "importForce");
+    (void)idMe;
+    (void)idPeer;
+  }
+
   // downloads of the sort (pid, sorted bid, bindex with UINT32_MAX = empty)
   void getBlocks(std::vector<uint32_t> &pid, std::vector<uint32_t> &bid,
                  std::vector<uint32_t> &bindex) {

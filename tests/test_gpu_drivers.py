@@ -3,9 +3,10 @@ call sequences through the class layer and the C ABI on the GPU: LJ
 (testMD1.cu order, per-call API and fused; and the testMD1 gas with
 initialAnnealing and the neighbour-list path), liquid argon under the Gaussian
 thermostat (testMD2.cu order), checkpoint / restart of MD, DEM and SPH
-objects (bit-exact continuation), SPH dam break (testSPH5.cu
-order with AdaptiveTime; reference and config physics) and DEM settle (testDEM5.cu order with
-AdaptiveTime).  Each must finish and report a physically sane end state."""
+objects (bit-exact continuation), SPH dam break (testSPH5.cu order with
+AdaptiveTime; reference and config physics), DEM settle (testDEM5.cu order
+with AdaptiveTime) and the force / acceleration imports of cudaCutoffBlock.
+Each must finish and report a physically sane end state."""
 import os
 import re
 import subprocess
@@ -89,3 +90,10 @@ def test_restart_exact(mode, tmp_path):
     d = run("restart", mode, 15, str(tmp_path / f"{mode}.ck"))
     assert int(d["k"]) == 15 and int(d["N"]) > 500
     assert float(d["maxdiff"]) == 0.0
+
+
+def test_import_force_and_acceleration():
+    # cudaCutoffBlock::importForce / importAcceleration between two objects:
+    # the other object's device array and its TMP host array are added
+    d = run("import_check")
+    assert int(d["ok"]) == 1 and int(d["N"]) == 1000
