@@ -67,6 +67,13 @@ const char *pb_ctx_last_error(const pb_ctx *ctx);
 int         pb_ctx_sync(pb_ctx *ctx);
 /* Device name and compute-unit count (CUDAenv::setThnum, CUDAenv.hh:36-50). */
 int         pb_ctx_info(pb_ctx *ctx, char *name, size_t name_len, int *cu_count);
+/* Device limits gathered by CUDAenv's constructor (CUDAenv.hh:110-118):
+ * global memory in bytes (totalGlobalMem), threads per block, grid size in
+ * x, LDS bytes per block (sharedMemPerBlock) and vector lanes per compute
+ * unit (numberOfCores).  Null pointers are skipped. */
+int         pb_ctx_limits(pb_ctx *ctx, uint64_t *global_mem, uint32_t *max_threads,
+                          uint32_t *max_grid, uint32_t *lds_per_block,
+                          uint32_t *lanes_per_cu);
 
 /* ---------------------------------------------------------------- system */
 /* cudaParticleBase::setup(int n) (cudaParticleBase.cu:18-66).  nrep = 1 is

@@ -36,6 +36,7 @@ FIELD = {"r": 0, "v": 1, "a": 2, "F": 3, "minv": 4, "type": 5, "move": 6,
 # symbols declared in include/pbamd.h (checked by the CPU test suite)
 EXPORTS = [
     "pb_ctx_create", "pb_ctx_destroy", "pb_ctx_last_error", "pb_ctx_sync", "pb_ctx_info",
+    "pb_ctx_limits",
     "pb_sys_create", "pb_sys_destroy", "pb_sys_count", "pb_sys_replicas", "pb_sys_set_cell",
     "pb_sys_import", "pb_sys_get", "pb_sys_set", "pb_clear_force", "pb_calc_acceleration",
     "pb_add_force", "pb_add_acceleration", "pb_treat_periodic", "pb_treat_absolute",
@@ -87,6 +88,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         "pb_ctx_last_error": (C.c_char_p, [vp]),
         "pb_ctx_sync": (i32, [vp]),
         "pb_ctx_info": (i32, [vp, C.c_char_p, C.c_size_t, P(i32)]),
+        "pb_ctx_limits": (i32, [vp, P(C.c_uint64), P(u32), P(u32), P(u32), P(u32)]),
         "pb_sys_create": (i32, [vp, u32, P(u32), P(vp)]),
         "pb_sys_destroy": (None, [vp]),
         "pb_sys_count": (u32, [vp]),
@@ -208,6 +210,16 @@ class Context:
         cu = C.c_int()
         self.check(self.lib.pb_ctx_info(self.h, buf, 128, C.byref(cu)))
         return buf.value.decode(), cu.value
+
+    def limits(self):
+        """Device limits: global memory bytes, threads per block, grid x,
+        LDS bytes per block, lanes per compute unit."""
+        mem = C.c_uint64()
+        th, grid, lds, lanes = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self.check(self.lib.pb_ctx_limits(self.h, C.byref(mem), C.byref(th), C.byref(grid),
+                                          C.byref(lds), C.byref(lanes)))
+        return {"global_mem": mem.value, "max_threads": th.value, "max_grid": grid.value,
+                "lds_per_block": lds.value, "lanes_per_cu": lanes.value}
 
     def sync(self):
         self.check(self.lib.pb_ctx_sync(self.h))

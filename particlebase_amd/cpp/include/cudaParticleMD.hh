@@ -141,6 +141,20 @@ public:
     pbamd::check(pb_mv2(sys, &out), "calcMV2");
     return (real)out;
   }
+  // statMV2 (cudaParticleMD.cu:157-166): two blank lines, then "i<TAB>v_i^2"
+  // per particle; like the reference it lists the squared speed (the mass
+  // enters only calcMV2's sum)
+  void statMV2(std::ostream &o = std::cout) {
+    calcMV2();
+    const std::vector<double> v = getState(PB_FIELD_V);
+    o << std::endl << std::endl;
+    for (uint32_t i = 0; i < N; ++i) {
+      const double v2 = v[3 * (size_t)i] * v[3 * (size_t)i] +
+                        v[3 * (size_t)i + 1] * v[3 * (size_t)i + 1] +
+                        v[3 * (size_t)i + 2] * v[3 * (size_t)i + 2];
+      o << i << "\t" << (float)v2 << std::endl;
+    }
+  }
   real calcTemp(void) {
     mv2 = calcMV2();
     return mv2 / (N * 3 * kB);

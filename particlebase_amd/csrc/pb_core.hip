@@ -35,6 +35,10 @@ extern "C" int pb_ctx_create(int device, pb_ctx **out) {
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
     c->cu_count = prop.multiProcessorCount;
     c->name = prop.gcnArchName;
+    c->global_mem = (uint64_t)prop.totalGlobalMem;
+    c->max_threads = (uint32_t)prop.maxThreadsPerBlock;
+    c->max_grid = (uint32_t)prop.maxGridSize[0];
+    c->lds_per_block = (uint32_t)prop.sharedMemPerBlock;
   }
   // default-priority stream (never a different priority)
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -72,6 +76,19 @@ extern "C" int pb_ctx_info(pb_ctx *ctx, char *name, size_t name_len, int *cu_cou
     name[name_len - 1] = 0;
   }
   if (cu_count) *cu_count = ctx->cu_count;
+  return PB_OK;
+}
+
+extern "C" int pb_ctx_limits(pb_ctx *ctx, uint64_t *global_mem, uint32_t *max_threads,
+                             uint32_t *max_grid, uint32_t *lds_per_block,
+                             uint32_t *lanes_per_cu) {
+  if (!ctx) return PB_ERR_ARG;
+  if (global_mem) *global_mem = ctx->global_mem;
+  if (max_threads) *max_threads = ctx->max_threads;
+  if (max_grid) *max_grid = ctx->max_grid;
+  if (lds_per_block) *lds_per_block = ctx->lds_per_block;
+  // a CDNA compute unit issues to 4 SIMDs of 16 lanes each
+  if (lanes_per_cu) *lanes_per_cu = 64;
   return PB_OK;
 }
 

@@ -49,6 +49,8 @@ struct pb_ctx {
   std::string err;
   int cu_count = 0;
   std::string name;
+  uint64_t global_mem = 0;
+  uint32_t max_threads = 0, max_grid = 0, lds_per_block = 0;
 };
 
 // device buffer helper

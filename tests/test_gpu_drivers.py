@@ -97,3 +97,12 @@ def test_import_force_and_acceleration():
     # the other object's device array and its TMP host array are added
     d = run("import_check")
     assert int(d["ok"]) == 1 and int(d["N"]) == 1000
+
+
+def test_env_queries_and_stat_mv2():
+    # CUDAenv::getShMem / getNumTh / getMaxGrid / getMemSize / getNumCpMP /
+    # display and cudaParticleMD::statMV2 (per-particle v^2 listing)
+    d = run("env_stat_check")
+    assert int(d["ok"]) == 1 and int(d["rows"]) == int(d["N"]) == 500
+    # 6 significant digits in the listing (the stream's default precision)
+    assert float(d["relerr"]) < 1e-5 and int(d["cp"]) == 64
