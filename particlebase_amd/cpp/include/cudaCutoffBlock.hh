@@ -76,7 +76,9 @@ public:
   uint32_t numBlocks(void) { return totalNumBlock; }
   // the whole domain is on this GPU: only the full range is accepted
   void setBlockRange(uint32_t blockNum, uint32_t n, uint32_t myID) {
-    if (n != 1 || myID != 0 || blockNum != totalNumBlock) {
+    uint32_t off = 0, num = 0;
+    calcBlockRange(blockNum, n, myID, [&](uint32_t o, uint32_t k) { off = o; num = k; });
+    if (off != 0 || num != totalNumBlock) {
       std::cerr << "setBlockRange: one GPU owns every block in this build" << std::endl;
       std::exit(1);
     }
@@ -84,6 +86,20 @@ public:
   void switchBlockAlgorithm(const ParticleBlockType t) {
     if (uninitialized) SingleParticleBlock = (t == ParticleBlockType::single);
     else std::cerr << "cudaCutoffBlock::switchBlockAlgorithm() rejects" << std::endl;
+  }
+  // calcBlockRange (cudaCutoffBlock.hh:55-70): split blockNum cells over n
+  // devices, the first blockNum % n taking one extra; setPara(offset, count)
+  // receives device myID's share
+  template <typename F>
+  static void calcBlockRange(uint32_t blockNum, uint32_t n, uint32_t myID, F setPara) {
+    if (n == 0 || myID >= n) {
+      std::cerr << "calcBlockRange: device " << myID << " of " << n << std::endl;
+      std::exit(1);
+    }
+    const uint32_t base = blockNum / n, extra = blockNum % n;
+    const uint32_t count = base + (myID < extra ? 1u : 0u);
+    const uint32_t offset = myID * base + (myID < extra ? myID : extra);
+    setPara(offset, count);
   }
   // importAcceleration / importForce (cudaCutoffBlock.cu:260-286): add the
   // other object's a or F (directAccess: its device array; otherwise its

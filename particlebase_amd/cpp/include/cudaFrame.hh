@@ -4,6 +4,7 @@
 // THnum* stay as fields so drivers that set or print them still compile.
 #pragma once
 #include <cstdint>
+#include <ctime>
 #include <mutex>
 #include <ostream>
 #include <string>
@@ -34,6 +35,13 @@ public:
       std::cerr << s << ": " << e << std::endl;
       std::exit(1);
     }
+  }
+
+  // nowtime (cudaFrame.cu:93-97): print the wall-clock date, return it
+  time_t nowtime(std::ostream &o = std::cout) {
+    const time_t T = time(nullptr);
+    o << ctime(&T);
+    return T;
   }
 
   // cudaFrame::setCell, cudaFrame.cu:85-91

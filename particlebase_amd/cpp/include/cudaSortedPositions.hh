@@ -7,5 +7,8 @@
 class cudaSortedPositions : public virtual cudaCutoffBlock {
 public:
   void setupSortedPositions(void) {}
+  // reset (cudaSortedPositions.hh:11-18) re-sizes r_s / typeID_s after a
+  // load; the cell-ordered storage is sized with the system
+  void reset(void) {}
   void calcBlockID(void) {}
 };

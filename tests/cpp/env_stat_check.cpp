@@ -1,5 +1,7 @@
 // env_stat_check.cpp -- CUDAenv's device queries and display (CUDAenv.hh:
-// 29-34, 177-194) and cudaParticleMD::statMV2 (cudaParticleMD.cu:157-166).
+// 29-34, 177-194), cudaCutoffBlock::calcBlockRange (cudaCutoffBlock.hh:
+// 55-70), cudaFrame::nowtime and cudaParticleMD::statMV2
+// (cudaParticleMD.cu:157-166).
 // statMV2's listing is parsed back and compared with v^2 of the imported
 // velocities; prints "done ok=1" with the queried limits.  Built with the
 // drivers (make -C particlebase_amd/cpp).
@@ -37,6 +39,14 @@ int main() {
   ok = ok && env.getShMem() >= 65536 && env.getNumTh() >= 256 && env.getMaxGrid() > 0 &&
        env.getMemSize() > (1ull << 32) && env.getNumCpMP() == 64 && env.getNumMP() > 0;
   env.exchangeForceSelected(ExchangeMode::acceleration);   // one GPU: nothing to move
+  // calcBlockRange: 10 cells over 3 devices -> [0,4) [4,7) [7,10)
+  const uint32_t want_off[3] = {0, 4, 7}, want_num[3] = {4, 3, 3};
+  for (uint32_t id = 0; id < 3; ++id)
+    cudaCutoffBlock::calcBlockRange(10, 3, id, [&](uint32_t o, uint32_t k) {
+      ok = ok && o == want_off[id] && k == want_num[id];
+    });
+  std::ostringstream clock;
+  ok = ok && env[0].nowtime(clock) > 0 && !clock.str().empty();
 
   cudaParticleMD &P = env[0];
   P.setup(n);
