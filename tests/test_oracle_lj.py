@@ -35,7 +35,7 @@ def test_two_body_closed_form(orc, dist, rng):
     assert np.allclose(F[1], fmag * u, rtol=1e-13, atol=1e-13)
     assert pe == pytest.approx(phi, rel=1e-13, abs=1e-15)
     assert w == pytest.approx(fmag * dist, rel=1e-13)
-    # the reference's phi(rc) at rc = 2.5 (SURVEY config 1)
+    # unshifted phi(rc) at rc = 2.5 in reduced units (the 1M-atom LJ fluid)
     assert lj_closed(RC)[1] == 0.0
     assert 4 * (RC ** -12 - RC ** -6) == pytest.approx(-0.016316891136, rel=1e-9)
 
