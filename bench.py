@@ -323,6 +323,21 @@ def main():
     f32 = s.get("F")
     frms = float(np.sqrt((f64 ** 2).sum(axis=1).mean()))
     ferr = float(np.abs(f32 - f64).max() / frms)
+    # the same run from host buffers to host buffers: import (host -> HBM,
+    # cell order restored by the next rebuild) before and positions and
+    # velocities back to the host after the timed steps
+    ctx.sync()
+    t0 = time.perf_counter()
+    s.import_particles(w.r, w.m, v=w.v, type=w.type)
+    ctx.sync()
+    t_in = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    s.get("r")
+    s.get("v")
+    t_out = time.perf_counter() - t0
+    host_io = {"value": w.n * a.steps / (el + t_in + t_out), "unit": "atom-steps/s",
+               "import_ms": 1e3 * t_in, "get_r_v_ms": 1e3 * t_out, "steps": a.steps}
+    log(f"host-buffer inclusive: {host_io}")
     del s
     if a.profile:
         return
@@ -362,7 +377,8 @@ def main():
                   drift_direct(ctx, pb.PB_MIXED, a.skin, a.rebuild_every),
                   "tiles": tiles,
                   "max_nn": stats["max_nn"], "skin_violation": stats["skin_violation"],
-                  "rebuild_ms_avg": tm["rebuild_ms"] / max(tm["nrebuild"], 1)},
+                  "rebuild_ms_avg": tm["rebuild_ms"] / max(tm["nrebuild"], 1),
+                  "host_buffers_inclusive": host_io},
     }
 
     # ---- batched replicas vs plain per-replica FP64
