@@ -6,6 +6,7 @@
 #include <vector>
 #include "cudaCutoffBlock.hh"
 #include "cudaParticleLF.hh"
+#include "putTMPselected.hh"
 
 class cudaParticleRotation : public virtual cudaParticleBase {
 protected:
@@ -60,7 +61,8 @@ public:
 
 class cudaParticleDEM : public cudaParticleLF,
                         public cudaParticleRotation,
-                        virtual public cudaCutoffBlock {
+                        virtual public cudaCutoffBlock,
+                        public putTMPselected {
 protected:
   uint32_t lastNNZ = 0;
   // the arguments of setDEMProperties, kept for restarts
@@ -114,6 +116,7 @@ public:
     saveBase(a);
     saveBlock(a);
     saveRotation(a);
+    saveSelected(a);
     a.scalar("cudaParticleDEM.demSet", (uint8_t)demSet);
     a.scalar("cudaParticleDEM.autotunetimestep", (uint8_t)autotunetimestep);
     if (!demSet) return;
@@ -142,6 +145,7 @@ public:
   void load(const pbamd::IArchive &a) override {
     loadBase(a);
     loadBlock(a);
+    loadSelected(a);
     autotunetimestep = a.scalar<uint8_t>("cudaParticleDEM.autotunetimestep") != 0;
     if (a.scalar<uint8_t>("cudaParticleDEM.demSet")) {
       demR = a.get<double>("cudaParticleDEM.radius");

@@ -8,13 +8,21 @@
 // one GPU owns every particle here, so they are accepted and have no effect.
 #pragma once
 #include <cmath>
+#include <limits>
+#include <sstream>
 #include <valarray>
 #include "cudaCutoffBlock.hh"
 #include "cudaParticleLF.hh"
+#include "putTMPselected.hh"
 
-class cudaParticleSPH_NS : public cudaParticleLF, virtual public cudaCutoffBlock {
+class cudaParticleSPH_NS : public cudaParticleLF,
+                           virtual public cudaCutoffBlock,
+                           public putTMPselected {
 protected:
   real h = 0;
+  // particles held in TMP's first block: the n, rho, p block starts after
+  // them (cudaParticleSPH_NS.hh:111)
+  uint32_t offset_additional_output = 0;
   double meanNum = 0.0;
   std::vector<double> mu0s, cs;    // setSPHProperties arguments, kept for restarts
   // setWCSPHProperties arguments (h, rho0, c0, gamma, alpha, pmin); empty
@@ -25,6 +33,19 @@ public:
   void setup(int n) override {
     cudaParticleBase::setup(n);
     TMP.resize(8 * (size_t)n);       // x, y, z, _, num, rho, mu, _ (cudaParticleSPH_NS.cu:13-27)
+    offset_additional_output = (uint32_t)n;
+  }
+
+  // dump columns after x y z (cudaParticleSPH_NS.hh:37, cudaParticleSPH_NS.cu:
+  // 64-69): the three density-pass values of the particle, FP32 printed
+  // with the 9 digits that round-trip
+  std::string additionalTag(void) const override { return " n rho p"; }
+  std::string additionalOutput(uint32_t i) const override {
+    std::ostringstream o;
+    o.precision(std::numeric_limits<float>::max_digits10);
+    const size_t b = 4 * ((size_t)i + offset_additional_output);
+    o << " " << TMP[b] << " " << TMP[b + 1] << " " << TMP[b + 2];
+    return o.str();
   }
 
   // SPH needs no force pass of its own (testSPH5.cu:216 "do nothing")
@@ -62,6 +83,7 @@ public:
   void save(pbamd::OArchive &a) const override {
     saveBase(a);
     saveBlock(a);
+    saveSelected(a);
     a.scalar("cudaParticleSPHBase.h", h);
     a.scalar("cudaParticleSPHBase.w0", 105.0 / (16.0 * M_PI * std::pow((double)h, 7)));
     a.put("cudaParticleSPH_NS.viscosityMu", mu0s);
@@ -74,6 +96,7 @@ public:
   void load(const pbamd::IArchive &a) override {
     loadBase(a);
     loadBlock(a);
+    loadSelected(a);
     h = a.scalar<real>("cudaParticleSPHBase.h");
     mu0s = a.get<double>("cudaParticleSPH_NS.viscosityMu");
     cs = a.get<double>("cudaParticleSPH_NS.soundVelocityC");
@@ -116,8 +139,23 @@ public:
     std::vector<double> num(3 * (size_t)N);
     pbamd::check(pb_sys_get(sys, PB_FIELD_SPH_NUM, num.data()), "getPosition");
     std::lock_guard<std::mutex> lk(mutTMP);
+    offset_additional_output = N;
     for (size_t i = 0; i < N; ++i)
       for (int k = 0; k < 3; ++k) TMP[4 * (size_t)N + 4 * i + k] = (real)num[3 * i + k];
+  }
+  // getSelectedPosition (cudaParticleSPH_NS.cu:40-50): the selected range's
+  // x, y, z, then its n, rho, p block right after it
+  void getSelectedPosition(void) override {
+    fetchRange(selected[0], selected[1]);
+    std::vector<double> num(3 * (size_t)N);
+    pbamd::check(pb_sys_get(sys, PB_FIELD_SPH_NUM, num.data()), "getSelectedPosition");
+    std::lock_guard<std::mutex> lk(mutTMP);
+    offset_additional_output = selected[1];
+    const size_t b = 4 * (size_t)selected[1];
+    for (size_t i = 0; i < selected[1]; ++i) {
+      for (int k = 0; k < 3; ++k) TMP[b + 4 * i + k] = (real)num[3 * (selected[0] + i) + k];
+      TMP[b + 4 * i + 3] = 0;
+    }
   }
 
   // SPHBase::rollback / rollback2 (cudaParticleSPHBase.cu:127-132); the

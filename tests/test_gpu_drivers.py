@@ -106,3 +106,12 @@ def test_env_queries_and_stat_mv2():
     assert int(d["ok"]) == 1 and int(d["rows"]) == int(d["N"]) == 500
     # 6 significant digits in the listing (the stream's default precision)
     assert float(d["relerr"]) < 1e-5 and int(d["cp"]) == 64
+
+
+def test_put_tmp_selected(tmp_path):
+    # putTMPselected on an SPH block: the walls through putUnSelected, the
+    # fluid through getSelectedTypeID / getSelectedPosition / putTMP with
+    # its n, rho, p columns, both parsed back and compared
+    d = run("select_check", str(tmp_path))
+    assert int(d["ok"]) == 1 and int(d["N1"]) == 12 * 12 * 8
+    assert float(d["poserr"]) < 1e-5 and float(d["numerr"]) < 1e-7
