@@ -11,6 +11,7 @@
 #include <limits>
 #include <sstream>
 #include <valarray>
+#include "ExchangeMode.hh"
 #include "cudaCutoffBlock.hh"
 #include "cudaParticleLF.hh"
 #include "putTMPselected.hh"
@@ -19,6 +20,14 @@ class cudaParticleSPH_NS : public cudaParticleLF,
                            virtual public cudaCutoffBlock,
                            public putTMPselected {
 protected:
+  static int selectedField(const ExchangeMode typeID, const char *what) {
+    switch (typeID) {
+      case ExchangeMode::force: return PB_FIELD_F;
+      case ExchangeMode::acceleration: return PB_FIELD_A;
+      case ExchangeMode::density: return PB_FIELD_SPH_NUM;
+      default: throw std::runtime_error(std::string(what) + ": SPH exchanges force, acceleration or density");
+    }
+  }
   real h = 0;
   // particles held in TMP's first block: the n, rho, p block starts after
   // them (cudaParticleSPH_NS.hh:111)
@@ -162,6 +171,32 @@ public:
   // second stage re-initialises v with the (empty) SPH force, as there
   void rollback(real dt) { cudaParticleLF::rollback(dt); }
   void rollback2(real dt) { cudaParticleLF::calcVinit(dt / 2); }
+
+  // getForceSelected / importForceSelected (cudaParticleSPH_NS.hh:96-108,
+  // cudaSelectedBlock.cu:46-81).  One GPU owns every moving particle, so
+  // the selected range is the whole system.  force / acceleration / density
+  // (n, rho, p) are staged in TMP (4 per particle); the import copies the
+  // other object's values (no summation) from its device arrays
+  // (directAccess) or from its TMP.
+  void getForceSelected(const ExchangeMode typeID = ExchangeMode::force) {
+    fetch(selectedField(typeID, "getForceSelected"));
+  }
+  void importForceSelected(const cudaParticleSPH_NS &A, const ExchangeMode typeID = ExchangeMode::force,
+                           bool directAccess = false, int idMe = 0, int idPeer = 0) {
+    (void)idMe;
+    (void)idPeer;
+    const int field = selectedField(typeID, "importForceSelected");
+    if (A.N != N) throw std::runtime_error("importForceSelected: particle counts differ");
+    std::vector<double> other(3 * (size_t)N);
+    if (directAccess) {
+      pbamd::check(pb_sys_get(A.sys, field, other.data()), "importForceSelected");
+    } else {
+      if (A.TMP.size() < 4 * (size_t)N) throw std::runtime_error("importForceSelected: TMP");
+      for (size_t i = 0; i < N; ++i)
+        for (int c = 0; c < 3; ++c) other[3 * i + c] = (double)A.TMP[4 * i + c];
+    }
+    pbamd::check(pb_sys_set(sys, field, other.data()), "importForceSelected");
+  }
 
   // multi-GPU exchange ranges: one GPU owns all particles
   void getExchangePidRange1(void) {}

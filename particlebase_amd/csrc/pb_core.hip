@@ -394,6 +394,8 @@ extern "C" int pb_sys_set(pb_sys *s, int field, const void *host) {
   if ((field == PB_FIELD_W || field == PB_FIELD_L || field == PB_FIELD_T) && (rc = pb_ensure_rotation(s)))
     return rc;
   if (field == PB_FIELD_RADIUS && (rc = pb_ensure_radius(s))) return rc;
+  if (field == PB_FIELD_SPH_NUM && !s->sph_num)
+    return pb_fail(s->ctx, PB_ERR_STATE, "pb_sys_set: no SPH density yet");
   std::vector<uint32_t> o = pb_download_orig(s, &rc);
   if (rc) return rc;
   const size_t np = s->npad;

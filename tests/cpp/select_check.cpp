@@ -3,7 +3,9 @@
 // [0, N1)) over wall particles (ids [N1, N)).  putUnSelected writes the
 // walls; getSelectedTypeID + getSelectedPosition + putTMP write the fluid
 // with its n, rho, p columns.  Both dumps are parsed back and compared with
-// the imported positions and the density pass.  Prints "done ok=1".
+// the imported positions and the density pass.  importForceSelected copies
+// one block's density (device) and acceleration (TMP) into a second block.
+// Prints "done ok=1".
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -122,6 +124,28 @@ int main(int argc, char **argv) {
   }
   // positions print with the stream's 6 digits; n, rho, p with 9
   ok = ok && worst < 1e-5 && worst_num < 1e-7;
+
+  // importForceSelected between two blocks (copies, no summation): the
+  // density block from the device, then the acceleration from TMP
+  CUDAenv<cudaParticleSPH_NS> env2;
+  env2.setup();
+  cudaParticleSPH_NS &B = env2[0];
+  B.setup(N);
+  B.setCell(cell);
+  B.import(G);
+  B.setWCSPHProperties(h, 1000.0, 10.0, 7.0, 0.1, 0.0);
+  B.setupCutoffBlock((real)(2 * h), false);
+  B.calcBlockID();
+  B.calcDensity();
+  std::vector<double> zero(3 * (size_t)N, 0.0);
+  pbamd::check(pb_sys_set(B.handle(), PB_FIELD_SPH_NUM, zero.data()), "zero num");
+  B.importForceSelected(P, ExchangeMode::density, true);
+  ok = ok && B.getState(PB_FIELD_SPH_NUM) == num;
+  P.calcAcceleration();
+  P.getForceSelected(ExchangeMode::acceleration);
+  B.importForceSelected(P, ExchangeMode::acceleration, false);
+  const std::vector<double> pa = P.getState(PB_FIELD_A), ba = B.getState(PB_FIELD_A);
+  for (size_t k = 0; ok && k < pa.size(); ++k) ok = ba[k] == (double)(float)pa[k];
   std::cout << "done ok=" << (ok ? 1 : 0) << " N=" << N << " N1=" << N1 << " poserr=" << worst
             << " numerr=" << worst_num << std::endl;
   return ok ? 0 : 1;
