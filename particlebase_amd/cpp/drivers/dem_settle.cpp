@@ -60,6 +60,7 @@ int main(int argc, char **argv) {
   particles.setDEMProperties(2.11e9f, 0.5f, 0.29f, (real)gamma, 0.1f, r0);
   particles.setInertia(r0);
   particles.setupCutoffBlock((real)(2.0 * rmax), false);
+  particles.checkPidRange();       // testDEM.cu:93
   particles.calcBlockID();
 
   const real initDeltaT = 2e-6f, ulim = 0.01f * 4, llim = ulim / 16.0f;
@@ -89,6 +90,7 @@ int main(int argc, char **argv) {
   for (uint32_t i = 0; i < N; ++i) zmean += r[3 * i + 2];
   std::cout << "done N=" << N << " steps=" << j << " t=" << thistime() << " dt="
             << thistime.currentDeltaT() << " max_contacts=" << maxc << " zmean=" << zmean / N
-            << std::endl;
+            << " selected_cells=" << particles.numSelectedBlocks()
+            << " cells=" << particles.numBlocks() << std::endl;
   return 0;
 }

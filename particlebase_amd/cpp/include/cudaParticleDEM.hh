@@ -65,6 +65,8 @@ class cudaParticleDEM : public cudaParticleLF,
                         public putTMPselected {
 protected:
   uint32_t lastNNZ = 0;
+  long p1 = 0, p2 = 0;          //!< id range of this device (checkPidRange)
+  uint32_t p3 = 0, p4 = 0;      //!< moving-particle id range of this device
   // the arguments of setDEMProperties, kept for restarts
   double demE = 0, demMu = 0, demSigma = 0, demGamma = 0, demMuR = 0;
   std::vector<double> demR;
@@ -172,7 +174,39 @@ public:
     }
   }
 
+  // selectBlocks (cudaSelectedBlock.cu:18-44) marks the cells holding
+  // movable particles for the multi-GPU split; the contact search here
+  // skips fixed pairs by itself, so the call has nothing to prepare
   void selectBlocks(void) {}
+  // numSelectedBlocks (cudaSelectedBlock.hh:47): cells of the last
+  // calcBlockID that hold at least one movable particle
+  long numSelectedBlocks(void) {
+    std::vector<uint32_t> pid(N), bid(N), bindex(totalNumBlock + 1);
+    pbamd::check(pb_get_bins(sys, pid.data(), bid.data(), bindex.data()), "numSelectedBlocks");
+    std::vector<uint8_t> mv(N);
+    pbamd::check(pb_sys_get(sys, PB_FIELD_MOVE, mv.data()), "numSelectedBlocks");
+    long n = 0;
+    uint32_t last = UINT32_MAX;
+    for (uint32_t k = 0; k < N; ++k)      // bid is sorted: one count per cell
+      if (mv[pid[k]] && bid[k] != last) {
+        ++n;
+        last = bid[k];
+      }
+    return n;
+  }
+  // checkPidRange (cudaSelectedBlock.hh:50-61): the id range [p1, p2) and
+  // the moving-particle range [p3, p4) of device devId of ndev
+  void checkPidRange(int devId = 0, int ndev = 1, uint32_t Mstart = 0, uint32_t Mend = UINT32_MAX) {
+    if (ndev != 1 || devId != 0) {
+      std::cerr << "checkPidRange: one GPU owns every particle in this build" << std::endl;
+      std::exit(1);
+    }
+    p1 = 0;
+    p2 = N;
+    p3 = Mstart;
+    p4 = Mend == UINT32_MAX ? N : Mend;
+    std::cerr << "prange:" << p1 << ":" << p2 << ", " << p3 << ":" << p4 << std::endl;
+  }
   void calcBlockID(void) { cudaCutoffBlock::calcBlockID(); }
 
   // cudaParticleDEM::rollback / rollback2 (cudaParticleDEM.cu:302-309)

@@ -57,6 +57,10 @@ def test_dem_settle():
     d = run("dem_settle", 3000, 200)
     assert int(d["steps"]) == 200
     assert 0.0 < float(d["zmean"]) < 0.4
+    # numSelectedBlocks: every sphere moves, so the occupied cells, at
+    # least N / (spheres a cell can hold) of them and at most all cells
+    sel, cells = int(d["selected_cells"]), int(d["cells"])
+    assert 3000 // 8 <= sel <= cells
 
 
 def test_md_gauss():
