@@ -323,6 +323,8 @@ def main():
     f32 = s.get("F")
     frms = float(np.sqrt((f64 ** 2).sum(axis=1).mean()))
     ferr = float(np.abs(f32 - f64).max() / frms)
+    if a.profile:
+        return
     # the same run from host buffers to host buffers: import (host -> HBM,
     # cell order restored by the next rebuild) before and positions and
     # velocities back to the host after the timed steps
@@ -339,8 +341,6 @@ def main():
                "import_ms": 1e3 * t_in, "get_r_v_ms": 1e3 * t_out, "steps": a.steps}
     log(f"host-buffer inclusive: {host_io}")
     del s
-    if a.profile:
-        return
 
     # ---- plain FP64 form
     s = make_system(ctx, w)
