@@ -157,6 +157,8 @@ public:
   void getTypeID(void) {
     std::vector<uint16_t> t(N);
     pbamd::check(pb_sys_get(sys, PB_FIELD_TYPE, t.data()), "getTypeID");
+    std::lock_guard<std::mutex> lk(mutTMP);
+    if (TMP2.size() != N) TMP2.resize(N);   // a selected-range dump may have shrunk it
     for (size_t i = 0; i < N; ++i) TMP2[i] = t[i];
   }
   // FP64 copies of the state in the caller's particle order (no reference

@@ -125,6 +125,27 @@ int main(int argc, char **argv) {
   // positions print with the stream's 6 digits; n, rho, p with 9
   ok = ok && worst < 1e-5 && worst_num < 1e-7;
 
+  // back to whole-system dumps: getTypeID restores TMP2 to N particles and
+  // the n, rho, p block moves behind all N positions
+  P.getTypeID();
+  P.getPosition();
+  std::ostringstream o2;
+  P.putTMP(o2);
+  P.waitPutTMP();
+  {
+    std::istringstream in2(o2.str());
+    const Dump d2 = parse(in2);
+    ok = ok && d2.n == N && d2.rows.size() == N;
+    for (size_t i = 0; ok && i < d2.rows.size(); ++i) {
+      const std::vector<double> &v = d2.rows[i];
+      ok = v.size() == 8 && v[1] == (i < N1 ? 0.0 : 1.0);
+      for (int k = 0; ok && k < 3; ++k) {
+        const double want = (double)(float)num[3 * i + k];
+        ok = std::fabs(v[5 + k] - want) <= 1e-7 * std::fabs(want);
+      }
+    }
+  }
+
   // importForceSelected between two blocks (copies, no summation): the
   // density block from the device, then the acceleration from TMP
   CUDAenv<cudaParticleSPH_NS> env2;
