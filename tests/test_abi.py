@@ -47,3 +47,17 @@ def test_context_fails_loudly_without_device():
         pytest.skip("a GPU driver is present")
     with pytest.raises(pb.PBError):
         pb.Context(0)
+
+
+def test_null_handles_rejected():
+    # a null context or system is an argument error, returned before any
+    # device call (so this runs without a GPU)
+    import particlebase_amd as pb
+    lib = pb.load()
+    u64, u32 = ctypes.c_uint64(), ctypes.c_uint32()
+    assert lib.pb_ctx_limits(None, ctypes.byref(u64), ctypes.byref(u32), None, None, None) == pb.PB_ERR_ARG
+    assert lib.pb_ctx_info(None, None, 0, None) == pb.PB_ERR_ARG
+    assert lib.pb_ctx_sync(None) == pb.PB_ERR_ARG
+    buf = (ctypes.c_double * 3)()
+    assert lib.pb_sys_get(None, 0, buf) == pb.PB_ERR_ARG
+    assert lib.pb_sys_set(None, 0, buf) == pb.PB_ERR_ARG
