@@ -208,3 +208,15 @@ def test_argument_and_state_errors(ctx):
     pid, bid, bindex = s.get_bins()
     assert np.array_equal(np.sort(pid), np.arange(w.n))
     assert np.allclose(s.get("r"), w.r % w.L, rtol=0, atol=1e-12)
+
+
+def test_context_info_and_limits(ctx):
+    # pb_ctx_info / pb_ctx_limits: the CDNA4 device's figures (gfx950, 256
+    # compute units of 64 lanes, 160 KiB of LDS, HBM above 256 GB)
+    say("context info and limits")
+    name, cus = ctx.info()
+    lim = ctx.limits()
+    assert name.startswith("gfx950") and cus == 256
+    assert lim["lanes_per_cu"] == 64 and lim["lds_per_block"] == 160 * 1024
+    assert lim["max_threads"] == 1024 and lim["max_grid"] >= 2 ** 31 - 1
+    assert lim["global_mem"] > 256 * 10 ** 9
