@@ -60,7 +60,9 @@ typedef struct pb_sys pb_sys;
 /* --------------------------------------------------------------- context */
 /* Replaces CUDAenv<T>'s per-device setup (CUDAenv.hh:93-156) for one GPU. */
 int         pb_ctx_create(int device, pb_ctx **out);
-void        pb_ctx_destroy(pb_ctx *ctx);
+/* Releases the context.  Refused with PB_ERR_STATE (the context stays valid)
+ * while systems created on it are alive: destroy those first. */
+int         pb_ctx_destroy(pb_ctx *ctx);
 /* Replaces cudaFrame::ErrorInfo (cudaFrame.cu:20-32): message of the last
  * failing call on this context ("" if none). */
 const char *pb_ctx_last_error(const pb_ctx *ctx);
@@ -324,6 +326,15 @@ int pb_dem_get_contacts(pb_sys *sys, uint32_t *rowptr, uint32_t *colidx, double 
  * original indices, columns ascending, val[nnz]. */
 int pb_dem_set_contacts(pb_sys *sys, const uint32_t *rowptr, const uint32_t *colidx,
                         const double *val, uint32_t nnz);
+
+/* cudaParticleDEM::rollback's `_N -= 1` (cudaParticleDEM.cu:302-306): the
+ * contact matrix before the latest becomes the latest again, so the next
+ * pb_dem_calc_force (after the rollback) carries the tangential history of
+ * the step before the discarded one.  A latest matrix with no predecessor
+ * (the first call, or one restored by pb_dem_set_contacts) is dropped, and
+ * the next call starts from zero history.  One step is kept: a second rewind
+ * without a pb_dem_calc_force in between is PB_ERR_STATE. */
+int pb_dem_rewind(pb_sys *sys);
 
 /* Contacts of the last pb_dem_calc_force whose time step exceeded the
  * Cundall-Strack bound (counted even when autotune allowed the step). */

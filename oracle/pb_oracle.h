@@ -188,6 +188,12 @@ void pbo_wc_accel(const double *r, const double *v, const double *m, const doubl
 /* propagateLeapFrog_F4 (kernelfuncs.cu:270-296), FP64 */
 void pbo_lf_step(double *r, double *v, const double *a, const uint8_t *move, uint32_t n,
                  double dt);
+/* rollbackLeapFrog_F4 (kernelfuncs.cu:298-327), FP64.
+ * pinned by: two leap-frog steps under a constant acceleration followed by
+ * the rollback give back r(0) and the first half-step velocity
+ * (tests/test_oracle_dem.py). */
+void pbo_lf_rollback(double *r, double *v, const double *a, const uint8_t *move, uint32_t n,
+                     double dt);
 
 /* ------------------------------------------------------------------ DEM */
 /* cudaParticleDEM restated in FP64 (pb_oracle_dem.c).

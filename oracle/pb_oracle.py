@@ -77,6 +77,7 @@ def _extra(L):
         ("pbo_wc_density", None, [P(d), P(d), u32, d, d, d, d, d, P(d), P(d)]),
         ("pbo_wc_accel", None, [P(d), P(d), P(d), P(d), P(d), u32, d, d, d, P(d)]),
         ("pbo_lf_step", None, [P(d), P(d), P(d), P(u8), u32, d]),
+        ("pbo_lf_rollback", None, [P(d), P(d), P(d), P(u8), u32, d]),
         ("pbo_dem_contacts", u32, [P(d), P(d), P(u8), u32, P(u32), P(u32)]),
         ("pbo_dem_carry", None, [P(u32), P(u32), P(d), P(u32), P(u32), P(d), u32]),
         ("pbo_dem_force", u32, [P(d), P(d), P(d), P(d), P(d), P(u8), u32, P(u32), P(u32),
@@ -342,6 +343,14 @@ def lf_step(r, v, a, move, dt):
     mv = np.ascontiguousarray(move, dtype=np.uint8)
     d = C.c_double
     lib().pbo_lf_step(_p(r, d), _p(v, d), _p(a, d), _p(mv, C.c_uint8), r.shape[0], dt)
+    return r, v
+
+
+def lf_rollback(r, v, a, move, dt):
+    r, v, a = f64(r).copy(), f64(v).copy(), f64(a)
+    mv = np.ascontiguousarray(move, dtype=np.uint8)
+    d = C.c_double
+    lib().pbo_lf_rollback(_p(r, d), _p(v, d), _p(a, d), _p(mv, C.c_uint8), r.shape[0], dt)
     return r, v
 
 

@@ -111,6 +111,21 @@ void pbo_lf_step(double *r, double *v, const double *a, const uint8_t *move, uin
   }
 }
 
+/* rollbackLeapFrog_F4 (kernelfuncs.cu:298-327), FP64: with v = v(t+dt/2)
+ * and a = a(t), r(t+dt) -> r(t) = r - v dt, v -> v(t-dt/2) = v - a dt, then
+ * r(t) -> r(t-dt) = r(t) - v(t-dt/2) dt; fixed particles stay. */
+void pbo_lf_rollback(double *r, double *v, const double *a, const uint8_t *move, uint32_t n,
+                     double dt) {
+  for (uint32_t i = 0; i < n; ++i) {
+    const double d = dt * (double)move[i];
+    for (int c = 0; c < 3; ++c) {
+      const double rt = r[3 * i + c] - v[3 * i + c] * d;
+      v[3 * i + c] -= a[3 * i + c] * d;
+      r[3 * i + c] = rt - v[3 * i + c] * d;
+    }
+  }
+}
+
 /* ---------------------------------------------------------------------------
  * Config physics of the dam-break workload (weakly compressible SPH).  The
  * reference's kernels hold no such form; this follows the published method:

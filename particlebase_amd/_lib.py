@@ -51,7 +51,7 @@ EXPORTS = [
     "pb_dem_get_contacts", "pb_dem_last_violations",
     "pb_gt_step", "pb_gt_xi", "pb_adjust_velocities", "pb_const_temp",
     "pb_nlist_build", "pb_nlist_get", "pb_lj_force_nlist", "pb_softcore_force", "pb_euler_step",
-    "pb_dem_set_contacts", "pb_lj_set_tiles",
+    "pb_dem_set_contacts", "pb_lj_set_tiles", "pb_dem_rewind",
 ]
 
 
@@ -84,7 +84,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
     P = C.POINTER
     sig = {
         "pb_ctx_create": (i32, [i32, P(vp)]),
-        "pb_ctx_destroy": (None, [vp]),
+        "pb_ctx_destroy": (i32, [vp]),
         "pb_ctx_last_error": (C.c_char_p, [vp]),
         "pb_ctx_sync": (i32, [vp]),
         "pb_ctx_info": (i32, [vp, C.c_char_p, C.c_size_t, P(i32)]),
@@ -164,6 +164,7 @@ def _extra_signatures():
         "pb_dem_calc_force": (i32, [vp, d, i32, P(u32)]),
         "pb_dem_get_contacts": (i32, [vp, P(u32), P(u32), P(d), u32, P(u32)]),
         "pb_dem_last_violations": (i32, [vp, P(u32)]),
+        "pb_dem_rewind": (i32, [vp]),
         "pb_lj_autotune": (i32, [vp, P(pb_run_opts), u32, P(d), P(u32), P(d)]),
     }
 
@@ -228,7 +229,7 @@ class Context:
         if getattr(self, "h", None):
             for s in list(self.systems):
                 s.close()
-            self.lib.pb_ctx_destroy(self.h)
+            self.check(self.lib.pb_ctx_destroy(self.h))
             self.h = None
 
     def __del__(self):
@@ -566,6 +567,9 @@ class System:
         val = np.ascontiguousarray(val, dtype=np.float64)
         self._c(self.lib.pb_dem_set_contacts(self.h, _p(row, C.c_uint32), _p(col, C.c_uint32),
                                              _p(val, C.c_double), int(col.size)))
+
+    def dem_rewind(self):
+        self._c(self.lib.pb_dem_rewind(self.h))
 
     def dem_violations(self):
         z = C.c_uint32()

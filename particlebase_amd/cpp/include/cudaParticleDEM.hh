@@ -209,7 +209,16 @@ public:
   }
   void calcBlockID(void) { cudaCutoffBlock::calcBlockID(); }
 
-  // cudaParticleDEM::rollback / rollback2 (cudaParticleDEM.cu:302-309)
-  void rollback(real dt) { cudaParticleLF::rollback(dt); }
-  void rollback2(real dt) { cudaParticleLF::calcVinit(dt / 2); }
+  // cudaParticleDEM::rollback / rollback2 (cudaParticleDEM.cu:302-310):
+  // both undo the rotation step; rollback also rewinds the contact matrix so
+  // the next calcForce carries the history of the step before the discarded one
+  void rollback(real dt) {
+    cudaParticleLF::rollback(dt);
+    cudaParticleRotation::TimeEvolution(-dt);
+    pbamd::check(pb_dem_rewind(sys), "cudaParticleDEM::rollback");
+  }
+  void rollback2(real dt) {
+    cudaParticleLF::calcVinit(dt / 2);
+    cudaParticleRotation::TimeEvolution(-dt);
+  }
 };

@@ -39,6 +39,11 @@ protected:
   std::vector<double> wcsph;
 
 public:
+  // the dump thread calls this class's additionalTag / additionalOutput:
+  // join it while the object is still whole (the base destructors would
+  // otherwise run while a final dump is being written)
+  ~cudaParticleSPH_NS() override { waitPutTMP(); }
+
   void setup(int n) override {
     cudaParticleBase::setup(n);
     TMP.resize(8 * (size_t)n);       // x, y, z, _, num, rho, mu, _ (cudaParticleSPH_NS.cu:13-27)

@@ -317,6 +317,10 @@ extern "C" int pb_coord_counts(pb_sys *s, float rmax2t, uint32_t *counts) {
 // periodic box) is counted in registers and added once per tile pair, so
 // it is not a serialising LDS atomic hot spot.
 #define GR_T 256
+// A tile pair adds at most GR_T * GR_T = 2^16 pairs to one bin of the
+// workgroup's uint32 LDS histogram; flushing at least every 2^15 tile pairs
+// keeps every bin below 2^31, whatever the replica's size.
+#define GR_FLUSH_TILES 32768u
 struct GrRep {
   uint32_t atom0, natoms, ntile;
   uint64_t pair0;      // first tile-pair index of this replica
@@ -328,6 +332,7 @@ __global__ void __launch_bounds__(GR_T) k_gr(const double4 *r, const GrRep *reps
   extern __shared__ uint32_t h[];
   __shared__ float4 tj[GR_T];
   for (uint32_t b = threadIdx.x; b < rnum; b += GR_T) h[b] = 0;
+  uint32_t since = 0;                  // tile pairs in the LDS histogram since its last flush
   for (uint64_t p = blockIdx.x; p < npairs; p += gridDim.x) {
     uint32_t lo = 0, hi = nrep;
     while (hi - lo > 1) {
@@ -375,9 +380,10 @@ __global__ void __launch_bounds__(GR_T) k_gr(const double4 *r, const GrRep *reps
       if (pb_lane() == 0 && wz) atomicAdd(&h[0], wz);
       zero = 0;
     }
-    // flush when the replica changes or at the end (histograms are per replica)
+    // flush when the replica changes, at the end (histograms are per
+    // replica) and every GR_FLUSH_TILES tile pairs (no uint32 bin can wrap)
     const uint64_t pn = p + gridDim.x;
-    bool flush = pn >= npairs;
+    bool flush = pn >= npairs || ++since >= GR_FLUSH_TILES;
     if (!flush) {
       flush = !(pn >= R.pair0 && (lo + 1 >= nrep || pn < reps[lo + 1].pair0));
     }
@@ -388,6 +394,7 @@ __global__ void __launch_bounds__(GR_T) k_gr(const double4 *r, const GrRep *reps
         if (v) atomicAdd((unsigned long long *)&hist[(uint64_t)lo * rnum + b], (unsigned long long)v);
         h[b] = 0;
       }
+      since = 0;
     }
   }
 }

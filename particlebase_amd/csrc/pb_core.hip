@@ -49,14 +49,22 @@ extern "C" int pb_ctx_create(int device, pb_ctx **out) {
   return PB_OK;
 }
 
-extern "C" void pb_ctx_destroy(pb_ctx *ctx) {
-  if (!ctx) return;
+extern "C" int pb_ctx_destroy(pb_ctx *ctx) {
+  if (!ctx) return PB_ERR_ARG;
+  // a system keeps its context's device and stream: destroying the context
+  // first would leave every later call on the system (its destroy included)
+  // with a freed context
+  if (ctx->nsys)
+    return pb_fail(ctx, PB_ERR_STATE,
+                   "pb_ctx_destroy: " + std::to_string(ctx->nsys) +
+                       " system(s) still alive on this context; pb_sys_destroy them first");
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamDestroy(ctx->stream);
   }
   delete ctx;
+  return PB_OK;
 }
 
 extern "C" const char *pb_ctx_last_error(const pb_ctx *ctx) {
@@ -107,6 +115,7 @@ extern "C" int pb_sys_create(pb_ctx *ctx, uint32_t nrep, const uint32_t *counts,
   PB_HIP(ctx, hipSetDevice(ctx->device));
   pb_sys *s = new pb_sys();
   s->ctx = ctx;
+  ++ctx->nsys;
   s->nrep = nrep;
   s->n = (uint32_t)total;
   s->npad = (uint32_t)totpad;
@@ -194,6 +203,7 @@ extern "C" void pb_sys_destroy(pb_sys *s) {
   pb_free(s->gr_hist);
   pb_free(s->list_segs);
   if (s->hmon) (void)hipHostFree(s->hmon);
+  if (s->ctx) --s->ctx->nsys;
   delete s;
 }
 
@@ -412,6 +422,15 @@ extern "C" int pb_sys_set(pb_sys *s, int field, const void *host) {
         h[k].z = in[3 * (size_t)o[k] + 2];
       }
     PB_HIP(s->ctx, hipMemcpyAsync(dst, h.data(), sizeof(double4) * np, hipMemcpyHostToDevice, st));
+    if (field == PB_FIELD_SPH_NUM && s->sph_rhoinv) {
+      // 1/rho follows the imported rho (calcDensityPost's reciprocal,
+      // cudaParticleSPH_NS.cu:190-195), so an acceleration after a density
+      // import uses the imported densities throughout
+      std::vector<double> hr(np, 0.0);
+      for (size_t k = 0; k < np; ++k)
+        if (o[k] != PB_NONE) hr[k] = 1.0 / in[3 * (size_t)o[k] + 1];
+      PB_HIP(s->ctx, hipMemcpyAsync(s->sph_rhoinv, hr.data(), 8 * np, hipMemcpyHostToDevice, st));
+    }
     PB_HIP(s->ctx, hipStreamSynchronize(st));
     return PB_OK;
   }

@@ -468,6 +468,8 @@ extern "C" int pb_dem_calc_force(pb_sys *s, double dt, int autotune, uint32_t *n
     PB_HIP(s->ctx, hipStreamSynchronize(st));
   }
   s->cm_nnz[b] = nnz;
+  s->cm_prev = p >= 0;
+  s->cm_rewound = false;
   s->cm_cur = b;
   s->dem_bad = bad;
   if (nnz_out) *nnz_out = nnz;
@@ -523,6 +525,28 @@ extern "C" int pb_dem_set_contacts(pb_sys *s, const uint32_t *rowptr, const uint
   PB_HIP(s->ctx, hipStreamSynchronize(st));
   s->cm_nnz[b] = nnz;
   s->cm_cur = b;
+  s->cm_prev = false;                        // the other buffer is not its predecessor
+  s->cm_rewound = false;
+  return PB_OK;
+}
+
+// cudaParticleDEM::rollback's `_N -= 1` (cudaParticleDEM.cu:302-306): the
+// matrix before the latest becomes the latest again, so the next
+// pb_dem_calc_force overwrites the discarded one and carries the history
+// of the step before it.  When the latest matrix has no predecessor (the
+// first call, or a matrix restored by pb_dem_set_contacts) the next call
+// starts every contact at zero history, as the reference's cleared second
+// matrix gives.  Only one earlier step is kept, so a second rewind without a
+// pb_dem_calc_force in between is refused.
+extern "C" int pb_dem_rewind(pb_sys *s) {
+  if (!s) return PB_ERR_ARG;
+  if (s->cm_rewound)
+    return pb_fail(s->ctx, PB_ERR_STATE, "pb_dem_rewind: already rewound (one step is kept)");
+  if (s->cm_cur < 0)
+    return pb_fail(s->ctx, PB_ERR_STATE, "pb_dem_rewind: no contact matrix to rewind");
+  s->cm_cur = s->cm_prev ? (s->cm_cur ^ 1) : -1;
+  s->cm_prev = false;
+  s->cm_rewound = true;
   return PB_OK;
 }
 

@@ -51,6 +51,7 @@ struct pb_ctx {
   std::string name;
   uint64_t global_mem = 0;
   uint32_t max_threads = 0, max_grid = 0, lds_per_block = 0;
+  uint32_t nsys = 0;   // live systems created on this context (destroy refuses while > 0)
 };
 
 // device buffer helper
@@ -163,6 +164,8 @@ struct pb_sys {
   double *cm_val[2] = {nullptr, nullptr};
   uint32_t cm_cap[2] = {0, 0}, cm_nnz[2] = {0, 0};
   int cm_cur = -1;                             // buffer of the latest CSR, -1 before any
+  bool cm_prev = false;                        // the other buffer holds the matrix before it
+  bool cm_rewound = false;                     // the last matrix operation was pb_dem_rewind
   uint32_t *dem_inv = nullptr;                 // original index -> storage slot
   double4 *dem_rs = nullptr;                   // cell order: (x, y, z, radius)
   uint32_t *dem_so = nullptr;                  // cell order: original index

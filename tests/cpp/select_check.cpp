@@ -151,23 +151,39 @@ int main(int argc, char **argv) {
   CUDAenv<cudaParticleSPH_NS> env2;
   env2.setup();
   cudaParticleSPH_NS &B = env2[0];
+  // B's own densities come from a compressed copy of the particles (every
+  // coordinate scaled by 0.97), then B takes P's positions back, so only the
+  // imported densities give P's acceleration
+  std::vector<ParticleBase> Gc(G);
+  for (auto &p : Gc)
+    for (int c = 0; c < 3; ++c) p.r[c] *= 0.97;
   B.setup(N);
   B.setCell(cell);
-  B.import(G);
+  B.import(Gc);
   B.setWCSPHProperties(h, 1000.0, 10.0, 7.0, 0.1, 0.0);
   B.setupCutoffBlock((real)(2 * h), false);
   B.calcBlockID();
   B.calcDensity();
-  std::vector<double> zero(3 * (size_t)N, 0.0);
-  pbamd::check(pb_sys_set(B.handle(), PB_FIELD_SPH_NUM, zero.data()), "zero num");
+  const std::vector<double> rP = P.getState(PB_FIELD_R);
+  pbamd::check(pb_sys_set(B.handle(), PB_FIELD_R, rP.data()), "positions of P");
+  B.calcBlockID();
+  std::vector<double> zero(3 * (size_t)N, 1.0);
+  pbamd::check(pb_sys_set(B.handle(), PB_FIELD_SPH_NUM, zero.data()), "reset num");
   B.importForceSelected(P, ExchangeMode::density, true);
   ok = ok && B.getState(PB_FIELD_SPH_NUM) == num;
+  // the sequence of testSPH5.cu:208-220: density import, calcDensityPost,
+  // calcAcceleration -- equal to P's acceleration from its own densities
+  B.calcDensityPost();
+  B.calcAcceleration();
   P.calcAcceleration();
+  const bool acc_from_import = B.getState(PB_FIELD_A) == P.getState(PB_FIELD_A);
+  ok = ok && acc_from_import;
   P.getForceSelected(ExchangeMode::acceleration);
   B.importForceSelected(P, ExchangeMode::acceleration, false);
   const std::vector<double> pa = P.getState(PB_FIELD_A), ba = B.getState(PB_FIELD_A);
   for (size_t k = 0; ok && k < pa.size(); ++k) ok = ba[k] == (double)(float)pa[k];
   std::cout << "done ok=" << (ok ? 1 : 0) << " N=" << N << " N1=" << N1 << " poserr=" << worst
-            << " numerr=" << worst_num << std::endl;
+            << " numerr=" << worst_num << " acc_from_import=" << (acc_from_import ? 1 : 0)
+            << std::endl;
   return ok ? 0 : 1;
 }

@@ -61,3 +61,16 @@ def test_null_handles_rejected():
     buf = (ctypes.c_double * 3)()
     assert lib.pb_sys_get(None, 0, buf) == pb.PB_ERR_ARG
     assert lib.pb_sys_set(None, 0, buf) == pb.PB_ERR_ARG
+    assert lib.pb_ctx_destroy(None) == pb.PB_ERR_ARG
+    assert lib.pb_dem_rewind(None) == pb.PB_ERR_ARG
+
+
+def test_gr_histogram_flush_bound():
+    # k_gr counts pairs in a uint32 LDS histogram: a tile pair adds at most
+    # GR_T^2 pairs to a bin and the histogram is flushed at least every
+    # GR_FLUSH_TILES tile pairs, which must keep every bin below 2^32
+    src = open(os.path.join(ROOT, "particlebase_amd", "csrc", "pb_bin.hip")).read()
+    gt = int(re.search(r"#define GR_T (\d+)", src).group(1))
+    ft = int(re.search(r"#define GR_FLUSH_TILES (\d+)u", src).group(1))
+    assert gt * gt * ft < 2 ** 32
+    assert "++since >= GR_FLUSH_TILES" in src

@@ -220,3 +220,46 @@ def test_context_info_and_limits(ctx):
     assert lim["lanes_per_cu"] == 64 and lim["lds_per_block"] == 160 * 1024
     assert lim["max_threads"] == 1024 and lim["max_grid"] >= 2 ** 31 - 1
     assert lim["global_mem"] > 256 * 10 ** 9
+
+
+def test_scale_temp_closed_form(ctx, orc):
+    # scaleTemp (cudaParticleMD.cu:168-175): s = sqrt((3N-1)/(3N) T / T0) with
+    # T0 = sum m v^2 / (3 N kB), computed in FP32 as the class computes it;
+    # v is multiplied by s (mulArray), so afterwards sum m v^2 / (3N kB) is
+    # (3N-1)/(3N) T to the rounding of s
+    say("scaleTemp: velocity scaling to (3N-1)/(3N) T")
+    w = workloads.lj_fcc(6, 0.8442, 1.3, seed=21)
+    n = w.n
+    s = pb.System(ctx, [n])
+    s.set_cell(w.cell)
+    s.import_particles(w.r, w.m, v=w.v, type=w.type)
+    minv = 1.0 / w.m
+    mv2 = s.mv2()[0]
+    assert mv2 == pytest.approx(orc.mv2(w.v, minv), rel=1e-14)
+    f = np.float32
+    T0 = f(mv2) / (f(n) * f(3) * f(1.0))
+    target = f(0.72)
+    sc = np.sqrt(f(3 * n - 1) / f(3 * n) * target / T0, dtype=np.float32)
+    s.scale_velocities(float(sc))
+    v1 = s.get("v")
+    assert np.array_equal(v1, w.v * float(sc))
+    T1 = s.mv2()[0] / (3 * n)
+    assert T1 == pytest.approx((3 * n - 1) / (3 * n) * 0.72, rel=2e-6)
+
+
+def test_context_destroy_order(ctx):
+    # a context outlives its systems: pb_ctx_destroy with a live system is
+    # refused (PB_ERR_STATE) and leaves both usable; after pb_sys_destroy
+    # the context goes
+    say("pb_ctx_destroy refused while a system is alive")
+    c2 = pb.Context(0)
+    lib = c2.lib
+    s = pb.System(c2, [100])
+    s.set_cell([0, 5, 0, 5, 0, 5])
+    assert lib.pb_ctx_destroy(c2.h) == pb.PB_ERR_STATE
+    assert "alive" in lib.pb_ctx_last_error(c2.h).decode()
+    s.import_particles(np.full((100, 3), 1.0), np.ones(100))
+    assert np.all(s.get("r") == 1.0)
+    s.close()
+    assert lib.pb_ctx_destroy(c2.h) == pb.PB_OK
+    c2.h = None

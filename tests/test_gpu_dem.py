@@ -6,6 +6,9 @@ three calls, the reference driver loop of testDEM5.cu:181-234 (bin, force,
 acceleration, gravity, leap-frog, rotation, reflecting walls) over several
 steps, and the Cundall-Strack error.  FP64 on both sides: forces to 1e-11
 of max|F|."""
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
@@ -194,3 +197,127 @@ def test_long_rows(ctx, orc):
     assert (lens > 16).sum() > 100 and (lens[lens > 0] <= 16).sum() > 0
     assert nnz == len(ocol)
     assert np.array_equal(row, orow) and np.array_equal(col, ocol)
+
+
+BIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                   "particlebase_amd", "bin")
+
+
+def _read_iterations(path, n, niter):
+    out = []
+    with open(path, "rb") as f:
+        for _ in range(niter):
+            st = [np.frombuffer(f.read(24 * n), dtype=np.float64).reshape(n, 3) for _ in range(4)]
+            nnz = int(np.frombuffer(f.read(4), dtype=np.uint32)[0])
+            if nnz == 0xFFFFFFFF:
+                out.append((st, None))
+                continue
+            row = np.frombuffer(f.read(4 * (n + 1)), dtype=np.uint32)
+            col = np.frombuffer(f.read(4 * nnz), dtype=np.uint32)
+            val = np.frombuffer(f.read(8 * nnz), dtype=np.float64)
+            out.append((st, (row, col, val)))
+    return out
+
+
+@pytest.mark.parametrize("script", [[0, 0, 0, 1, 0, 0, 0, 0], [1, 0, 0, 0, 0]],
+                         ids=["rollback_at_step3", "rollback_of_first_matrix"])
+def test_rollback_through_adaptive_time(orc, tmp_path, script):
+    """cudaParticleDEM::rollback / rollback2 (cudaParticleDEM.cu:302-310) as
+    AdaptiveTime::Progress calls them (AdaptiveTime.hh:33-86): leap-frog
+    rollback, rotation step of -dt and the contact matrix rewound (the next
+    calcForce carries the history of the step before the discarded one),
+    then rollback2 and dt / 4.  The class layer runs the loop on the GPU
+    (tests/cpp/dem_rollback_check.cpp); the oracle runs the same sequence."""
+    say(f"DEM rollback through AdaptiveTime, script {script}")
+    r, rad, fixed, m, v, cell = lattice(7, 0.98, 11)
+    n, niter = r.shape[0], len(script)
+    f32 = lambda x: float(np.float32(x))
+    dt0, g = f32(2e-4), f32(9.8)
+    E_, MU_, SIG_, G2_, MUR_ = map(f32, (E, MU, SIG, G2, MU_R))
+    rad32 = rad.astype(np.float32)
+    rcell = f32(2 * rad32.max())
+    inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(inp, "wb") as f:
+        f.write(np.array([n, niter], dtype=np.uint32).tobytes())
+        for x in (r, v, m):
+            f.write(np.ascontiguousarray(x, dtype=np.float64).tobytes())
+        f.write(rad32.tobytes())
+        f.write(fixed.astype(np.uint8).tobytes())
+        f.write(np.array(script, dtype=np.int32).tobytes())
+        f.write(np.array([dt0, g, *cell, E_, MU_, SIG_, G2_, MUR_, rcell],
+                         dtype=np.float32).tobytes())
+    exe = os.path.join(BIN, "dem_rollback_check")
+    p = subprocess.run([exe, str(inp), str(outp)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr[-2000:]
+    assert "rollbacks=1" in p.stdout, p.stdout
+    gpu = _read_iterations(outp, n, niter)
+
+    # the same loop on the oracle, in FP64 from the float-rounded inputs
+    radd = rad32.astype(np.float64)
+    e_eff = E_ * (1 - SIG_ ** 2)
+    s2 = (2 - SIG_) / 2 / (1 - SIG_)
+    move = (1 - fixed).astype(np.uint8)
+    minv = 1 / m
+    m0inv = minv * (1.0 / (0.4 * radd * radd))     # setInertia's arithmetic
+    lo, hi = cell[0::2], cell[1::2]
+    x, u = r.copy(), v.copy()
+    om, L = np.zeros((n, 3)), np.zeros((n, 3))
+    mats = []
+    dt, rolling = dt0, False
+    carried = 0
+    for it, res in enumerate(script):
+        row, col = orc.dem_contacts(x, radd, move)
+        if mats:
+            cval = orc.dem_carry(*mats[-1], row, col)
+            carried += np.count_nonzero(cval)
+        else:
+            cval = np.zeros(len(col))
+        F, T, val, _ = orc.dem_force(x, u, om, radd, minv, move, row, col, cval, e_eff, MU_, s2,
+                                     G2_, MUR_, dt)
+        mats.append((row, col, val))
+        a = F * minv[:, None]
+        a[:, 2] -= g
+        x, u = orc.lf_step(x, u, a, move, dt)
+        om, L = orc.rot_step(om, L, T, m0inv, move, dt)
+        if not rolling:
+            if res == 1:      # rollback(dt): LF rollback, rotation -dt, matrix rewound
+                x, u = orc.lf_rollback(x, u, a, move, dt)
+                om, L = orc.rot_step(om, L, T, m0inv, move, -dt)
+                mats.pop()
+                rolling = True
+        else:                 # rollback2(dt): F = 0, v -= 0, rotation -dt; dt / 4
+            om, L = orc.rot_step(om, L, T, m0inv, move, -dt)
+            rolling = False
+            dt = f32(dt / 4.0)
+        u = np.where(x < lo, np.abs(u), u)
+        u = np.where(x > hi, -np.abs(u), u)
+        (gr, gv, gw, gL), csr = gpu[it]
+        rel = lambda a, b: np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+        errs = {"r": rel(gr, x), "v": rel(gv, u), "w": rel(gw, om), "L": rel(gL, L)}
+        assert max(errs.values()) <= 1e-11, (it, errs)
+        if not mats:
+            assert csr is None, it
+            continue
+        orow, ocol, oval = mats[-1]
+        assert csr is not None, it
+        assert np.array_equal(csr[0], orow) and np.array_equal(csr[1], ocol), it
+        assert rel(csr[2], oval) <= 1e-11, (it, rel(csr[2], oval))
+    assert carried > 100      # persistent contacts carried a tangential history
+
+
+def test_rewind_errors(ctx):
+    say("pb_dem_rewind: before any contact matrix and twice in a row")
+    r, rad, fixed, m, v, w, cell = packing(300, 12)
+    s = make(ctx, r, rad, fixed, m, v, cell, 2 * rad.max())
+    with pytest.raises(pb.PBError) as e:
+        s.dem_rewind()
+    assert e.value.code == pb._lib.PB_ERR_STATE
+    s.bin()
+    s.dem_calc_force(1e-4)
+    s.dem_calc_force(1e-4)
+    s.dem_rewind()
+    with pytest.raises(pb.PBError) as e:
+        s.dem_rewind()
+    assert e.value.code == pb._lib.PB_ERR_STATE
+    s.dem_calc_force(1e-4)
+    s.dem_rewind()

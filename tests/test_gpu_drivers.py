@@ -119,3 +119,6 @@ def test_put_tmp_selected(tmp_path):
     d = run("select_check", str(tmp_path))
     assert int(d["ok"]) == 1 and int(d["N1"]) == 12 * 12 * 8
     assert float(d["poserr"]) < 1e-5 and float(d["numerr"]) < 1e-7
+    # calcDensityPost + calcAcceleration after a density import use the
+    # imported rho and its reciprocal (testSPH5.cu:208-220)
+    assert int(d["acc_from_import"]) == 1

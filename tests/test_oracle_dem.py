@@ -172,3 +172,25 @@ def test_rotation_update(orc):
     assert np.allclose(L, T * dt * mv, rtol=1e-15)
     # omega = L / I with the solid-sphere inertia I = 2/5 m r^2
     assert np.allclose(w, L / (0.4 * m * rad ** 2)[:, None] * mv, rtol=1e-14)
+
+
+def test_lf_rollback_inverts_two_steps(orc):
+    # under a constant acceleration two leap-frog steps from (r0, v0) and the
+    # rollback give r0 back and v1 = v0 + a dt (the velocity after the first
+    # step); fixed particles never move
+    rng = np.random.default_rng(7)
+    n, dt = 200, 0.01
+    r0 = rng.uniform(0, 5, (n, 3))
+    v0 = rng.normal(size=(n, 3))
+    a = rng.normal(size=(n, 3))
+    move = (rng.uniform(size=n) > 0.1).astype(np.uint8)
+    r, v = orc.lf_step(r0, v0, a, move, dt)
+    r, v = orc.lf_step(r, v, a, move, dt)
+    rb, vb = orc.lf_rollback(r, v, a, move, dt)
+    m = move[:, None].astype(float)
+    assert np.abs(rb - r0).max() < 1e-13
+    assert np.abs(vb - (v0 + a * dt * m)).max() < 1e-13
+    assert np.array_equal(rb[move == 0], r0[move == 0])
+    # a rollback is not a single backward step: it moves r by two steps
+    r1, _ = orc.lf_step(r0, v0, a, move, dt)
+    assert np.abs(rb - r1)[move == 1].max() > 1e-4
