@@ -187,51 +187,56 @@ def bench_sph(ctx, a, config=False):
             "steps": a.sph_steps}
 
 
-def bench_dem(ctx, a):
-    """DEM settle (Hertz contacts with rotation), the reference driver's
-    step: bin, contacts + force, acceleration, gravity, leap-frog, rotation,
-    inspectVelocity, reflecting walls."""
-    # a packed bed (the lowest sites of a lattice of pitch 2.1 mm < 2 r_max)
-    # so that contacts are present from the first step, as in the settled bed
-    w = workloads.dem_settle(20000 if a.quick else 200_000, gap=-0.045, fill="bottom")
-    s = pb.System(ctx, [w.n])
-    s.set_cell(w.cell)
-    s.import_particles(w.r, w.m, is_fixed=w.fixed)
-    gamma = -math.log(0.8) / math.sqrt(math.pi ** 2 + math.log(0.8) ** 2) * 2
-    # Y = 2e8 Pa: the Hertz stiffness 2/3 Y sqrt(R xi) is ~1e4 N/m at 10 um
-    # overlap, the config's linear spring constant
-    s.dem_set_properties(2.0e8, 0.5, 0.29, gamma, 0.1, w.rad)
-    s.rot_set_inertia(w.rad)
-    s.setup_cells(2 * w.rad.max(), periodic=False)
-    dt = 2e-6
-    nnz = [0]
+DEM_ALG_BYTES = 156.0        # per particle-step: FP64 r, v, w/L read + written, radius, mass
+DEM_ALG_BYTES_CONTACT = 52.0  # per contact: column index + 3-double history in and out
 
-    def step():
-        s.bin()
-        nnz[0] = s.dem_calc_force(dt, autotune=True)
-        s.calc_acceleration()
-        s.add_acceleration(2, -9.8)
-        s.lf_step(dt)
-        s.rot_step(dt)
-        s.inspect_velocity(2 * 0.9e-3 / dt, 0.04, 0.0025)
-        s.treat_reflect()
 
-    for _ in range(3):
-        step()
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(a.dem_steps):
-        step()
-    ctx.sync()
-    el = time.perf_counter() - t0
-    v = w.n * a.dem_steps / el
-    log(f"dem: N={w.n}, {v:.4g} particle-steps/s, {1e3 * el / a.dem_steps:.3f} ms/step, "
-        f"nnz={nnz[0]}")
-    return {"workload": f"DEM settle, {w.n} spheres r in [0.9, 1.1] mm in 0.1x0.1x0.4 m from a "
-                        f"packed bed (lattice pitch 2.1 mm), Hertz + Coulomb + rotation, dt 2e-6, "
-                        f"FP64",
-            "value": v, "unit": "particle-steps/s",
-            "ms_per_step": 1e3 * el / a.dem_steps, "steps": a.dem_steps, "contacts": nnz[0]}
+def bench_dem(a, law="linear"):
+    """DEM settle as the workload states it: 200,000 spheres, radii U[0.9,
+    1.1] mm, random non-overlapping insertion in 0.1 x 0.1 x 0.4 m, at rest;
+    linear spring-dashpot contacts (kn 1e4 N/m, restitution 0.8) with a
+    tangential spring (kt = 2/7 kn) and Coulomb friction mu 0.5, rotation,
+    gravity, reflecting walls, adaptive time step (AdaptiveTime with
+    rollback; under the linear law held at 0.2 sqrt(meff_min / kn)), 5000
+    steps.  Run by the class-layer driver in the reference driver's call
+    order (testDEM5.cu:181-234) from the workload written to a file; the
+    driver times its step loop.  law="hertz": the reference's Hertz law on
+    the same spheres."""
+    import subprocess
+    import tempfile
+    w = workloads.dem_settle_config(20000 if a.quick else 200_000)
+    exe = os.path.join(ROOT, "particlebase_amd", "bin", "dem_settle")
+    with tempfile.NamedTemporaryFile(suffix=".bin", delete=False) as f:
+        f.write(np.array([w.n], dtype=np.uint32).tobytes())
+        for x in (w.r, w.rad, w.m, np.zeros_like(w.r)):
+            f.write(np.ascontiguousarray(x, dtype=np.float64).tobytes())
+        path = f.name
+    try:
+        p = subprocess.run([exe, str(w.n), str(a.dem_steps), law, path], capture_output=True,
+                           text=True, timeout=600)
+    finally:
+        os.unlink(path)
+    if p.returncode != 0:
+        raise RuntimeError(f"dem_settle failed: {p.stderr[-2000:]}")
+    line = [x for x in p.stdout.splitlines() if x.startswith("done")][-1]
+    d = {k: float(v) for k, v in (t.split("=") for t in line.split()[1:])}
+    iters = d["iterations"]
+    v = d["particle_steps_per_s"]
+    alg = (DEM_ALG_BYTES * w.n + DEM_ALG_BYTES_CONTACT * d["mean_contacts"]) * iters / d["time_s"]
+    log(f"dem ({law}): N={w.n}, {v:.4g} particle-steps/s, {1e3 * d['time_s'] / iters:.3f} ms/step,"
+        f" {line}")
+    lawtxt = ("linear spring-dashpot kn 1e4 N/m e 0.8, tangential spring + Coulomb mu 0.5"
+              if law == "linear" else "reference Hertz law (E 2.11e9 Pa, sigma 0.29, e 0.8)")
+    return {"workload": f"DEM settle, {w.n} spheres r in [0.9, 1.1] mm, random non-overlapping "
+                        f"insertion in 0.1x0.1x0.4 m at rest, {lawtxt}, rotation, gravity, "
+                        f"reflecting walls, adaptive dt, FP64",
+            "value": v, "unit": "particle-steps/s", "ms_per_step": 1e3 * d["time_s"] / iters,
+            "steps": int(d["steps"]), "iterations": int(iters), "rollbacks": int(d["rollbacks"]),
+            "final_dt": d["dt"], "sim_time_s": d["t"], "mean_contacts": d["mean_contacts"],
+            "max_contacts": int(d["max_contacts"]),
+            "roofline_frac": alg / 1e9 / HBM_PEAK_GBS,
+            "alg_bytes": f"{DEM_ALG_BYTES:g} B per particle-step + "
+                         f"{DEM_ALG_BYTES_CONTACT:g} B per contact"}
 
 
 def load_traffic():
@@ -262,7 +267,7 @@ def main():
     ap.add_argument("--no-drift", action="store_true",
                     help="skip the 10^4-step N=4000 energy-drift runs")
     ap.add_argument("--sph-steps", type=int, default=50)
-    ap.add_argument("--dem-steps", type=int, default=50)
+    ap.add_argument("--dem-steps", type=int, default=5000)
     ap.add_argument("--quick", action="store_true", help="small sizes (debug)")
     ap.add_argument("--profile", action="store_true",
                     help="only the mixed 1M run (for rocprofv3 kernel traces / counters)")
@@ -416,7 +421,8 @@ def main():
         out["sph"] = bench_sph(ctx, a)
         out["sph_config"] = bench_sph(ctx, a, config=True)
     if not a.no_dem:
-        out["dem"] = bench_dem(ctx, a)
+        out["dem"] = bench_dem(a)
+        out["dem_hertz"] = bench_dem(a, law="hertz")
 
     if not a.no_cpu:
         log("cpu baseline ...")

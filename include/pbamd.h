@@ -294,6 +294,27 @@ int pb_sph_inspect_density(pb_sys *sys, double *mean_num);
 int pb_dem_set_properties(pb_sys *sys, double E, double mu, double sigma, double gamma,
                           double mu_r, const double *r0, double r0_all);
 
+/* Linear spring-dashpot contact law with a tangential spring and Coulomb
+ * friction (Cundall & Strack 1979) behind the same DEM calls.  No
+ * counterpart in the reference, which has only the Hertz law of
+ * setDEMProperties (cudaParticleDEM.cu:221-252, kernelinline.hh:235-363):
+ * it is the granular settle workload's law, set up in place of
+ * setDEMProperties.  Normal force (kn delta - 2 zeta sqrt(meff kn) v_n) n
+ * with zeta the damping ratio of restitution e (0 < e <= 1), a fixed
+ * partner acting as an infinite mass; tangential spring kt on the
+ * accumulated tangential displacement of the contact points (kept in the
+ * tangent plane), capped at mu |Fn|; torque (-r_i n) x Ft.  Radius r0[n]
+ * (original order) or r0_all.  The contact history becomes 3 doubles per
+ * contact (pb_dem_history_width); switching the law drops the contact
+ * matrices. */
+int pb_dem_set_linear(pb_sys *sys, double kn, double e, double kt, double mu,
+                      const double *r0, double r0_all);
+
+/* History doubles per contact of the current law: 1 (the reference's Hertz
+ * slip length, cudaSparseMat val) or 3 (the linear law's spring vector).
+ * No reference counterpart (the reference has one law). */
+int pb_dem_history_width(pb_sys *sys, uint32_t *hw);
+
 /* cudaParticleRotation::setInertia, both overloads (cudaParticleRotation.cu:
  * 35-61): w = L = 0, m0inv = minv / (0.4 r0^2). */
 int pb_rot_set_inertia(pb_sys *sys, const double *r0, double r0_all);
@@ -307,7 +328,8 @@ int pb_rot_step(pb_sys *sys, double dt);
  * of the movable particles (|r_ij| < r_i + r_j, rows and columns by
  * original index, columns ascending), tangential history carried over from
  * the previous call's matrix, Hertz contact force and torque into F and T
- * (overwritten).  Needs pb_bin (calcBlockID) on the current positions with
+ * (overwritten; the linear law of pb_dem_set_linear when that was the
+ * last set-up).  Needs pb_bin (calcBlockID) on the current positions with
  * cells from pb_setup_cells(sys, rmax, 0, single).  autotune = 0 makes a
  * step above the Cundall-Strack bound an error (PB_ERR_DT; the reference
  * asserts unless autotunetimestep).  nnz_out (may be NULL) gets nnz. */
@@ -315,7 +337,8 @@ int pb_dem_calc_force(pb_sys *sys, double dt, int autotune, uint32_t *nnz_out);
 
 /* Download of the latest contact matrix (cudaSparseMat rowPtr / colIdx /
  * val, cudaSparseMat.hh): rowptr[n+1]; colidx and val need cap >= nnz
- * (either may be NULL). */
+ * (either may be NULL); val holds pb_dem_history_width doubles per
+ * contact. */
 int pb_dem_get_contacts(pb_sys *sys, uint32_t *rowptr, uint32_t *colidx, double *val,
                         uint32_t cap, uint32_t *nnz);
 
@@ -323,7 +346,7 @@ int pb_dem_get_contacts(pb_sys *sys, uint32_t *rowptr, uint32_t *colidx, double 
  * fields of cudaParticleDEM's archive, cudaParticleDEM.hh:124-180,
  * cudaSparseMat.hh:70-122) as the latest one, so the next
  * pb_dem_calc_force carries its tangential history.  rowptr[n+1] over
- * original indices, columns ascending, val[nnz]. */
+ * original indices, columns ascending, val[nnz * history width]. */
 int pb_dem_set_contacts(pb_sys *sys, const uint32_t *rowptr, const uint32_t *colidx,
                         const double *val, uint32_t nnz);
 

@@ -211,6 +211,32 @@ uint32_t pbo_dem_force(const double *r, const double *v, const double *w, const 
                        double *T);
 void pbo_rot_step(double *w, double *L, const double *T, const double *m0inv,
                   const uint8_t *move, uint32_t n, double dt);
+/* Linear spring-dashpot + tangential spring + Coulomb law (published
+ * method, no reference counterpart; hist = 3 doubles per contact).
+ * pinned by: head-on collision time pi/omega_d and restitution e from a
+ * two-sphere integration, e = 1 conserving energy, the static spring
+ * force kn delta, stick (Ft = -kt sum vt dt) and slip (|Ft| = mu |Fn|,
+ * opposite vt) regimes, torque = (-rad_i n) x Ft, Newton's third law,
+ * the tangent-plane projection of the history, a fixed sphere acting as an
+ * infinite mass (tests/test_oracle_dem.py). */
+uint32_t pbo_dem_force_lsd(const double *r, const double *v, const double *w, const double *rad,
+                           const double *minv, const uint8_t *move, uint32_t n,
+                           const uint32_t *rowptr, const uint32_t *colidx, double *hist,
+                           double kn, double zeta, double kt, double mu, double dt, double *F,
+                           double *T);
+/* pbo_dem_contacts over a cell grid for large N (same test, same CSR).
+ * pinned by: equality with the brute-force pbo_dem_contacts on random
+ * packings, cells at the box faces and centres outside the box included
+ * (tests/test_oracle_dem.py). */
+uint32_t pbo_dem_contacts_cells(const double *r, const double *rad, const uint8_t *move,
+                                uint32_t n, const double cell[6], double cellsize,
+                                uint32_t *rowptr, uint32_t *colidx);
+/* pbo_dem_carry with hw doubles of history per contact.
+ * pinned by: equality with pbo_dem_carry at hw = 1 and per-component
+ * scripted sequences at hw = 3 (tests/test_oracle_dem.py). */
+void pbo_dem_carry_w(const uint32_t *prow, const uint32_t *pcol, const double *pval,
+                     const uint32_t *crow, const uint32_t *ccol, double *cval, uint32_t n,
+                     uint32_t hw);
 
 /* ------------------------------------------- MD neighbour list, annealing */
 /* makeNList (cudaParticleMD.cu:338-394): CSR list by original index over the

@@ -83,6 +83,10 @@ def _extra(L):
         ("pbo_dem_force", u32, [P(d), P(d), P(d), P(d), P(d), P(u8), u32, P(u32), P(u32),
                                 P(d), d, d, d, d, d, d, P(d), P(d)]),
         ("pbo_rot_step", None, [P(d), P(d), P(d), P(d), P(u8), u32, d]),
+        ("pbo_dem_force_lsd", u32, [P(d), P(d), P(d), P(d), P(d), P(u8), u32, P(u32), P(u32),
+                                    P(d), d, d, d, d, d, P(d), P(d)]),
+        ("pbo_dem_contacts_cells", u32, [P(d), P(d), P(u8), u32, P(d), d, P(u32), P(u32)]),
+        ("pbo_dem_carry_w", None, [P(u32), P(u32), P(d), P(u32), P(u32), P(d), u32, u32]),
         ("pbo_gt_step", C.c_int, [P(d), P(d), P(d), P(d), P(d), u32, d, P(d), d]),
         ("pbo_adjust_velocities", None, [P(d), u32, d, P(d)]),
         ("pbo_const_temp", d, [P(d), P(d), P(d), u32, P(d)]),
@@ -369,6 +373,23 @@ def dem_contacts(r, rad, move):
     return row, col[:nnz]
 
 
+def dem_contacts_cells(r, rad, move, cell, cellsize):
+    """dem_contacts over a cell grid of edge >= cellsize (>= 2 max radius)."""
+    r, rad, cell = f64(r), f64(rad), f64(cell)
+    mv = np.ascontiguousarray(move, dtype=np.uint8)
+    n = r.shape[0]
+    row = np.zeros(n + 1, dtype=np.uint32)
+    d, u32 = C.c_double, C.c_uint32
+    nnz = lib().pbo_dem_contacts_cells(_p(r, d), _p(rad, d), _p(mv, C.c_uint8), n, _p(cell, d),
+                                       cellsize, _p(row, u32), None)
+    if nnz == 0xFFFFFFFF:
+        raise ValueError("dem_contacts_cells: bad cell grid")
+    col = np.zeros(max(nnz, 1), dtype=np.uint32)
+    lib().pbo_dem_contacts_cells(_p(r, d), _p(rad, d), _p(mv, C.c_uint8), n, _p(cell, d),
+                                 cellsize, _p(row, u32), _p(col, u32))
+    return row, col[:nnz]
+
+
 def dem_carry(prow, pcol, pval, crow, ccol):
     n = len(crow) - 1
     prow, pcol, crow, ccol = (np.ascontiguousarray(x, dtype=np.uint32) for x in
@@ -399,6 +420,46 @@ def dem_force(r, v, w, rad, minv, move, row, col, val, E, mu, s2, gamma2, mu_r, 
                               _p(mv, C.c_uint8), n, _p(row, u32), _p(col, u32), _p(val, d),
                               E, mu, s2, gamma2, mu_r, dt, _p(F, d), _p(T, d))
     return F, T, val[:int(row[-1])], bad
+
+
+def lsd_zeta(e):
+    """damping ratio of a linear spring-dashpot with restitution e"""
+    le = np.log(e)
+    return -le / np.sqrt(np.pi ** 2 + le ** 2)
+
+
+def dem_force_lsd(r, v, w, rad, minv, move, row, col, hist, kn, zeta, kt, mu, dt):
+    """Linear spring-dashpot law -> F (n,3), T (n,3), updated history
+    (nnz,3), #contacts above the stability bound."""
+    r, v, w, rad, minv = map(f64, (r, v, w, rad, minv))
+    mv = np.ascontiguousarray(move, dtype=np.uint8)
+    row = np.ascontiguousarray(row, dtype=np.uint32)
+    nnz = int(row[-1])
+    col = np.ascontiguousarray(col if len(col) else np.zeros(1), dtype=np.uint32)
+    h = f64(hist).reshape(-1).copy() if nnz else np.zeros(3)
+    n = r.shape[0]
+    F = np.zeros((n, 3))
+    T = np.zeros((n, 3))
+    d, u32 = C.c_double, C.c_uint32
+    bad = lib().pbo_dem_force_lsd(_p(r, d), _p(v, d), _p(w, d), _p(rad, d), _p(minv, d),
+                                  _p(mv, C.c_uint8), n, _p(row, u32), _p(col, u32), _p(h, d),
+                                  kn, zeta, kt, mu, dt, _p(F, d), _p(T, d))
+    return F, T, h[:3 * nnz].reshape(nnz, 3), bad
+
+
+def dem_carry_w(prow, pcol, pval, crow, ccol, hw):
+    n = len(crow) - 1
+    prow, pcol, crow, ccol = (np.ascontiguousarray(x, dtype=np.uint32) for x in
+                              (prow, pcol, crow, ccol))
+    pval = f64(pval).reshape(-1)
+    cval = np.zeros(max(len(ccol) * hw, 1))
+    u32, d = C.c_uint32, C.c_double
+    pc = pcol if len(pcol) else np.zeros(1, dtype=np.uint32)
+    pv = pval if len(pval) else np.zeros(1)
+    cc = ccol if len(ccol) else np.zeros(1, dtype=np.uint32)
+    lib().pbo_dem_carry_w(_p(prow, u32), _p(pc, u32), _p(pv, d), _p(crow, u32), _p(cc, u32),
+                          _p(cval, d), n, hw)
+    return cval[:len(ccol) * hw]
 
 
 def rot_step(w, L, T, m0inv, move, dt):

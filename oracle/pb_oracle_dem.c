@@ -159,3 +159,181 @@ void pbo_rot_step(double *w, double *L, const double *T, const double *m0inv,
     }
   }
 }
+
+/* Linear spring-dashpot contact with a tangential spring and Coulomb
+ * friction (the granular settle workload's law; the reference has only the
+ * Hertz law above, so this restates the published method: Cundall & Strack,
+ * Geotechnique 29 (1979) 47-65, in the "Hookean with shear history" form of
+ * granular MD codes).  Rows, contacts and the movable/fixed rules are those
+ * of pbo_dem_force; FP64 throughout.  Per contact (i, j):
+ *   d = r_i - r_j, R = |d|, n = d/R, delta = rad_i + rad_j - R > 0
+ *   1/meff = minv_i + (move_j ? minv_j : 0)   (a fixed sphere is a wall)
+ *   gamma_n = 2 zeta sqrt(meff kn),  zeta = -ln e / sqrt(pi^2 + ln^2 e)
+ *   v_ij = v_i - v_j, vn = v_ij . n
+ *   Fn = (kn delta - gamma_n vn) n          (not clamped at zero)
+ *   W = rad_i w_i + rad_j w_j, v_c = v_ij - W x n, vt = v_c - vn n
+ *   s  = hist - (hist . n) n + vt dt        (spring kept in the tangent plane)
+ *   Ft = -kt s; if |Ft| > mu |Fn|: Ft *= mu |Fn| / |Ft|, s = -Ft / kt
+ *   hist = s (3 doubles per contact, updated in place)
+ *   F_i += Fn + Ft, T_i += (-rad_i n) x Ft
+ * zeta is passed (the caller derives it from the restitution coefficient).
+ * Returns the contacts with dt >= 2 sqrt(meff / kn) (the stability bound of
+ * the explicit update, the counterpart of the Hertz law's check). */
+uint32_t pbo_dem_force_lsd(const double *r, const double *v, const double *w, const double *rad,
+                           const double *minv, const uint8_t *move, uint32_t n,
+                           const uint32_t *rowptr, const uint32_t *colidx, double *hist,
+                           double kn, double zeta, double kt, double mu, double dt, double *F,
+                           double *T) {
+  uint32_t bad = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    double f[3] = {0, 0, 0}, t[3] = {0, 0, 0};
+    for (uint32_t q = rowptr[i]; q < rowptr[i + 1]; ++q) {
+      const uint32_t j = colidx[q];
+      const double d[3] = {r[3 * i] - r[3 * j], r[3 * i + 1] - r[3 * j + 1],
+                           r[3 * i + 2] - r[3 * j + 2]};
+      const double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      const double rs = rad[i] + rad[j];
+      if (!(R2 < rs * rs) || j == i) continue;
+      const double R = sqrt(R2);
+      const double nv[3] = {d[0] / R, d[1] / R, d[2] / R};
+      const double delta = rs - R;
+      const double meff = 1.0 / (minv[i] + (move[j] ? minv[j] : 0.0));
+      const double gn = 2.0 * zeta * sqrt(meff * kn);
+      if (!(dt < 2.0 * sqrt(meff / kn))) ++bad;
+      const double vij[3] = {v[3 * i] - v[3 * j], v[3 * i + 1] - v[3 * j + 1],
+                             v[3 * i + 2] - v[3 * j + 2]};
+      const double vn = vij[0] * nv[0] + vij[1] * nv[1] + vij[2] * nv[2];
+      const double fn = kn * delta - gn * vn;
+      const double W[3] = {rad[i] * w[3 * i] + rad[j] * w[3 * j],
+                           rad[i] * w[3 * i + 1] + rad[j] * w[3 * j + 1],
+                           rad[i] * w[3 * i + 2] + rad[j] * w[3 * j + 2]};
+      const double Wxn[3] = {W[1] * nv[2] - W[2] * nv[1], W[2] * nv[0] - W[0] * nv[2],
+                             W[0] * nv[1] - W[1] * nv[0]};
+      double vt[3];
+      for (int k = 0; k < 3; ++k) vt[k] = vij[k] - Wxn[k] - vn * nv[k];
+      double *h = &hist[3 * (size_t)q];
+      const double hn = h[0] * nv[0] + h[1] * nv[1] + h[2] * nv[2];
+      double s[3], ft[3];
+      for (int k = 0; k < 3; ++k) {
+        s[k] = h[k] - hn * nv[k] + vt[k] * dt;
+        ft[k] = -kt * s[k];
+      }
+      const double ftm = sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
+      const double cap = mu * fabs(fn);
+      if (ftm > cap) {
+        const double c = cap / ftm;
+        for (int k = 0; k < 3; ++k) {
+          ft[k] *= c;
+          s[k] = -ft[k] / kt;
+        }
+      }
+      for (int k = 0; k < 3; ++k) h[k] = s[k];
+      const double nxf[3] = {nv[1] * ft[2] - nv[2] * ft[1], nv[2] * ft[0] - nv[0] * ft[2],
+                             nv[0] * ft[1] - nv[1] * ft[0]};
+      for (int k = 0; k < 3; ++k) {
+        f[k] += fn * nv[k] + ft[k];
+        t[k] += -rad[i] * nxf[k];
+      }
+    }
+    for (int k = 0; k < 3; ++k) {
+      F[3 * i + k] = f[k];
+      T[3 * i + k] = t[k];
+    }
+  }
+  return bad;
+}
+
+/* pbo_dem_carry for histories of width hw doubles per contact */
+void pbo_dem_carry_w(const uint32_t *prow, const uint32_t *pcol, const double *pval,
+                     const uint32_t *crow, const uint32_t *ccol, double *cval, uint32_t n,
+                     uint32_t hw) {
+  for (uint32_t i = 0; i < n; ++i) {
+    for (uint32_t q = crow[i]; q < crow[i + 1]; ++q)
+      for (uint32_t k = 0; k < hw; ++k) cval[(size_t)q * hw + k] = 0.0;
+    for (uint32_t p = prow[i]; p < prow[i + 1]; ++p)
+      for (uint32_t q = crow[i]; q < crow[i + 1]; ++q)
+        if (ccol[q] == pcol[p])
+          for (uint32_t k = 0; k < hw; ++k) cval[(size_t)q * hw + k] = pval[(size_t)p * hw + k];
+  }
+}
+
+static int cmp_u32(const void *a, const void *b) {
+  const uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+  return (x > y) - (x < y);
+}
+
+/* pbo_dem_contacts over a cell grid (cells of edge >= cellsize, at least
+ * twice the largest radius, covering cell[6]; a centre outside the box is
+ * clamped into the edge cell), for large N: the same contact test and the
+ * same CSR (rows of movable i, columns ascending).  colidx may be NULL to
+ * count only; returns nnz, or UINT32_MAX on a bad grid. */
+uint32_t pbo_dem_contacts_cells(const double *r, const double *rad, const uint8_t *move,
+                                uint32_t n, const double cell[6], double cellsize,
+                                uint32_t *rowptr, uint32_t *colidx) {
+  uint32_t nb[3];
+  for (int a = 0; a < 3; ++a) {
+    const double L = cell[2 * a + 1] - cell[2 * a];
+    if (!(L > 0) || !(cellsize > 0)) return UINT32_MAX;
+    nb[a] = (uint32_t)floor(L / cellsize);
+    if (nb[a] < 1) nb[a] = 1;
+  }
+  const size_t nc = (size_t)nb[0] * nb[1] * nb[2];
+  uint32_t *cid = malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t *start = calloc(nc + 1, sizeof(uint32_t));
+  uint32_t *list = malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t *fillp = malloc(sizeof(uint32_t) * (nc + 1));
+  uint32_t *row = malloc(sizeof(uint32_t) * (n ? n : 1));
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t c[3];
+    for (int a = 0; a < 3; ++a) {
+      const double L = cell[2 * a + 1] - cell[2 * a];
+      double x = (r[3 * i + a] - cell[2 * a]) / L * nb[a];
+      long k = (long)floor(x);
+      if (k < 0) k = 0;
+      if (k >= (long)nb[a]) k = nb[a] - 1;
+      c[a] = (uint32_t)k;
+    }
+    cid[i] = (c[0] * nb[1] + c[1]) * nb[2] + c[2];
+    ++start[cid[i] + 1];
+  }
+  for (size_t c = 0; c < nc; ++c) start[c + 1] += start[c];
+  memcpy(fillp, start, sizeof(uint32_t) * (nc + 1));
+  for (uint32_t i = 0; i < n; ++i) list[fillp[cid[i]]++] = i;
+  uint32_t nnz = 0;
+  rowptr[0] = 0;
+  for (uint32_t i = 0; i < n; ++i) {
+    uint32_t cnt = 0;
+    if (move[i]) {
+      const uint32_t c = cid[i];
+      const int c0 = (int)(c / (nb[1] * nb[2])), c1 = (int)((c / nb[2]) % nb[1]),
+                c2 = (int)(c % nb[2]);
+      for (int a = -1; a <= 1; ++a)
+        for (int b = -1; b <= 1; ++b)
+          for (int d = -1; d <= 1; ++d) {
+            const int x = c0 + a, y = c1 + b, z = c2 + d;
+            if (x < 0 || y < 0 || z < 0 || x >= (int)nb[0] || y >= (int)nb[1] || z >= (int)nb[2])
+              continue;
+            const uint32_t J = ((uint32_t)x * nb[1] + (uint32_t)y) * nb[2] + (uint32_t)z;
+            for (uint32_t q = start[J]; q < start[J + 1]; ++q) {
+              const uint32_t j = list[q];
+              if (j == i) continue;
+              const double dx = r[3 * i] - r[3 * j], dy = r[3 * i + 1] - r[3 * j + 1],
+                           dz = r[3 * i + 2] - r[3 * j + 2];
+              const double R2 = dx * dx + dy * dy + dz * dz;
+              const double rc = rad[i] + rad[j];
+              if (R2 < rc * rc) row[cnt++] = j;
+            }
+          }
+      qsort(row, cnt, sizeof(uint32_t), cmp_u32);
+      if (colidx) memcpy(colidx + nnz, row, sizeof(uint32_t) * cnt);
+    }
+    nnz += cnt;
+    rowptr[i + 1] = nnz;
+  }
+  free(cid);
+  free(start);
+  free(list);
+  free(fillp);
+  free(row);
+  return nnz;
+}

@@ -9,5 +9,5 @@ binding (the counterpart of the reference's SWIG module) plus the synthetic
 workload generators used by the benchmark and tests.
 """
 from ._lib import (Context, System, PBError, PB_FP64, PB_MIXED, LIB_PATH, load,  # noqa: F401
-                   EXPORTS, PB_ERR_ARG, PB_ERR_HIP, PB_ERR_STATE, PB_ERR_CELLS, PB_ERR_NLIST,
+                   EXPORTS, PB_OK, PB_ERR_ARG, PB_ERR_HIP, PB_ERR_STATE, PB_ERR_CELLS, PB_ERR_NLIST,
                    PB_ERR_SKIN, PB_ERR_DT)

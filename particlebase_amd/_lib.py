@@ -51,7 +51,8 @@ EXPORTS = [
     "pb_dem_get_contacts", "pb_dem_last_violations",
     "pb_gt_step", "pb_gt_xi", "pb_adjust_velocities", "pb_const_temp",
     "pb_nlist_build", "pb_nlist_get", "pb_lj_force_nlist", "pb_softcore_force", "pb_euler_step",
-    "pb_dem_set_contacts", "pb_lj_set_tiles", "pb_dem_rewind",
+    "pb_dem_set_contacts", "pb_lj_set_tiles", "pb_dem_rewind", "pb_dem_set_linear",
+    "pb_dem_history_width",
 ]
 
 
@@ -165,6 +166,8 @@ def _extra_signatures():
         "pb_dem_get_contacts": (i32, [vp, P(u32), P(u32), P(d), u32, P(u32)]),
         "pb_dem_last_violations": (i32, [vp, P(u32)]),
         "pb_dem_rewind": (i32, [vp]),
+        "pb_dem_set_linear": (i32, [vp, d, d, d, d, P(d), d]),
+        "pb_dem_history_width": (i32, [vp, P(u32)]),
         "pb_lj_autotune": (i32, [vp, P(pb_run_opts), u32, P(d), P(u32), P(d)]),
     }
 
@@ -535,6 +538,20 @@ class System:
             self._c(self.lib.pb_dem_set_properties(self.h, E, mu, sigma, gamma, mu_r,
                                                    _p(r, C.c_double), 0.0))
 
+    def dem_set_linear(self, kn, e, kt, mu, r0):
+        """Linear spring-dashpot law (normal stiffness kn, restitution e,
+        tangential stiffness kt, Coulomb friction mu) in place of Hertz."""
+        if np.ndim(r0) == 0:
+            self._c(self.lib.pb_dem_set_linear(self.h, kn, e, kt, mu, None, float(r0)))
+        else:
+            r = np.ascontiguousarray(r0, dtype=np.float64)
+            self._c(self.lib.pb_dem_set_linear(self.h, kn, e, kt, mu, _p(r, C.c_double), 0.0))
+
+    def dem_history_width(self):
+        z = C.c_uint32()
+        self._c(self.lib.pb_dem_history_width(self.h, C.byref(z)))
+        return z.value
+
     def rot_set_inertia(self, r0):
         if np.ndim(r0) == 0:
             self._c(self.lib.pb_rot_set_inertia(self.h, None, float(r0)))
@@ -555,16 +572,20 @@ class System:
         row = np.zeros(self.n + 1, dtype=np.uint32)
         self._c(self.lib.pb_dem_get_contacts(self.h, _p(row, C.c_uint32), None, None, 0,
                                              C.byref(z)))
+        hw = self.dem_history_width()
         col = np.zeros(max(z.value, 1), dtype=np.uint32)
-        val = np.zeros(max(z.value, 1), dtype=np.float64)
+        val = np.zeros(max(z.value, 1) * hw, dtype=np.float64)
         self._c(self.lib.pb_dem_get_contacts(self.h, _p(row, C.c_uint32), _p(col, C.c_uint32),
                                              _p(val, C.c_double), col.size, C.byref(z)))
-        return row, col[:z.value], val[:z.value]
+        val = val[:z.value * hw]
+        return row, col[:z.value], (val if hw == 1 else val.reshape(-1, hw))
 
     def dem_set_contacts(self, row, col, val):
         row = np.ascontiguousarray(row, dtype=np.uint32)
         col = np.ascontiguousarray(col, dtype=np.uint32)
-        val = np.ascontiguousarray(val, dtype=np.float64)
+        val = np.ascontiguousarray(val, dtype=np.float64).reshape(-1)
+        if val.size != col.size * self.dem_history_width():
+            raise ValueError("dem_set_contacts: val must hold the history width per contact")
         self._c(self.lib.pb_dem_set_contacts(self.h, _p(row, C.c_uint32), _p(col, C.c_uint32),
                                              _p(val, C.c_double), int(col.size)))
 

@@ -71,6 +71,9 @@ protected:
   double demE = 0, demMu = 0, demSigma = 0, demGamma = 0, demMuR = 0;
   std::vector<double> demR;
   bool demSet = false;
+  // the arguments of setLinearContact (law 1), kept for restarts
+  int demLaw = 0;
+  double demKn = 0, demRest = 0, demKt = 0;
 
 public:
   bool autotunetimestep = false;   //!< true when AdaptiveTime drives dt (no Cundall-Strack abort)
@@ -107,6 +110,23 @@ public:
     demMuR = mu_r;
     pbamd::check(pb_dem_set_properties(sys, demE, demMu, demSigma, demGamma, demMuR, demR.data(), 0.0),
                  "setDEMProperties");
+    demLaw = 0;
+    demSet = true;
+  }
+
+  // Linear spring-dashpot contacts (normal stiffness kn, restitution e),
+  // tangential spring kt with Coulomb friction mu: the granular settle
+  // workload's law, set up in place of setDEMProperties' Hertz law.  No
+  // counterpart in the reference's class, which has only the Hertz law.
+  void setLinearContact(real kn, real e, real kt, real mu, const std::vector<real> &r0) {
+    demR.assign(r0.begin(), r0.end());
+    demKn = kn;
+    demRest = e;
+    demKt = kt;
+    demMu = mu;
+    pbamd::check(pb_dem_set_linear(sys, demKn, demRest, demKt, demMu, demR.data(), 0.0),
+                 "setLinearContact");
+    demLaw = 1;
     demSet = true;
   }
 
@@ -122,6 +142,10 @@ public:
     a.scalar("cudaParticleDEM.demSet", (uint8_t)demSet);
     a.scalar("cudaParticleDEM.autotunetimestep", (uint8_t)autotunetimestep);
     if (!demSet) return;
+    a.scalar("cudaParticleDEM.law", (uint32_t)demLaw);
+    a.scalar("cudaParticleDEM.kn", demKn);
+    a.scalar("cudaParticleDEM.restitution", demRest);
+    a.scalar("cudaParticleDEM.kt", demKt);
     a.scalar("cudaParticleDEM.E", demE);
     a.scalar("cudaParticleDEM.mu", demMu);
     a.scalar("cudaParticleDEM.sigma", demSigma);
@@ -135,8 +159,10 @@ public:
       a.scalar("cudaSparseMat.NNZ", (uint32_t)0xFFFFFFFFu);   // no matrix yet
       return;
     }
+    uint32_t hw = 1;
+    pbamd::check(pb_dem_history_width(sys, &hw), "writeSerialization");
     col.resize(nnz);
-    val.resize(nnz);
+    val.resize((size_t)nnz * hw);
     pbamd::check(pb_dem_get_contacts(sys, row.data(), col.data(), val.data(), nnz, &nnz),
                  "writeSerialization");
     a.scalar("cudaSparseMat.NNZ", nnz);
@@ -157,8 +183,17 @@ public:
       demSigma = a.scalar<double>("cudaParticleDEM.sigma");
       demGamma = a.scalar<double>("cudaParticleDEM.gamma");
       demMuR = a.scalar<double>("cudaParticleDEM.mu_r");
-      pbamd::check(pb_dem_set_properties(sys, demE, demMu, demSigma, demGamma, demMuR, demR.data(), 0.0),
-                   "readSerialization");
+      demLaw = (int)a.scalar<uint32_t>("cudaParticleDEM.law");
+      demKn = a.scalar<double>("cudaParticleDEM.kn");
+      demRest = a.scalar<double>("cudaParticleDEM.restitution");
+      demKt = a.scalar<double>("cudaParticleDEM.kt");
+      if (demLaw == 1)
+        pbamd::check(pb_dem_set_linear(sys, demKn, demRest, demKt, demMu, demR.data(), 0.0),
+                     "readSerialization");
+      else
+        pbamd::check(pb_dem_set_properties(sys, demE, demMu, demSigma, demGamma, demMuR,
+                                           demR.data(), 0.0),
+                     "readSerialization");
       demSet = true;
     }
     loadRotation(a);

@@ -140,10 +140,11 @@ __global__ void k_dem_fill(DemGrid G, const uint32_t *row, uint32_t *col) {
 }
 
 // succeedPrevState (kernelfuncs.cu:1292-1309): both rows are sorted, so a
-// merge finds the contacts present in both matrices; new contacts start at 0
+// merge finds the contacts present in both matrices; new contacts start at 0.
+// hw history doubles per contact (1: Hertz slip length, 3: tangential spring).
 __global__ void k_dem_carry(const uint32_t *prow, const uint32_t *pcol, const double *pval,
                             const uint32_t *crow, const uint32_t *ccol, double *cval,
-                            uint32_t n, uint32_t cap) {
+                            uint32_t n, uint32_t cap, uint32_t hw) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   uint32_t p = prow ? prow[i] : 0u;
@@ -152,7 +153,9 @@ __global__ void k_dem_carry(const uint32_t *prow, const uint32_t *pcol, const do
   for (uint32_t q = crow[i]; q < qe; ++q) {
     const uint32_t j = ccol[q];
     while (p < pe && pcol[p] < j) ++p;
-    cval[q] = (p < pe && pcol[p] == j) ? pval[p] : 0.0;
+    const bool hit = p < pe && pcol[p] == j;
+    for (uint32_t k = 0; k < hw; ++k)
+      cval[(size_t)q * hw + k] = hit ? pval[(size_t)p * hw + k] : 0.0;
   }
 }
 
@@ -168,6 +171,7 @@ struct DemForceArgs {
   uint32_t n, npad;
   uint32_t cap;                          // capacity of col / val
   double E, mu, s2, gamma2, mu_r, dt;
+  double kn, zeta, kt;                   // linear law
 };
 
 // DEMContact_F4 (kernelinline.hh:235-363) over the CSR rows
@@ -246,6 +250,83 @@ __global__ void __launch_bounds__(256) k_dem_force(DemForceArgs A) {
   if (bad) atomicAdd(A.bad, bad);
 }
 
+// Linear spring-dashpot normal force with a tangential spring and a Coulomb
+// cap (Cundall & Strack, Geotechnique 29 (1979) 47-65; no reference
+// counterpart: the granular settle workload's law, behind the same calls as
+// the Hertz kernel above).  Per contact, n = (r_i - r_j)/|r_ij|,
+// delta = rad_i + rad_j - |r_ij|, 1/meff = minv_i + minv_j (0 for a fixed j):
+//   Fn = (kn delta - 2 zeta sqrt(meff kn) v_ij.n) n
+//   vt = tangential part of v_ij - (rad_i w_i + rad_j w_j) x n
+//   s  = history projected on the tangent plane + vt dt, Ft = -kt s,
+//        capped at mu |Fn| (the spring then follows the cap, s = -Ft/kt)
+//   F_i += Fn + Ft, T_i += (-rad_i n) x Ft
+// The history (3 doubles per contact) belongs to row i and is updated in
+// place.  FP64, one thread per storage slot as k_dem_force.
+__global__ void __launch_bounds__(256) k_dem_force_lsd(DemForceArgs A) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= A.npad) return;
+  const uint32_t i = A.orig[s];
+  if (i == PB_NONE) return;
+  const double4 pi = A.r[s], vi = A.v[s], wi = A.w[s];
+  const double ri = A.rad[s];
+  double f[3] = {0, 0, 0}, t[3] = {0, 0, 0};
+  uint32_t bad = 0;
+  const uint32_t qe = min(A.row[i + 1], A.cap);
+  for (uint32_t q = A.row[i]; q < qe; ++q) {
+    const uint32_t sj = A.inv[A.col[q]];
+    const double4 pj = A.r[sj], vj = A.v[sj], wj = A.w[sj];
+    const double rj = A.rad[sj];
+    const double d[3] = {pi.x - pj.x, pi.y - pj.y, pi.z - pj.z};
+    const double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    const double rs = ri + rj;
+    if (!(R2 < rs * rs)) continue;
+    const double R = sqrt(R2);
+    const double nv[3] = {d[0] / R, d[1] / R, d[2] / R};
+    const double delta = rs - R;
+    const double meff = 1.0 / (vi.w + (A.move[sj] ? vj.w : 0.0));
+    const double gn = 2.0 * A.zeta * sqrt(meff * A.kn);
+    if (!(A.dt < 2.0 * sqrt(meff / A.kn))) ++bad;
+    const double vij[3] = {vi.x - vj.x, vi.y - vj.y, vi.z - vj.z};
+    const double vn = vij[0] * nv[0] + vij[1] * nv[1] + vij[2] * nv[2];
+    const double fn = A.kn * delta - gn * vn;
+    const double W[3] = {ri * wi.x + rj * wj.x, ri * wi.y + rj * wj.y, ri * wi.z + rj * wj.z};
+    const double Wxn[3] = {W[1] * nv[2] - W[2] * nv[1], W[2] * nv[0] - W[0] * nv[2],
+                           W[0] * nv[1] - W[1] * nv[0]};
+    double *h = A.val + (size_t)q * 3;
+    const double h0 = h[0], h1 = h[1], h2 = h[2];
+    const double hn = h0 * nv[0] + h1 * nv[1] + h2 * nv[2];
+    double sp[3], ft[3];
+    sp[0] = h0 - hn * nv[0] + (vij[0] - Wxn[0] - vn * nv[0]) * A.dt;
+    sp[1] = h1 - hn * nv[1] + (vij[1] - Wxn[1] - vn * nv[1]) * A.dt;
+    sp[2] = h2 - hn * nv[2] + (vij[2] - Wxn[2] - vn * nv[2]) * A.dt;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ft[k] = -A.kt * sp[k];
+    const double ftm = sqrt(ft[0] * ft[0] + ft[1] * ft[1] + ft[2] * ft[2]);
+    const double cap = A.mu * fabs(fn);
+    if (ftm > cap) {
+      const double c = cap / ftm;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        ft[k] *= c;
+        sp[k] = -ft[k] / A.kt;
+      }
+    }
+    h[0] = sp[0];
+    h[1] = sp[1];
+    h[2] = sp[2];
+    const double nxf[3] = {nv[1] * ft[2] - nv[2] * ft[1], nv[2] * ft[0] - nv[0] * ft[2],
+                           nv[0] * ft[1] - nv[1] * ft[0]};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      f[k] += fn * nv[k] + ft[k];
+      t[k] += -ri * nxf[k];
+    }
+  }
+  A.F[s] = make_double4(f[0], f[1], f[2], 0.0);
+  A.T[s] = make_double4(t[0], t[1], t[2], 0.0);
+  if (bad) atomicAdd(A.bad, bad);
+}
+
 __global__ void k_dem_inv(const uint32_t *orig, uint32_t n, uint32_t *inv) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < n && orig[s] != PB_NONE) inv[orig[s]] = s;
@@ -294,6 +375,24 @@ static int upload_radius(pb_sys *s, const double *r0, double r0_all) {
   return PB_OK;
 }
 
+// a change of contact law changes the history width: the contact matrices
+// start again (as after the first set-up)
+static void set_law(pb_sys *s, int law, uint32_t hw) {
+  if (s->dem_hw != hw) {
+    for (int b = 0; b < 2; ++b) {
+      pb_free(s->cm_val[b]);
+      s->cm_val[b] = nullptr;
+      s->cm_cap[b] = 0;
+      s->cm_nnz[b] = 0;
+    }
+    s->cm_cur = -1;
+    s->cm_prev = false;
+    s->cm_rewound = false;
+  }
+  s->dem_law = law;
+  s->dem_hw = hw;
+}
+
 extern "C" int pb_dem_set_properties(pb_sys *s, double E, double mu, double sigma, double gamma,
                                      double mu_r, const double *r0, double r0_all) {
   if (!s) return PB_ERR_ARG;
@@ -308,6 +407,28 @@ extern "C" int pb_dem_set_properties(pb_sys *s, double E, double mu, double sigm
   s->dem_mu_r = mu_r;
   if (int rc = upload_radius(s, r0, r0_all)) return rc;
   if (int rc = pb_ensure_rotation(s)) return rc;
+  set_law(s, 0, 1);
+  s->dem_set = true;
+  return PB_OK;
+}
+
+extern "C" int pb_dem_set_linear(pb_sys *s, double kn, double e, double kt, double mu,
+                                 const double *r0, double r0_all) {
+  if (!s) return PB_ERR_ARG;
+  if (s->nrep != 1) return pb_fail(s->ctx, PB_ERR_ARG, "DEM: single replica only");
+  if (!(kn > 0) || !(kt > 0) || !(mu >= 0))
+    return pb_fail(s->ctx, PB_ERR_ARG, "pb_dem_set_linear: need kn > 0, kt > 0, mu >= 0");
+  if (!(e > 0 && e <= 1))
+    return pb_fail(s->ctx, PB_ERR_ARG, "pb_dem_set_linear: restitution must be in (0, 1]");
+  PB_HIP(s->ctx, hipSetDevice(s->ctx->device));
+  const double le = std::log(e);
+  s->dem_kn = kn;
+  s->dem_kt = kt;
+  s->dem_zeta = -le / std::sqrt(M_PI * M_PI + le * le);   // damping ratio of restitution e
+  s->dem_mu = mu;
+  if (int rc = upload_radius(s, r0, r0_all)) return rc;
+  if (int rc = pb_ensure_rotation(s)) return rc;
+  set_law(s, 1, 3);
   s->dem_set = true;
   return PB_OK;
 }
@@ -363,7 +484,8 @@ static int ensure_csr(pb_sys *s, int b, uint32_t nnz) {
   if (!s->cm_row[b] && (rc = pb_alloc(s->ctx, &s->cm_row[b], (size_t)s->n + 1))) return rc;
   if (s->cm_cap[b] < nnz || !s->cm_col[b]) {
     const uint32_t cap = (uint32_t)std::min<uint64_t>(((uint64_t)nnz * 5 / 4 + 512), 0xFFFFFFFFull);
-    if ((rc = pb_alloc(s->ctx, &s->cm_col[b], cap)) || (rc = pb_alloc(s->ctx, &s->cm_val[b], cap)))
+    if ((rc = pb_alloc(s->ctx, &s->cm_col[b], cap)) ||
+        (rc = pb_alloc(s->ctx, &s->cm_val[b], (size_t)cap * s->dem_hw)))
       return rc;
     s->cm_cap[b] = cap;
   }
@@ -433,6 +555,9 @@ extern "C" int pb_dem_calc_force(pb_sys *s, double dt, int autotune, uint32_t *n
   A.gamma2 = s->dem_gamma2;
   A.mu_r = s->dem_mu_r;
   A.dt = dt;
+  A.kn = s->dem_kn;
+  A.zeta = s->dem_zeta;
+  A.kt = s->dem_kt;
   const int p = s->cm_cur;
   // fill + sort, history carry, contact forces and torques up to the
   // capacity; then nnz and the time-step check in one read
@@ -447,13 +572,18 @@ extern "C" int pb_dem_calc_force(pb_sys *s, double dt, int autotune, uint32_t *n
                          p >= 0 ? s->cm_row[p] : (const uint32_t *)nullptr,
                          p >= 0 ? s->cm_col[p] : (const uint32_t *)nullptr,
                          p >= 0 ? s->cm_val[p] : (const double *)nullptr, s->cm_row[b],
-                         s->cm_col[b], s->cm_val[b], n, s->cm_cap[b]);
+                         s->cm_col[b], s->cm_val[b], n, s->cm_cap[b], s->dem_hw);
       PB_LAUNCH_CHECK(s->ctx, "k_dem_fill/carry");
     }
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s->d_counter + 8, 1u, 0u);
     // every real particle's F and T is written (empty rows give zero);
     // padding slots are never written and stay zero
-    if (n) hipLaunchKernelGGL(k_dem_force, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, A);
+    if (n) {
+      if (s->dem_law == 1)
+        hipLaunchKernelGGL(k_dem_force_lsd, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, A);
+      else
+        hipLaunchKernelGGL(k_dem_force, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, A);
+    }
     PB_LAUNCH_CHECK(s->ctx, "k_dem_force");
     return PB_OK;
   };
@@ -494,7 +624,8 @@ extern "C" int pb_dem_get_contacts(pb_sys *s, uint32_t *rowptr, uint32_t *colidx
   if (colidx && z)
     PB_HIP(s->ctx, hipMemcpyAsync(colidx, s->cm_col[b], 4 * (size_t)z, hipMemcpyDeviceToHost, st));
   if (val && z)
-    PB_HIP(s->ctx, hipMemcpyAsync(val, s->cm_val[b], 8 * (size_t)z, hipMemcpyDeviceToHost, st));
+    PB_HIP(s->ctx, hipMemcpyAsync(val, s->cm_val[b], 8 * (size_t)z * s->dem_hw,
+                                  hipMemcpyDeviceToHost, st));
   PB_HIP(s->ctx, hipStreamSynchronize(st));
   return PB_OK;
 }
@@ -520,7 +651,8 @@ extern "C" int pb_dem_set_contacts(pb_sys *s, const uint32_t *rowptr, const uint
   PB_HIP(s->ctx, hipMemcpyAsync(s->cm_row[b], rowptr, 4 * ((size_t)n + 1), hipMemcpyHostToDevice, st));
   if (nnz) {
     PB_HIP(s->ctx, hipMemcpyAsync(s->cm_col[b], colidx, 4 * (size_t)nnz, hipMemcpyHostToDevice, st));
-    PB_HIP(s->ctx, hipMemcpyAsync(s->cm_val[b], val, 8 * (size_t)nnz, hipMemcpyHostToDevice, st));
+    PB_HIP(s->ctx, hipMemcpyAsync(s->cm_val[b], val, 8 * (size_t)nnz * s->dem_hw,
+                                  hipMemcpyHostToDevice, st));
   }
   PB_HIP(s->ctx, hipStreamSynchronize(st));
   s->cm_nnz[b] = nnz;
@@ -547,6 +679,12 @@ extern "C" int pb_dem_rewind(pb_sys *s) {
   s->cm_cur = s->cm_prev ? (s->cm_cur ^ 1) : -1;
   s->cm_prev = false;
   s->cm_rewound = true;
+  return PB_OK;
+}
+
+extern "C" int pb_dem_history_width(pb_sys *s, uint32_t *hw) {
+  if (!s || !hw) return PB_ERR_ARG;
+  *hw = s->dem_hw;
   return PB_OK;
 }
 

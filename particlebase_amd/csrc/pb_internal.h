@@ -159,6 +159,11 @@ struct pb_sys {
   double *m0inv = nullptr;                     // minv / (0.4 r0^2), storage order
   double dem_E = 0, dem_mu = 0, dem_s2 = 0, dem_gamma2 = 0, dem_mu_r = 0;
   bool dem_set = false;
+  // contact law: 0 = the reference's Hertz law (scalar history per contact),
+  // 1 = linear spring-dashpot + tangential spring (3 doubles per contact)
+  int dem_law = 0;
+  uint32_t dem_hw = 1;                         // history doubles per contact
+  double dem_kn = 0, dem_zeta = 0, dem_kt = 0;
   // contact CSR by original index, double-buffered for the tangential history
   uint32_t *cm_row[2] = {nullptr, nullptr}, *cm_col[2] = {nullptr, nullptr};
   double *cm_val[2] = {nullptr, nullptr};

@@ -239,6 +239,7 @@ class DEMSystem:
     rad: np.ndarray
     fixed: np.ndarray
     cell: np.ndarray
+    v: np.ndarray | None = None
 
     @property
     def n(self):
@@ -273,3 +274,57 @@ def dem_settle(n: int = 200_000, rmin: float = 0.9e-3, rmax: float = 1.1e-3,
     m = density * 4.0 / 3.0 * math.pi * rad ** 3
     cell = np.array([0.0, box[0], 0.0, box[1], 0.0, box[2]])
     return DEMSystem(r=r, m=m, rad=rad, fixed=np.zeros(n, dtype=np.uint8), cell=cell)
+
+
+def dem_settle_config(n: int = 200_000, rmin: float = 0.9e-3, rmax: float = 1.1e-3,
+                      box=(0.1, 0.1, 0.4), density: float = 7874.0, seed: int = 4,
+                      vscale: float = 0.0, batch: int = 65536) -> DEMSystem:
+    """Granular settle as the workload states it: n spheres with radii
+    uniform in [rmin, rmax] at random non-overlapping positions inside the
+    box (random sequential insertion: candidates with uniform centres in
+    [rad, L - rad] per axis are taken in stream order, each kept when it
+    overlaps no sphere kept before it).  200,000 spheres of ~1 mm in
+    0.1 x 0.1 x 0.4 m fill ~21 % of the volume, below the insertion's
+    jamming limit (~38 %).  vscale > 0 gives every sphere normal velocity
+    components of that standard deviation (a granular gas); the workload
+    itself starts at rest.  Masses are those of solid spheres of the given
+    density; none is fixed (the walls are the box's reflecting faces)."""
+    from scipy.spatial import cKDTree
+
+    L = np.asarray(box, dtype=np.float64)
+    pos = np.empty((0, 3))
+    rad = np.empty(0)
+    off = 0
+    while rad.size < n:
+        k = batch
+        u = uniform53(seed, 4 * k, offset=4 * off).reshape(k, 4)
+        off += k
+        rc = rmin + (rmax - rmin) * u[:, 3]
+        c = rc[:, None] + u[:, :3] * (L[None, :] - 2.0 * rc[:, None])
+        ok = np.ones(k, dtype=bool)
+        if rad.size:
+            tree = cKDTree(pos)
+            d, idx = tree.query(c, k=16, distance_upper_bound=rmax + rmax)
+            hit = np.isfinite(d)
+            idx = np.where(hit, idx, 0)
+            ok &= ~np.any(hit & (d < rc[:, None] + rad[idx]), axis=1)
+        # conflicts inside the batch: the later candidate of an overlapping
+        # pair goes when the earlier one is still kept
+        pairs = cKDTree(c).query_pairs(2.0 * rmax, output_type="ndarray")
+        if pairs.size:
+            pairs = np.sort(pairs, axis=1)
+            dd = np.sqrt(((c[pairs[:, 0]] - c[pairs[:, 1]]) ** 2).sum(1))
+            pairs = pairs[dd < rc[pairs[:, 0]] + rc[pairs[:, 1]]]
+            pairs = pairs[np.lexsort((pairs[:, 0], pairs[:, 1]))]
+            for a, b in pairs:
+                if ok[a] and ok[b]:
+                    ok[b] = False
+        keep = np.nonzero(ok)[0][: n - rad.size]
+        pos = np.concatenate([pos, c[keep]])
+        rad = np.concatenate([rad, rc[keep]])
+    m = density * 4.0 / 3.0 * math.pi * rad ** 3
+    cell = np.array([0.0, box[0], 0.0, box[1], 0.0, box[2]])
+    sysd = DEMSystem(r=pos, m=m, rad=rad, fixed=np.zeros(n, dtype=np.uint8), cell=cell)
+    if vscale > 0:
+        sysd.v = normals(seed + 3, 3 * n).reshape(n, 3) * vscale
+    return sysd

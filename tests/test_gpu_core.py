@@ -261,5 +261,7 @@ def test_context_destroy_order(ctx):
     s.import_particles(np.full((100, 3), 1.0), np.ones(100))
     assert np.all(s.get("r") == 1.0)
     s.close()
-    assert lib.pb_ctx_destroy(c2.h) == pb.PB_OK
-    c2.h = None
+    rc = lib.pb_ctx_destroy(c2.h)
+    if rc == pb.PB_OK:
+        c2.h = None          # the handle is gone: the Context must not destroy it again
+    assert rc == pb.PB_OK

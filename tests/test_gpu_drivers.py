@@ -63,6 +63,28 @@ def test_dem_settle():
     assert 3000 // 8 <= sel <= cells
 
 
+def test_dem_settle_linear_config(tmp_path):
+    """The granular settle workload at its size through the driver: 200,000
+    spheres from random non-overlapping insertion (moving as a gas, so that
+    contacts form in a short run), the linear spring-dashpot law, adaptive
+    time step with its cap."""
+    import numpy as np
+    from particlebase_amd import workloads
+    w = workloads.dem_settle_config(vscale=0.3)
+    path = tmp_path / "dem.bin"
+    with open(path, "wb") as f:
+        f.write(np.array([w.n], dtype=np.uint32).tobytes())
+        for x in (w.r, w.rad, w.m, w.v):
+            f.write(np.ascontiguousarray(x, dtype=np.float64).tobytes())
+    d = run("dem_settle", w.n, 300, "linear", path)
+    assert int(d["N"]) == 200_000 and int(d["steps"]) == 300
+    assert int(d["max_contacts"]) > 100
+    # the step grew from 2e-6 to its cap 0.2 sqrt(meff_min / kn) and no further
+    mmin = w.m.min()
+    assert 2e-6 < float(d["dt"]) <= 0.2 * np.sqrt(0.5 * mmin / 1e4) * 1.0001
+    assert 0.0 < float(d["zmean"]) < 0.4
+
+
 def test_md_gauss():
     # testMD2 order: liquid argon under the Gaussian thermostat
     d = run("md_gauss", 300)

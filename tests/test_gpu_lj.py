@@ -263,7 +263,7 @@ def test_autotune_restores_state(ctx, orc):
     s.clear_force(True)
     r0, v0 = s.get("r"), s.get("v")
     t = s.lj_autotune(0.005, steps=8, precision=pb.PB_FP64, skin=0.3, rebuild_every=10)
-    assert t["tile_w"] in (1, 2) and 0.3 < t["tile_fill"] < 1.0 and t["ms_per_step"] > 0
+    assert t["tile_w"] in (1, 2) and 0.3 < t["tile_fill"] <= 1.0 and t["ms_per_step"] > 0
     assert np.array_equal(s.get("r"), r0) and np.array_equal(s.get("v"), v0)
     s.lj_force(pb.PB_FP64)
     p = orc.lj_params([1.0, 1.0], 1, RC2, True)
@@ -273,47 +273,76 @@ def test_autotune_restores_state(ctx, orc):
 
 def test_force_1M_vs_oracle(ctx, orc):
     # the full 1M-atom workload: every atom's force on the first call of a
-    # fresh system and again after a fused run (window tables of all 3087
-    # segments, XCD-remapped blocks)
-    say("lj force N=1,048,576 jittered, FP64 and mixed, fresh system and after a run")
+    # fresh, jittered system in FP64 (window tables of all segments,
+    # XCD-remapped blocks); then the benchmarked state -- the lattice melted
+    # by 120 fused mixed steps -- with the mixed pair kernel held to the
+    # stated bound 1e-5 F_rms against the oracle
+    say("lj force N=1,048,576: jittered lattice (FP64), then the melting liquid (FP64, mixed)")
     w = jittered(workloads.lj_fcc(64, 0.8442, 0.72, seed=2), 0.05, 1)
     p = orc.lj_params([1.0, 1.0], 1, RC2, True)
     Fo, _, _ = orc.lj_forces_cells(w.r, w.type, w.cell, RC2, p)
     frms = np.sqrt((Fo ** 2).sum(axis=1).mean())
     s = lj_sys(ctx, w)
-    for prec, tol in ((pb.PB_FP64, 1e-12), (pb.PB_MIXED, 3 * MIXED_TOL), (pb.PB_FP64, 1e-12)):
-        s.lj_force(prec, energies=False)
-        assert np.abs(s.get("F") - Fo).max() / frms < tol, prec
+    s.lj_force(pb.PB_FP64, energies=False)
+    assert np.abs(s.get("F") - Fo).max() / frms < 1e-12
+    s.close()
+    w = workloads.lj_fcc(64, 0.8442, 0.72, seed=2)
+    s = lj_sys(ctx, w)
     s.clear_force(True)
-    s.lj_run(0.005, 30, precision=pb.PB_MIXED, skin=0.3, rebuild_every=20, energy_every=0)
+    s.lj_run(0.005, 120, precision=pb.PB_MIXED, skin=0.3, rebuild_every=20, energy_every=0)
     r = s.get("r")
-    Fo, _, _ = orc.lj_forces_cells(r, w.type, w.cell, RC2, p)
-    for prec, tol in ((pb.PB_FP64, 1e-12), (pb.PB_MIXED, 3 * MIXED_TOL)):
+    Fo, _, _ = orc.lj_forces_cells(orc.wrap(r, w.cell).reshape(-1, 3), w.type, w.cell, RC2, p)
+    frms = np.sqrt((Fo ** 2).sum(axis=1).mean())
+    for prec, tol in ((pb.PB_FP64, 1e-12), (pb.PB_MIXED, MIXED_TOL), (pb.PB_FP64, 1e-12)):
         s.lj_force(prec, energies=False)
-        assert np.abs(s.get("F") - Fo).max() / frms < tol, prec
+        err = np.abs(s.get("F") - Fo).max() / frms
+        print(f"  precision {prec}: max |F - F_oracle| / F_rms = {err:.3e}", flush=True)
+        assert err < tol, prec
 
 
-@pytest.mark.parametrize("precision", [pb.PB_FP64, pb.PB_MIXED])
-def test_energy_drift_1e4_steps_n4000(ctx, precision):
-    print(f"\n[case] 10^4-step energy drift, N=4000, precision={precision}", flush=True)
-    # the LJ fluid melted from the lattice, then 10^4 fused steps: the total
-    # energy per atom moves by far less than kT (0.72) and the FP32 pair
-    # forces drift no more than the FP64 ones by a wide margin
-    w = workloads.lj_config1()
-    s = pb.System(ctx, [w.n])
-    s.set_cell(w.cell)
-    s.import_particles(w.r, w.m, v=w.v, type=w.type)
+def test_energy_drift_1e4_steps_n4000(ctx):
+    print("\n[case] 10^4-step energy drift, 4 x N=4000 fluids, FP64 and mixed from the same "
+          "starts", flush=True)
+    # Four N=4000 fluids (the FCC 10^3 x 4 start with velocities of seeds
+    # 1..4), melted for 1000 FP64 steps as one batch, then 10^4 fused steps
+    # in each precision from those same states.  Per replica the total
+    # energy per atom moves by far less than kT (0.72); the drift is the
+    # fitted slope of E/N over the run, and the mixed form's mean |drift|
+    # over the four is at most twice the FP64 form's (floored at 2e-5 per
+    # 10^4 steps, the FP64 integration error at dt = 0.005).  One trajectory
+    # is not enough: the fitted slope of a single run wanders by about its
+    # own size between starts.
+    ws = [workloads.lj_fcc(10, 0.8442, 0.72, seed=k) for k in (1, 2, 3, 4)]
+    n = ws[0].n
+    r = np.concatenate([w.r for w in ws])
+    v = np.concatenate([w.v for w in ws])
+    m = np.concatenate([w.m for w in ws])
+    t = np.concatenate([w.type for w in ws])
+    s = pb.System(ctx, [n] * 4)
+    s.set_cell(np.concatenate([w.cell for w in ws]))
+    s.import_particles(r, m, v=v, type=t)
     s.lj_set_params([1.0, 1.0], 1)
     s.lj_set_cutoff(6.25, shifted=True)
     s.clear_force(True)
-    s.lj_run(0.005, 1000, precision=precision, skin=0.3, rebuild_every=20, energy_every=0)
-    e, _ = s.lj_run(0.005, 10000, precision=precision, skin=0.3, rebuild_every=20,
-                    energy_every=100)
-    et = (e[:, 0, 0] + e[:, 0, 1]) / w.n
-    assert et.size == 100 and np.all(np.isfinite(et))
-    assert abs(et[-1] - et[0]) < 2e-3      # measured: about 4e-5 (mixed), 2e-5 (FP64)
-    assert np.ptp(et) < 1e-2
-    assert s.lj_run_stats()["skin_violation"] == 0
+    s.lj_run(0.005, 1000, precision=pb.PB_FP64, skin=0.3, rebuild_every=20, energy_every=0)
+    r0, v0, a0 = s.get("r"), s.get("v"), s.get("a")
+    drift = {}
+    for precision in (pb.PB_FP64, pb.PB_MIXED):
+        s.set("r", r0)
+        s.set("v", v0)
+        s.set("a", a0)
+        e, _ = s.lj_run(0.005, 10000, precision=precision, skin=0.3, rebuild_every=20,
+                        energy_every=100)
+        et = (e[:, :, 0] + e[:, :, 1]) / n               # records x replicas
+        assert et.shape == (100, 4) and np.all(np.isfinite(et))
+        assert np.all(np.abs(et[-1] - et[0]) < 2e-3)
+        assert np.all(np.ptp(et, axis=0) < 1e-2)
+        assert s.lj_run_stats()["skin_violation"] == 0
+        slopes = np.polyfit(np.arange(100) * 100.0, et, 1)[0] * 1e4
+        drift[precision] = float(np.abs(slopes).mean())
+        print(f"  precision {precision}: fitted dE/N per 10^4 steps {np.array2string(slopes, precision=2)}"
+              f", end - start {np.array2string(et[-1] - et[0], precision=2)}", flush=True)
+    assert drift[pb.PB_MIXED] <= 2.0 * max(drift[pb.PB_FP64], 2e-5)
 
 
 @pytest.mark.parametrize("prec", [pb.PB_FP64, pb.PB_MIXED])
@@ -346,14 +375,69 @@ def test_ragged_tiny_replicas(ctx, orc, prec):
     F = s.get("F")
     p = orc.lj_params([1.0, 1.0], 1, RC2, True)
     off = 0
-    tol = 1e-11 if prec == pb.PB_FP64 else 2e-5
+    tol = 1e-11 if prec == pb.PB_FP64 else MIXED_TOL
     for k, (n, L) in enumerate(sizes):
         x = rs[k]
         Fo, peo, _ = orc.lj_forces_brute(x, np.zeros(n, np.uint16), np.array(cells[k], float),
                                          RC2, p)
         Fk = F[off:off + n]
-        scale = max(np.abs(Fo).max(), 1.0)
-        assert np.abs(Fk - Fo).max() < tol * scale, k
+        # the replica's F_rms, floored at 1 epsilon/sigma: the 2-atom replica
+        # sits near the force zero r = 2^(1/6) sigma, where F is a small
+        # difference of terms of size ~24 epsilon/sigma whose FP32 rounding
+        # sets the error, so its own F_rms (~2e-3) is no scale for it
+        frms = np.sqrt((Fo ** 2).sum(axis=1).mean())
+        if n == 1:
+            assert np.all(Fk == 0), k
+        else:
+            assert np.abs(Fk - Fo).max() < tol * max(frms, 1.0), k
         assert pe[k] == pytest.approx(peo, rel=1e-5 if prec == pb.PB_MIXED else 1e-11, abs=1e-9)
         off += n
     assert np.all(F[:1] == 0)
+
+
+def test_batched_256_replicas_config_size(ctx, orc):
+    """The batched workload at its size: 256 replicas, N log-uniform in
+    [500, 20000] (1.39M atoms), densities in [0.7, 0.9], T in [0.7, 2.0].
+    After 120 fused mixed steps of the batch, every replica's mixed force is
+    within 1e-5 of its own F_rms of the oracle's FP64 force (the max over
+    replicas is asserted), and the integer g(r) histograms of all 256
+    replicas (200 bins to L/2) equal the oracle's FP32 brute force bit for
+    bit on the same snapshot."""
+    say("batched LJ: 256 replicas (config size), 120 mixed steps, per-replica force error + "
+        "g(r) bit-exact")
+    reps = workloads.lj_replicas(256, 500, 20000, seed=5)
+    r, v, m, t, cells = reps.concat()
+    s = pb.System(ctx, reps.counts)
+    s.set_cell(cells)
+    s.import_particles(r, m, v=v, type=t)
+    s.lj_set_params([1.0, 1.0], 1)
+    s.lj_set_cutoff(RC2, shifted=True)
+    s.clear_force(True)
+    s.lj_run(0.005, 120, precision=pb.PB_MIXED, skin=0.3, rebuild_every=20, energy_every=0)
+    assert s.lj_run_stats()["skin_violation"] == 0
+    s.lj_force(pb.PB_MIXED, energies=False)
+    F = s.get("F")
+    x = s.get("r")
+    p = orc.lj_params([1.0, 1.0], 1, RC2, True)
+    errs = []
+    off = 0
+    for k, w in enumerate(reps.systems):
+        xk = x[off:off + w.n]
+        Fo, _, _ = orc.lj_forces_cells(orc.wrap(xk, w.cell).reshape(-1, 3), w.type, w.cell, RC2, p)
+        frms = np.sqrt((Fo ** 2).sum(axis=1).mean())
+        errs.append(np.abs(F[off:off + w.n] - Fo).max() / frms)
+        off += w.n
+    errs = np.array(errs)
+    print(f"  per-replica max |F_mixed - F_oracle| / F_rms: max {errs.max():.3e}, "
+          f"median {np.median(errs):.3e}", flush=True)
+    assert errs.max() < MIXED_TOL
+    rsteps = np.array([np.float32(w.L / 400.0) for w in reps.systems], dtype=np.float32)
+    h = s.gr_hist(rsteps, 200)
+    off = 0
+    for k, w in enumerate(reps.systems):
+        xk = x[off:off + w.n].astype(np.float32)
+        ri = np.float32(1.0 / np.float64(rsteps[k]))
+        oh = orc.gr_hist_f32(xk, w.cell.astype(np.float32), ri, 200)
+        assert np.array_equal(h[k], oh), k
+        assert int(h[k].sum()) == w.n * (w.n - 1) // 2
+        off += w.n

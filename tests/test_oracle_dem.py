@@ -194,3 +194,169 @@ def test_lf_rollback_inverts_two_steps(orc):
     # a rollback is not a single backward step: it moves r by two steps
     r1, _ = orc.lf_step(r0, v0, a, move, dt)
     assert np.abs(rb - r1)[move == 1].max() > 1e-4
+
+
+# ------------------------------------------------ linear spring-dashpot law
+# Pins of the granular settle workload's contact law (linear normal spring +
+# dashpot, tangential spring with a Coulomb cap; Cundall & Strack 1979) by
+# the damped oscillator's closed forms and by contact-point kinematics.
+KN, KT, MU_L = 1.0e4, 1.0e4 * 2.0 / 7.0, 0.5
+
+
+def lsd(orc, r, v, w, rad, move, hist=None, minv=(1.0, 1.0), e=0.8, dt=1e-6):
+    row, col = orc.dem_contacts(r, rad, move)
+    if hist is None:
+        hist = np.zeros((len(col), 3))
+    return orc.dem_force_lsd(r, v, w, rad, np.array(minv, float), move, row, col, hist, KN,
+                             orc.lsd_zeta(e), KT, MU_L, dt)
+
+
+def collide(orc, e, move2=1, m=1e-3, v0=0.05, steps_per_tc=4000):
+    """Head-on collision of two spheres of radius 1 mm (or one on a fixed
+    sphere) integrated with leap-frog; returns (contact time, restitution,
+    closed-form contact time)."""
+    rad = np.array([1e-3, 1e-3])
+    move = np.array([1, move2], dtype=np.uint8)
+    minv = np.array([1.0 / m, 1.0 / m])
+    meff = 1.0 / (minv[0] + (minv[1] if move2 else 0.0))
+    z = orc.lsd_zeta(e)
+    w0 = np.sqrt(KN / meff)
+    tc = np.pi / (w0 * np.sqrt(1.0 - z * z))
+    dt = tc / steps_per_tc
+    x = np.array([[0.0, 0, 0], [2.0e-3 + 1e-7, 0, 0]])
+    v = np.array([[v0, 0, 0], [0.0, 0, 0]]) if move2 == 0 else \
+        np.array([[v0 / 2, 0, 0], [-v0 / 2, 0, 0]])
+    w = np.zeros((2, 3))
+    vrel0 = v[0, 0] - v[1, 0]
+    prow = np.zeros(3, np.uint32)
+    pcol = np.zeros(0, np.uint32)
+    hist = np.zeros((0, 3))
+    t_in = t_out = None
+    for k in range(3 * steps_per_tc):
+        row, col = orc.dem_contacts(x, rad, move)
+        hist = orc.dem_carry_w(prow, pcol, hist, row, col, 3).reshape(-1, 3)
+        F, T, hist, _ = orc.dem_force_lsd(x, v, w, rad, minv, move, row, col, hist, KN, z, KT,
+                                          MU_L, dt)
+        if len(col) and t_in is None:
+            t_in = k
+        if t_in is not None and not len(col):
+            t_out = k
+            break
+        prow, pcol = row, col
+        v = v + F * minv[:, None] * move[:, None] * dt
+        x = x + v * move[:, None] * dt
+    vrel1 = v[0, 0] - v[1, 0]
+    return (t_out - t_in) * dt, -vrel1 / vrel0, tc, dt
+
+
+@pytest.mark.parametrize("move2", [1, 0])
+def test_lsd_collision_time_and_restitution(orc, move2):
+    for e in (0.8, 0.5):
+        t, eps, tc, dt = collide(orc, e, move2)
+        assert abs(t - tc) <= 2 * dt
+        assert abs(eps - e) < 2e-3 * e
+    t, eps, tc, dt = collide(orc, 1.0, move2)      # no dashpot: elastic
+    assert abs(eps - 1.0) < 1e-3
+    assert abs(t - tc) <= 2 * dt
+
+
+def test_lsd_static_spring_and_newton(orc, rng):
+    for delta in (1e-6, 3e-5):
+        r, v, w, rad, move = two_spheres(delta, 1e-3, 1e-3, d=rng.normal(size=3))
+        F, T, h, bad = lsd(orc, r, v, w, rad, move)
+        n = (r[0] - r[1]) / np.linalg.norm(r[0] - r[1])
+        assert np.allclose(F[0], KN * delta * n, rtol=1e-9, atol=0)
+        assert np.allclose(F[1], -F[0], rtol=1e-12, atol=0)
+        assert np.all(T == 0) and np.all(h == 0) and bad == 0
+    # moving, spinning, unequal spheres: F_0 = -F_1, T_0 / rad_0 = T_1 / rad_1
+    r, v, w, rad, move = two_spheres(2e-5, 1.1e-3, 0.9e-3, d=rng.normal(size=3))
+    v = rng.normal(size=(2, 3)) * 0.01
+    w = rng.normal(size=(2, 3)) * 10.0
+    F, T, h, _ = lsd(orc, r, v, w, rad, move, hist=rng.normal(size=(2, 3)) * 0, minv=(2e3, 3e3))
+    assert np.allclose(F[0], -F[1], rtol=1e-12, atol=1e-18)
+    assert np.allclose(T[0] / rad[0], T[1] / rad[1], rtol=1e-12, atol=1e-18)
+
+
+def test_lsd_stick_slip_and_torque(orc, rng):
+    """Tangential spring: with zero history the first step's force is
+    -kt vt dt, where vt is the tangential part of the contact points'
+    relative velocity computed from rigid-body kinematics; the spring then
+    grows linearly until |Ft| reaches mu |Fn|, after which it slides at the
+    cap against vt; the torque is the lever (-rad_i n) times Ft."""
+    dt = 1e-5
+    r, v, w, rad, move = two_spheres(1e-5, 1.0e-3, 0.8e-3, d=rng.normal(size=3))
+    v = rng.normal(size=(2, 3)) * 1e-3
+    w = rng.normal(size=(2, 3)) * 2.0
+    n = (r[0] - r[1]) / np.linalg.norm(r[0] - r[1])
+    pc0 = v[0] + np.cross(w[0], -rad[0] * n)        # contact point on sphere 0
+    pc1 = v[1] + np.cross(w[1], rad[1] * n)         # and on sphere 1
+    vc = pc0 - pc1
+    vt = vc - (vc @ n) * n
+    fn = KN * 1e-5 - 2 * orc.lsd_zeta(0.8) * np.sqrt(0.5e-3 * KN) * (vc @ n)
+    F, T, h, _ = lsd(orc, r, v, w, rad, move, minv=(1e3, 1e3), dt=dt)
+    Ft = F[0] - (F[0] @ n) * n
+    assert np.allclose(F[0] @ n, fn, rtol=1e-10)
+    assert np.allclose(Ft, -KT * vt * dt, rtol=1e-9, atol=1e-20)
+    assert np.allclose(h[0], vt * dt, rtol=1e-9, atol=1e-20)
+    assert np.allclose(T[0], np.cross(-rad[0] * n, Ft), rtol=1e-9, atol=1e-22)
+    # stick: the history keeps adding vt dt while below the cap (static w, v)
+    k_stick = int(MU_L * abs(fn) / (KT * np.linalg.norm(vt) * dt))
+    hist = h
+    for k in range(2, min(k_stick, 50)):
+        F, T, hist, _ = lsd(orc, r, v, w, rad, move, hist=hist, minv=(1e3, 1e3), dt=dt)
+        assert np.allclose(hist[0], k * vt * dt, rtol=1e-9, atol=1e-20)
+    # slip: a history far beyond the cap gives |Ft| = mu |Fn| against the
+    # spring, and the spring is reset to -Ft / kt
+    big = -np.cross(n, np.cross(n, rng.normal(size=3)))     # a tangential vector
+    big *= 10 * MU_L * abs(fn) / KT / np.linalg.norm(big)
+    F, T, h, _ = lsd(orc, r, v, w, rad, move, hist=np.array([big, -big]), minv=(1e3, 1e3),
+                     dt=dt)
+    Ft = F[0] - (F[0] @ n) * n
+    assert np.isclose(np.linalg.norm(Ft), MU_L * abs(F[0] @ n), rtol=1e-12)
+    s = big + vt * dt
+    assert np.allclose(Ft / np.linalg.norm(Ft), -s / np.linalg.norm(s), rtol=1e-10)
+    assert np.allclose(h[0], -Ft / KT, rtol=1e-12)
+
+
+def test_lsd_history_projection_and_bound(orc, rng):
+    # no relative motion: the history loses its normal component only
+    r, v, w, rad, move = two_spheres(1e-6, 1e-3, 1e-3, d=rng.normal(size=3))
+    n = (r[0] - r[1]) / np.linalg.norm(r[0] - r[1])
+    h0 = rng.normal(size=3) * 1e-9
+    _, _, h, _ = lsd(orc, r, v, w, rad, move, hist=np.array([h0, -h0]))
+    assert abs(h[0] @ n) < 1e-24
+    assert np.allclose(h[0], h0 - (h0 @ n) * n, rtol=1e-12, atol=1e-25)
+    # stability bound dt < 2 sqrt(meff / kn): meff = 0.5 g for two 1 g spheres
+    lim = 2 * np.sqrt(0.5e-3 / KN)
+    assert lsd(orc, r, v, w, rad, move, minv=(1e3, 1e3), dt=0.99 * lim)[3] == 0
+    assert lsd(orc, r, v, w, rad, move, minv=(1e3, 1e3), dt=1.01 * lim)[3] == 2
+
+
+def test_carry_width(orc, rng):
+    prow = np.array([0, 2, 3, 3], np.uint32)
+    pcol = np.array([1, 2, 0], np.uint32)
+    pval = rng.normal(size=3)
+    crow = np.array([0, 1, 3, 3], np.uint32)
+    ccol = np.array([2, 0, 2], np.uint32)
+    a = orc.dem_carry(prow, pcol, pval, crow, ccol)
+    b = orc.dem_carry_w(prow, pcol, pval, crow, ccol, 1)
+    assert np.array_equal(a, b) and np.array_equal(a, [pval[1], pval[2], 0.0])
+    p3 = rng.normal(size=(3, 3))
+    c3 = orc.dem_carry_w(prow, pcol, p3, crow, ccol, 3).reshape(-1, 3)
+    assert np.array_equal(c3, np.array([p3[1], p3[2], [0, 0, 0]]))
+
+
+def test_contacts_cells_equal_brute_force(orc):
+    rng = np.random.default_rng(3)
+    for n, box in ((3000, 12.0), (500, 5.0)):
+        r = rng.uniform(0, box, (n, 3))
+        r[:20] = rng.uniform(-0.3, box + 0.3, (20, 3))     # a few centres outside the box
+        r[20:40, 0] = box                                   # on the upper face
+        rad = rng.uniform(0.35, 0.6, n)
+        move = (rng.uniform(size=n) > 0.1).astype(np.uint8)
+        row, col = orc.dem_contacts(r, rad, move)
+        cell = np.array([0.0, box, 0.0, box, 0.0, box])
+        for cs in (1.2, 1.7):
+            rowc, colc = orc.dem_contacts_cells(r, rad, move, cell, cs)
+            assert np.array_equal(row, rowc) and np.array_equal(col, colc)
+        assert len(col) > n
