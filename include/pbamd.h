@@ -242,6 +242,18 @@ int pb_lj_set_tiles(pb_sys *sys, uint32_t tile_w, double tile_fill);
 int pb_gr_hist(pb_sys *sys, const float *rstep, uint32_t rnum,
                uint64_t *hist);
 
+/* pb_gr_hist without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and errors as pb_gr_hist; the replica table and
+ * the histograms live in a static device array of the library (one per
+ * device, about 1.1 MB, held until the process ends), so nrep <= 4096 and
+ * nrep * rnum <= 131072 (PB_ERR_ARG beyond).  Every call on a device shares
+ * that array: calls that overlap on one device (two host threads, two
+ * contexts of one device) race and must be serialised by the caller. */
+int pb_gr_hist_noalloc(pb_sys *sys, const float *rstep, uint32_t rnum,
+                       uint64_t *hist);
+
 /* ------------------------------------------------------------------- SPH */
 /* Reference physics of cudaParticleSPH_NS: Lucy kernel of support h,
  * p = c^2 (rho - rho0) with rho0 = 1, physical viscosity mu; non-periodic

@@ -52,7 +52,7 @@ EXPORTS = [
     "pb_gt_step", "pb_gt_xi", "pb_adjust_velocities", "pb_const_temp",
     "pb_nlist_build", "pb_nlist_get", "pb_lj_force_nlist", "pb_softcore_force", "pb_euler_step",
     "pb_dem_set_contacts", "pb_lj_set_tiles", "pb_dem_rewind", "pb_dem_set_linear",
-    "pb_dem_history_width",
+    "pb_dem_history_width", "pb_gr_hist_noalloc",
 ]
 
 
@@ -168,6 +168,7 @@ def _extra_signatures():
         "pb_dem_rewind": (i32, [vp]),
         "pb_dem_set_linear": (i32, [vp, d, d, d, d, P(d), d]),
         "pb_dem_history_width": (i32, [vp, P(u32)]),
+        "pb_gr_hist_noalloc": (i32, [vp, P(C.c_float), u32, P(C.c_uint64)]),
         "pb_lj_autotune": (i32, [vp, P(pb_run_opts), u32, P(d), P(u32), P(d)]),
     }
 

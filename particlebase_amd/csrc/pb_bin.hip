@@ -445,6 +445,116 @@ extern "C" int pb_gr_hist(pb_sys *s, const float *rstep, uint32_t rnum, uint64_t
   return PB_OK;
 }
 
+// ---------------------------------------- g(r) without allocations
+// pb_gr_hist_noalloc keeps its replica table and histograms in this
+// file-scope device array instead of library-allocated memory: each device
+// holds its own copy for the life of the process.  Host code never passes
+// the array's name to a kernel (on the host it is only a stand-in symbol);
+// it uses the device address that pb_gr_scratch_init stores in the context.
+#define GR_NA_MAX_REPS 4096u                          // replicas per call
+#define GR_NA_MAX_SLOTS 131072u                       // nrep * rnum histogram slots
+#define GR_NA_REP_BYTES ((size_t)GR_NA_MAX_REPS * sizeof(GrRep))
+#define GR_NA_RSTEP_BYTES ((size_t)GR_NA_MAX_REPS * sizeof(float))
+#define GR_NA_BYTES (GR_NA_REP_BYTES + GR_NA_RSTEP_BYTES + (size_t)GR_NA_MAX_SLOTS * 8)
+static __device__ __align__(256) unsigned char scratch[GR_NA_BYTES];
+
+// the device address of `scratch` on the context's (current) device
+int pb_gr_scratch_init(pb_ctx *c) {
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) return pb_hip_check(c, e != hipSuccess ? e : hipErrorUnknown,
+                                                   "hipGetSymbolAddress(g(r) scratch)");
+  c->gr_scratch = ptr;
+  return PB_OK;
+}
+
+// replica table of the g(r) kernel from the device replica records: one
+// workgroup; per-replica tile-pair counts scanned in order (uint64)
+__global__ void __launch_bounds__(256) k_gr_prep(const RepInfo *reps, const float *rstep,
+                                                 uint32_t nrep, GrRep *out) {
+  __shared__ unsigned long long part[256];
+  const uint32_t per = (nrep + 255u) / 256u;
+  const uint32_t q0 = threadIdx.x * per, q1 = min(nrep, q0 + per);
+  unsigned long long mine = 0;
+  for (uint32_t q = q0; q < q1; ++q) {
+    const unsigned long long nt = (reps[q].natoms + GR_T - 1) / GR_T;
+    mine += nt * (nt + 1) / 2;
+  }
+  part[threadIdx.x] = mine;
+  __syncthreads();
+  unsigned long long base = 0;
+  for (uint32_t k = 0; k < threadIdx.x; ++k) base += part[k];
+  for (uint32_t q = q0; q < q1; ++q) {
+    const RepInfo &R = reps[q];
+    GrRep g;
+    g.atom0 = R.atom0;
+    g.natoms = R.natoms;
+    g.ntile = (R.natoms + GR_T - 1) / GR_T;
+    g.pair0 = base;
+    base += (unsigned long long)g.ntile * (g.ntile + 1) / 2;
+    // cudaFrame::setCell (cudaFrame.cu:85-91) on the FP32 cell values, as
+    // the host-side table of pb_gr_hist
+    float c[3];
+    for (int a = 0; a < 3; ++a) {
+      const float lo = (float)R.lo[a], hi = (float)R.hi[a];
+      c[a] = (hi - lo) * 0.5f;            // = (hi - lo) / 2 exactly
+      c[a] *= c[a];
+    }
+    g.cx = c[0];
+    g.cy = c[1];
+    g.cz = c[2];
+    g.rstepinv = (float)(1.0 / (double)rstep[q]);
+    out[q] = g;
+  }
+}
+
+__global__ void k_zero_u64(uint64_t *x, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = 0;
+}
+
+extern "C" int pb_gr_hist_noalloc(pb_sys *s, const float *rstep, uint32_t rnum, uint64_t *hist) {
+  if (!s || !rstep || !hist || rnum == 0) return PB_ERR_ARG;
+  if (rnum > 16384) return pb_fail(s->ctx, PB_ERR_ARG, "pb_gr_hist_noalloc: rnum > 16384");
+  if (!s->cell_set) return pb_fail(s->ctx, PB_ERR_STATE, "pb_gr_hist_noalloc: cell not set");
+  if (s->nrep > GR_NA_MAX_REPS)
+    return pb_fail(s->ctx, PB_ERR_ARG, "pb_gr_hist_noalloc: more than 4096 replicas");
+  if ((uint64_t)s->nrep * rnum > GR_NA_MAX_SLOTS)
+    return pb_fail(s->ctx, PB_ERR_ARG,
+                   "pb_gr_hist_noalloc: replicas x bins above the scratch limit of 131072");
+  if (!s->ctx->gr_scratch)
+    return pb_fail(s->ctx, PB_ERR_STATE, "pb_gr_hist_noalloc: no scratch address");
+  for (uint32_t q = 0; q < s->nrep; ++q)
+    if (!(rstep[q] > 0.0f)) return pb_fail(s->ctx, PB_ERR_ARG, "pb_gr_hist_noalloc: rstep <= 0");
+  // tile pairs of all replicas (host replica records, no allocation)
+  uint64_t tot = 0;
+  for (uint32_t q = 0; q < s->nrep; ++q) {
+    const uint64_t nt = (s->reps[q].natoms + GR_T - 1) / GR_T;
+    tot += nt * (nt + 1) / 2;
+  }
+  unsigned char *base = static_cast<unsigned char *>(s->ctx->gr_scratch);
+  GrRep *dreps = reinterpret_cast<GrRep *>(base);
+  float *drstep = reinterpret_cast<float *>(base + GR_NA_REP_BYTES);
+  uint64_t *dhist = reinterpret_cast<uint64_t *>(base + GR_NA_REP_BYTES + GR_NA_RSTEP_BYTES);
+  const uint32_t hn = s->nrep * rnum;
+  hipStream_t st = s->ctx->stream;
+  hipError_t e = hipMemcpyAsync(drstep, rstep, sizeof(float) * s->nrep, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_gr_prep, dim3(1), dim3(256), 0, st, s->d_reps, drstep, s->nrep, dreps);
+    hipLaunchKernelGGL(k_zero_u64, dim3(pb_blocks(hn, 256)), dim3(256), 0, st, dhist, hn);
+    if (tot) {
+      const uint32_t grid = (uint32_t)std::min<uint64_t>(tot, 8192);
+      hipLaunchKernelGGL(k_gr, dim3(grid), dim3(GR_T), sizeof(uint32_t) * rnum, st, s->r, dreps,
+                         s->nrep, tot, rnum, dhist);
+    }
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(hist, dhist, sizeof(uint64_t) * hn, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return pb_hip_check(s->ctx, e, "pb_gr_hist_noalloc");
+  return PB_OK;
+}
+
 __global__ void k_clear4_bin(double4 *x, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = make_double4(0.0, 0.0, 0.0, 0.0);

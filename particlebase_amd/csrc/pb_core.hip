@@ -45,6 +45,11 @@ extern "C" int pb_ctx_create(int device, pb_ctx **out) {
     delete c;
     return PB_ERR_HIP;
   }
+  if (pb_gr_scratch_init(c) != PB_OK) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PB_ERR_HIP;
+  }
   *out = c;
   return PB_OK;
 }

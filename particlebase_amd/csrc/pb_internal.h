@@ -46,6 +46,9 @@ struct LJTable {
 struct pb_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // device address of pb_bin.hip's static g(r) scratch array on this
+  // context's device (pb_gr_hist_noalloc); set when the context is created
+  void *gr_scratch = nullptr;
   std::string err;
   int cu_count = 0;
   std::string name;
@@ -226,6 +229,7 @@ int pb_reduce_waves(pb_sys *sys, int K);
 std::vector<uint32_t> pb_download_orig(pb_sys *sys, int *rc);
 // rotation / radius state, allocated on first use
 int pb_ensure_rotation(pb_sys *sys);
+int pb_gr_scratch_init(pb_ctx *ctx);          // pb_bin.hip: device address of the g(r) scratch
 int pb_ensure_radius(pb_sys *sys);
 
 // ------------------------------------------------------------ device util
