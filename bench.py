@@ -59,7 +59,14 @@ def make_system(ctx, w_or_batch):
     return s
 
 
-def timed_run(ctx, s, steps, precision, skin, rebuild_every, energy_every, timing=True):
+# Timed runs time every TIMING_STRIDE-th step kernel, the step after each
+# rebuild and every rebuild with HIP events (the kernel's own dispatch
+# timestamps on the library's stream); a timed dispatch idles the GPU about
+# 9 us before it, so timing every step would slow the run it measures.
+TIMING_STRIDE = 4
+
+
+def timed_run(ctx, s, steps, precision, skin, rebuild_every, energy_every, timing=TIMING_STRIDE):
     ctx.sync()
     t0 = time.perf_counter()
     e, rc = s.lj_run(DT, steps, precision=precision, skin=skin, rebuild_every=rebuild_every,
@@ -120,13 +127,32 @@ def bench_gr(sb, reps):
     return {"pairs": pairs, "ms": 1e3 * elg, "pairs_per_s": pairs / elg, "bins": 200}
 
 
+def host_info():
+    """This host's CPU: model name, logical CPUs, the CPUs this process may
+    use (affinity) and the thread count of the lease (OMP_NUM_THREADS)."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    omp = os.environ.get("OMP_NUM_THREADS")
+    lease = min(aff, int(omp)) if omp and omp.isdigit() else aff
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": aff,
+            "omp_num_threads": omp, "lease_threads": lease}
+
+
 def cpu_baseline(w, budget_s=20.0, cores=0):
     """The oracle's restatement of the reference loop (rebin, 27-cell forces,
     VV, wrap; FP64, OpenMP) on this host, on a bounded sample of steps;
-    `cores` threads (0: the host's share, at most 16)."""
+    `cores` threads (0: every thread of the lease, host_info()["lease_threads"])."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pb_oracle as orc
-    cores = cores or min(os.cpu_count() or 1, 16)
+    cores = cores or host_info()["lease_threads"]
     p = orc.lj_params([1.0, 1.0], 1, RC2, True)
     minv = 1.0 / w.m
     t0 = time.perf_counter()
@@ -184,9 +210,12 @@ def bench_sph(ctx, a, config=False):
     return {"workload": f"SPH dam break, {kind}, dx={w.dx:.6g}, {int((w.fixed == 0).sum())} "
                         f"fluid + {int(w.fixed.sum())} wall particles, FP64",
             "value": v, "unit": "particle-steps/s", "ms_per_step": 1e3 * el / a.sph_steps,
+            "roofline_frac": SPH_ALG_BYTES * v / 1e9 / HBM_PEAK_GBS,
+            "alg_bytes": f"{SPH_ALG_BYTES:g} B per particle-step",
             "steps": a.sph_steps}
 
 
+SPH_ALG_BYTES = 121.0        # per particle-step: FP64 r, v read + written, type/move, rho/1/rho/mu
 DEM_ALG_BYTES = 156.0        # per particle-step: FP64 r, v, w/L read + written, radius, mass
 DEM_ALG_BYTES_CONTACT = 52.0  # per contact: column index + 3-double history in and out
 
@@ -275,6 +304,8 @@ def main():
                     help="only a device copy of 256 MiB, 5 times (HBM counter calibration)")
     ap.add_argument("--gr-only", action="store_true",
                     help="only g(r) of the 256 batched replicas, 3 times (for counter runs)")
+    ap.add_argument("--sph-only", action="store_true",
+                    help="only the two SPH dam-break legs (for kernel traces / counter runs)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {a.gpus} is not supported "
@@ -287,6 +318,12 @@ def main():
         reps = workloads.lj_replicas(256, 500, 20000, seed=5)
         sb = make_system(ctx, reps)
         log(f"g(r) only: {bench_gr(sb, reps)}")
+        return
+
+    if a.sph_only:
+        ctx = pb.Context(0)
+        log(f"sph only: {bench_sph(ctx, a)}")
+        log(f"sph only (config physics): {bench_sph(ctx, a, config=True)}")
         return
 
     if a.copy_probe:
@@ -306,18 +343,24 @@ def main():
     w = workloads.lj_fcc(16 if a.quick else 64, 0.8442, 0.72, seed=2)
     log(f"workload N={w.n} L={w.L:.4f}")
 
-    # ---- mixed (chosen) form
+    # ---- mixed (chosen) form: tile shape first (the autotuner restores the
+    # state it starts from), then the warm-up, then the timed steps, which
+    # continue the warm-up's list and cell order
     s = make_system(ctx, w)
-    timed_run(ctx, s, a.warmup, pb.PB_MIXED, a.skin, a.rebuild_every, 0, timing=False)
     tiles = choose_tiles(s, a, pb.PB_MIXED)
     log(f"tiles: {tiles}")
+    timed_run(ctx, s, a.warmup, pb.PB_MIXED, a.skin, a.rebuild_every, 0, timing=False)
     el, e = timed_run(ctx, s, a.steps, pb.PB_MIXED, a.skin, a.rebuild_every, 10)
     tm = s.lj_run_timing()
     stats = s.lj_run_stats()
     value = w.n * a.steps / el
     kern_ms = tm["step_ms"] / max(tm["nstep"], 1)
     achieved = ALG_BYTES * w.n / (kern_ms * 1e-3) / 1e9
-    drift = drift_per_1e4(e, w.n, 10)
+    if a.profile:
+        return
+    # energy drift of the same system: 100 more steps with E recorded every step
+    _, e1 = timed_run(ctx, s, 100, pb.PB_MIXED, a.skin, a.rebuild_every, 1, timing=False)
+    drift = drift_per_1e4(e1, w.n, 1)
     log(f"mixed: {value:.4g} atom-steps/s, {1e3 * el / a.steps:.3f} ms/step, "
         f"step kernel {kern_ms * 1e3:.1f} us, rebuild {tm['rebuild_ms'] / max(tm['nrebuild'], 1):.3f} ms,"
         f" {stats}")
@@ -328,33 +371,37 @@ def main():
     f32 = s.get("F")
     frms = float(np.sqrt((f64 ** 2).sum(axis=1).mean()))
     ferr = float(np.abs(f32 - f64).max() / frms)
-    if a.profile:
-        return
-    # the same run from host buffers to host buffers: import (host -> HBM,
-    # cell order restored by the next rebuild) before and positions and
-    # velocities back to the host after the timed steps
+    # the same number of steps from host buffers to host buffers: import
+    # (host -> HBM), the run (its list and cell order built from scratch) and
+    # positions and velocities back to the host, all timed together
     ctx.sync()
     t0 = time.perf_counter()
     s.import_particles(w.r, w.m, v=w.v, type=w.type)
     ctx.sync()
     t_in = time.perf_counter() - t0
-    t0 = time.perf_counter()
+    s.lj_run(DT, a.steps, precision=pb.PB_MIXED, skin=a.skin, rebuild_every=a.rebuild_every,
+             energy_every=0)
+    ctx.sync()
+    t_run = time.perf_counter() - t0 - t_in
+    t1 = time.perf_counter()
     s.get("r")
     s.get("v")
-    t_out = time.perf_counter() - t0
-    host_io = {"value": w.n * a.steps / (el + t_in + t_out), "unit": "atom-steps/s",
-               "import_ms": 1e3 * t_in, "get_r_v_ms": 1e3 * t_out, "steps": a.steps}
+    t_out = time.perf_counter() - t1
+    host_io = {"value": w.n * a.steps / (t_in + t_run + t_out), "unit": "atom-steps/s",
+               "import_ms": 1e3 * t_in, "run_ms": 1e3 * t_run, "get_r_v_ms": 1e3 * t_out,
+               "steps": a.steps}
     log(f"host-buffer inclusive: {host_io}")
     del s
 
     # ---- plain FP64 form
     s = make_system(ctx, w)
-    timed_run(ctx, s, a.warmup, pb.PB_FP64, a.skin, a.rebuild_every, 0, timing=False)
     tiles64 = choose_tiles(s, a, pb.PB_FP64)
+    timed_run(ctx, s, a.warmup, pb.PB_FP64, a.skin, a.rebuild_every, 0, timing=False)
     el64, e64 = timed_run(ctx, s, a.steps, pb.PB_FP64, a.skin, a.rebuild_every, 10)
     tm64 = s.lj_run_timing()
     value64 = w.n * a.steps / el64
     kern64 = tm64["step_ms"] / max(tm64["nstep"], 1)
+    _, e64 = timed_run(ctx, s, 100, pb.PB_FP64, a.skin, a.rebuild_every, 1, timing=False)
     log(f"fp64: {value64:.4g} atom-steps/s, step kernel {kern64 * 1e3:.1f} us")
     del s
 
@@ -374,10 +421,12 @@ def main():
         "plain_fp64": {"value": value64, "ms_per_step": 1e3 * el64 / a.steps, "tiles": tiles64,
                        "kernel_us": kern64 * 1e3,
                        "roofline_frac": ALG_BYTES * w.n / (kern64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                       "energy_drift_per_1e4_steps": drift_per_1e4(e64, w.n, 10),
+                       "energy_drift_per_1e4_steps": drift_per_1e4(e64, w.n, 1),
                        "drift_n4000_1e4_steps": None if a.no_drift else
                        drift_direct(ctx, pb.PB_FP64, a.skin, a.rebuild_every)},
         "mixed": {"force_err_max_over_frms": ferr, "energy_drift_per_1e4_steps": drift,
+                  "energy_drift_run": "100 steps after the timed ones, E recorded every step, "
+                                      "fitted slope of E/N",
                   "drift_n4000_1e4_steps": None if a.no_drift else
                   drift_direct(ctx, pb.PB_MIXED, a.skin, a.rebuild_every),
                   "tiles": tiles,
@@ -391,11 +440,47 @@ def main():
         reps = workloads.lj_replicas(16 if a.quick else 256, 500, 20000, seed=5)
         ntot = int(reps.counts.sum())
         sb = make_system(ctx, reps)
-        timed_run(ctx, sb, 10, pb.PB_MIXED, a.skin, a.rebuild_every, 0, timing=False)
         tiles_b = choose_tiles(sb, a, pb.PB_MIXED)
+        timed_run(ctx, sb, 10, pb.PB_MIXED, a.skin, a.rebuild_every, 0, timing=False)
         elb, eb = timed_run(ctx, sb, a.batch_steps, pb.PB_MIXED, a.skin, a.rebuild_every, 0)
         vb = ntot * a.batch_steps / elb
-        log(f"batched: {len(reps.systems)} replicas, {ntot} atoms, {vb:.4g} atom-steps/s")
+        nreb = sb.lj_run_stats()["rebuilds"]
+        log(f"batched (eager, host-polled rebuilds): {len(reps.systems)} replicas, {ntot} atoms, "
+            f"{vb:.4g} atom-steps/s, {nreb} rebuilds")
+        # the hipGraph-captured form: periods of R steps + the rebuild,
+        # captured once and replayed.  The interval is fixed, so it is taken
+        # from the eager run's rebuild rate with a 30 % margin (the hottest
+        # replicas set it; a skin violation would raise)
+        rg = max(2, min(a.rebuild_every, int(0.7 * a.batch_steps / max(nreb, 1))))
+        ctx.sync()
+        t0 = time.perf_counter()
+        sb.lj_run(DT, a.batch_steps, precision=pb.PB_MIXED, skin=a.skin,
+                  rebuild_every=rg, energy_every=0, use_graph=True)
+        ctx.sync()
+        elg = time.perf_counter() - t0
+        vg = ntot * a.batch_steps / elg
+        log(f"batched (hipGraph replay): {vg:.4g} atom-steps/s")
+        # the same with the persistent step kernel (per-XCD work queues over
+        # the replica x tile segments) in the captured periods
+        ctx.sync()
+        t0 = time.perf_counter()
+        sb.lj_run(DT, a.batch_steps, precision=pb.PB_MIXED, skin=a.skin,
+                  rebuild_every=rg, energy_every=0, use_graph=True, persistent=True)
+        ctx.sync()
+        elp = time.perf_counter() - t0
+        vgp = ntot * a.batch_steps / elp
+        log(f"batched (hipGraph replay, persistent step kernel): {vgp:.4g} atom-steps/s")
+        # per-replica mixed force error against the FP64 kernel
+        sb.lj_force(pb.PB_FP64, energies=False)
+        fb64 = sb.get("F")
+        sb.lj_force(pb.PB_MIXED, energies=False)
+        fb32 = sb.get("F")
+        off = 0
+        berr = []
+        for nk in reps.counts:
+            f6, f3 = fb64[off:off + nk], fb32[off:off + nk]
+            berr.append(float(np.abs(f3 - f6).max() / np.sqrt((f6 ** 2).sum(1).mean())))
+            off += nk
         gr = bench_gr(sb, reps)
         log(f"g(r): {gr['pairs']} pairs in {gr['ms']:.2f} ms, {gr['pairs_per_s']:.4g} pairs/s")
         del sb
@@ -413,8 +498,14 @@ def main():
         vp = ntot * a.plain_steps / elp
         del plain_systems
         log(f"plain sequential FP64: {vp:.4g} atom-steps/s")
-        out["batched"] = {"replicas": len(reps.systems), "atoms": ntot, "value": vb, "tiles": tiles_b,
-                          "plain_sequential_fp64": vp, "speedup": vb / vp,
+        vbest = max(vb, vg, vgp)
+        out["batched"] = {"replicas": len(reps.systems), "atoms": ntot, "value": vbest,
+                          "eager": vb, "graph": vg, "graph_persistent": vgp, "tiles": tiles_b,
+                          "eager_rebuilds": nreb, "graph_rebuild_every": rg,
+                          "force_err_max_over_frms": max(berr),
+                          "force_err_median_over_frms": float(np.median(berr)),
+                          "force_err_reference": "GPU FP64 kernel, per replica F_rms",
+                          "plain_sequential_fp64": vp, "speedup": vbest / vp,
                           "steps": a.batch_steps, "plain_steps": a.plain_steps, "gr": gr}
 
     if not a.no_sph:
@@ -424,9 +515,11 @@ def main():
         out["dem"] = bench_dem(a)
         out["dem_hertz"] = bench_dem(a, law="hertz")
 
+    out["host"] = host_info()
     if not a.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(w, budget_s=5.0 if a.quick else 20.0)
+        out["cpu_baseline"]["host"] = out["host"]["cpu_model"]
         c1 = cpu_baseline(w, budget_s=3.0 if a.quick else 8.0, cores=1)
         out["cpu_baseline"]["single_core"] = {"value": c1["value"], "sample": c1["sample"]}
         log(f"cpu: {out['cpu_baseline']}")

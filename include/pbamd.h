@@ -189,8 +189,14 @@ typedef struct {
   uint32_t energy_every;   /* energies recorded every k steps (0 = never) */
   int      use_graph;      /* capture the step sequence in a hipGraph */
   uint32_t block_size;     /* threads per workgroup of the pair kernel (0 = auto) */
-  int      timing;         /* 1: time every step / rebuild with HIP events on the
-                              context's stream (pb_lj_last_run_timing) */
+  int      timing;         /* k > 0: time every k-th step, the step after each
+                              rebuild, and every rebuild with HIP events on the
+                              context's stream (pb_lj_last_run_timing); a timed
+                              step costs about 9 us of idle GPU before it */
+  int      persistent;     /* 1: the step kernel runs as a persistent grid (two
+                              workgroups per compute unit) draining the
+                              (replica x tile) segments from per-XCD work queues;
+                              0: one workgroup per segment */
 } pb_run_opts;
 
 /* Fused cudaParticleMD loop (the testMD1.cu:135-169 step: calcBlockID,
@@ -213,8 +219,8 @@ int pb_lj_last_run_stats(pb_sys *sys, uint32_t *rebuilds, uint32_t *max_nn,
                          int *skin_violation);
 
 /* (no reference counterpart) GPU time of the last pb_lj_run with
- * opts.timing = 1: summed HIP-event time
- * of the fused step kernels and of the list rebuilds, and their counts. */
+ * opts.timing > 0: summed HIP-event time of the timed step kernels and of
+ * the list rebuilds, and their counts. */
 int pb_lj_last_run_timing(pb_sys *sys, double *step_ms, uint32_t *nstep,
                           double *rebuild_ms, uint32_t *nrebuild);
 

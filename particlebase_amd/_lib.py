@@ -60,7 +60,7 @@ class pb_run_opts(C.Structure):
     _fields_ = [("dt", C.c_double), ("nsteps", C.c_uint32), ("precision", C.c_int),
                 ("skin", C.c_double), ("rebuild_every", C.c_uint32),
                 ("energy_every", C.c_uint32), ("use_graph", C.c_int),
-                ("block_size", C.c_uint32), ("timing", C.c_int)]
+                ("block_size", C.c_uint32), ("timing", C.c_int), ("persistent", C.c_int)]
 
 
 class PBError(RuntimeError):
@@ -470,9 +470,10 @@ class System:
         return out
 
     def lj_run(self, dt, nsteps, precision=PB_MIXED, skin=0.3, rebuild_every=10,
-               energy_every=1, use_graph=False, block_size=0, timing=False, check=True):
+               energy_every=1, use_graph=False, block_size=0, timing=False, check=True,
+               persistent=False):
         o = pb_run_opts(dt, nsteps, precision, skin, rebuild_every, energy_every,
-                        int(use_graph), block_size, int(timing))
+                        int(use_graph), block_size, int(timing), int(persistent))
         nrec = (nsteps + energy_every - 1) // energy_every if energy_every else 0
         e = np.zeros((max(nrec, 1), self.nrep, 3))
         rc = self.lib.pb_lj_run(self.h, C.byref(o), _p(e, C.c_double))
@@ -480,8 +481,9 @@ class System:
             self._c(rc)
         return e[:nrec], rc
 
-    def lj_autotune(self, dt, steps=10, precision=PB_MIXED, skin=0.3, rebuild_every=20):
-        o = pb_run_opts(dt, steps, precision, skin, rebuild_every, 0, 0, 0, 1)
+    def lj_autotune(self, dt, steps=10, precision=PB_MIXED, skin=0.3, rebuild_every=20,
+                    persistent=False):
+        o = pb_run_opts(dt, steps, precision, skin, rebuild_every, 0, 0, 0, 1, int(persistent))
         ms, w, f = C.c_double(), C.c_uint32(), C.c_double()
         self._c(self.lib.pb_lj_autotune(self.h, C.byref(o), steps, C.byref(ms), C.byref(w),
                                         C.byref(f)))

@@ -192,6 +192,7 @@ extern "C" void pb_sys_destroy(pb_sys *s) {
   pb_free(s->w); pb_free(s->Lm); pb_free(s->T); pb_free(s->radius);
   pb_free(s->move); pb_free(s->type); pb_free(s->orig); pb_free(s->waveRep);
   pb_free(s->d_reps); pb_free(s->partials); pb_free(s->d_counter); pb_free(s->d_out);
+  pb_free(s->d_hash); pb_free(s->lj_queue);
   pb_free(s->gt); pb_free(s->gt_bad);
   pb_free(s->md_row); pb_free(s->md_col); pb_free(s->md_inv);
   pb_free(s->ref_table); pb_free(s->ref_bid); pb_free(s->ref_pid); pb_free(s->ref_bindex);

@@ -125,6 +125,20 @@ struct pb_sys {
   uint32_t list_nsegs = 0;
   uint32_t tile_w = 2;             // list-cell columns per tile side (1 or 2)
   double tile_fill = 0.72;         // target fill of the LDS window stage
+  // Continuation of the fused LJ run: the cell order, Verlet list and list
+  // cells left by the last pb_lj_run stay valid for the next one when the
+  // positions and the storage order are exactly those that run left (a
+  // fingerprint of r and orig) and the list parameters are unchanged.  Any
+  // sort (pb_counting_sort) clears lj_cont.
+  bool lj_cont = false;
+  uint64_t lj_cont_hash = 0;       // fingerprint of r, orig at the end of the run
+  double lj_cont_rc2 = 0, lj_cont_fill = 0;
+  uint32_t lj_cont_tile_w = 0;
+  std::vector<double> lj_cont_skins;
+  uint32_t lj_cont_since = 0;      // steps since the list was built
+  uint32_t lj_cont_d2 = 0, lj_cont_v2 = 0;   // displacement / speed maxima (float bits)
+  unsigned long long *d_hash = nullptr;
+  uint32_t *lj_queue = nullptr;    // persistent step kernel: 8 work queues + exit counter
   uint32_t last_rebuilds = 0, last_maxnn = 0;
   int last_skin_violation = 0;
   double last_step_ms = 0.0, last_rebuild_ms = 0.0;

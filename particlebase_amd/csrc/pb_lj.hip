@@ -15,6 +15,7 @@
 // partial-force buffer, the reduce27 pass and every host synchronisation of
 // the reference loop are gone.
 #include "pb_internal.h"
+#include <hip/hip_ext.h>
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -148,7 +149,7 @@ __global__ void k_reorder(ReorderArgs A) {
   }
   A.r2[k] = p;
   A.v2[k] = A.v[s];
-  A.a2[k] = A.a[s];
+  if (A.a) A.a2[k] = A.a[s];
   A.orig2[k] = o;
   A.cell2[k] = A.cellid[s];
   A.type2[k] = A.type[s];
@@ -170,6 +171,12 @@ struct ListSeg {
   uint32_t rep, ci, cj, ck0, nz, tx, ty, whole;
 };
 #define TB 384            // list-build workgroup (6 waves)
+#ifndef BUILD_GROUP
+#define BUILD_GROUP 32u   // candidates tested per mask step of the list build (<= 32)
+#endif
+#ifndef BUILD_UNROLL
+#define BUILD_UNROLL 4
+#endif
 #define TBS 512           // step workgroup (8 waves; two per CU at <= 128 VGPRs)
 #define WCAP 2816         // staged window particles per workgroup (LDS)
 #define MAXCOL 16         // (tx+2)*(ty+2) window columns, tx, ty <= 2
@@ -184,7 +191,28 @@ struct WinTable {
   uint32_t nrun, rloc[4], rpref[5], rg0[4], rcol[4];
   uint32_t wsum[TBS / 64];
 };
-#define NL_PAD 0xFFFFu     // list padding entry (maps to an inert slot)
+// Two list formats, chosen per segment by the same test in the build and the
+// step kernel: segments whose window is staged in LDS and that need no
+// minimum image (the fast path) store 12-bit entries, five to a word; the
+// others (windows larger than the stage, whole-replica segments) store
+// 16-bit entries, four to a word.
+// Padding entries: WCAP in 12-bit rows (the inert LDS slot, so the step
+// kernel uses every entry as it stands), all ones in 16-bit rows (tested
+// for).  A batch of the step kernel is one word per lane: 5 * LPA or 4 * LPA
+// entries; rows are padded to whole batches.
+#define NL_PAD 0xFFFFu     // 16-bit list padding entry
+#define NL_PADW12 ((uint64_t)WCAP * 0x1001001001001ull)   // a 12-bit word of five WCAP entries
+static_assert(WCAP < 0xFFF, "12-bit list entries address the staged window and its inert slot");
+// Step kernel: lanes per particle, each taking every LPA-th list word.  One
+// lane (64 particles a wavefront) measured 125 us a step at 1M atoms
+// against 131 us with two lanes, 151 with four and 178 with eight: rows
+// are padded to 5 * LPA entries, and with fewer lanes to a row less of each
+// wavefront idles on padding, more than making up for the wavefront
+// running its longest row.
+#ifndef LPA
+#define LPA 1u
+#endif
+#define NL_ROW (20u * LPA) // list capacity granule: a multiple of both batch sizes
 #define PAD_FAR 1.0e10     // coordinate of that slot
 struct Stage {              // pair kernel: FP64 positions with shifts applied
   double x[WCAP + 1], y[WCAP + 1], z[WCAP + 1];   // slot WCAP: padding target
@@ -307,8 +335,8 @@ __device__ __forceinline__ uint32_t window_entry(const WinTable &W, uint32_t q) 
 }
 
 // image-shifted FP64 position of window particle q read from global memory
-__device__ __forceinline__ double4 window_global(const WinTable &W, const RepInfo &R,
-                                                 const double4 *r, uint32_t q) {
+__device__ __forceinline__ double4 window_global(const WinTable &W, const ListSeg &G,
+                                                 const RepInfo &R, const double4 *r, uint32_t q) {
   const uint32_t e = window_entry(W, q);
   const uint32_t sh = W.esh[e];
   double4 p = r[W.eg0[e] + (q - W.eoff[e])];
@@ -361,15 +389,17 @@ __device__ void stage_window(Stage &S, const ListSeg &G, const RepInfo &R, const
 
 // position of window-local particle q (from LDS, or from global memory when
 // the window did not fit)
-__device__ __forceinline__ double4 window_pos(const Stage &S, const RepInfo &R, const double4 *r,
-                                              bool staged, uint32_t q) {
+__device__ __forceinline__ double4 window_pos(const Stage &S, const ListSeg &G, const RepInfo &R,
+                                              const double4 *r, bool staged, uint32_t q) {
   if (staged) {
     q = q == NL_PAD ? WCAP : q;
     return make_double4(S.x[q], S.y[q], S.z[q], (double)S.t[q]);
   }
   if (q == NL_PAD) return make_double4(PAD_FAR, PAD_FAR, PAD_FAR, 0);
-  return window_global(S.W, R, r, q);
+  return window_global(S.W, G, R, r, q);
 }
+
+template <uint32_t V> struct UConst { static constexpr uint32_t value = V; };
 
 // tile particle u (0 <= u < rpref[nrun]) -> window-local index, storage slot, run
 __device__ __forceinline__ void tile_particle(const WinTable &W, uint32_t u, uint32_t &li,
@@ -388,12 +418,14 @@ __device__ __forceinline__ void tile_particle(const WinTable &W, uint32_t u, uin
 // window-local q with |r_iq| < rb, where rb = rc + skin + a margin covering
 // the FP32 rounding: the list is a superset of the FP64 pairs within
 // rc + skin.  A window larger than the stage reads FP64 from global memory;
-// a window of more than 65534 particles (u16 indices) raises CNT_WIN.
-// Layout: the entries of a slot are grouped in blocks of 4 (one 8-byte
-// word); block m of slot gi is word m * npad + gi, so the step kernel's lanes
-// load whole words and the 16 particles of a wavefront read adjacent words.
-// Lists are padded with NL_PAD to a multiple of 16 entries (nn holds the
-// padded length).
+// a window of more than 65534 particles raises CNT_WIN.
+// Layout: the entries of a slot are grouped in words of 8 bytes (five 12-bit
+// entries in staged segments, four 16-bit entries otherwise); word m of slot
+// gi is at m * npad + gi, so the step kernel's lanes load whole words and
+// the particles of a wavefront read adjacent words.  Rows are padded with
+// the format's inert entry to whole batches (nn holds the padded length).
+// 12-bit entries move 20 % fewer list bytes per step than 16-bit ones
+// (measured: the step kernel 159 -> 151 us at 1M atoms).
 // six wavefronts per SIMD (<= 80 VGPRs with the granule of 8): with the
 // 38.6 KB stage, four workgroups per CU
 __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_build_list(const double4 *r, const uint32_t *cstart,
@@ -415,6 +447,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
     return;
   }
   const bool staged = W.total <= WCAPB;
+  const bool fmt12 = staged && !G.whole;        // the step kernel's fast path
   const double rlist2 = R.rlist * R.rlist;
   const float rb2 = R.rb2;
   // window origin (FP64): lower corner of the first window cell
@@ -431,23 +464,29 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
   const float Lx = (float)R.L[0], Ly = (float)R.L[1], Lz = (float)R.L[2];
   const float Ix = (float)R.Linv[0], Iy = (float)R.Linv[1], Iz = (float)R.Linv[2];
   uint32_t mymax = 0;
+  // the format as compile-time constants (entries per word, bits per
+  // entry): the per-hit word arithmetic then has no division
+  auto tile_rows = [&](auto fmt) {
+  constexpr uint32_t epw = decltype(fmt)::value, eb = epw == 5u ? 12u : 16u;
+  constexpr uint64_t padw = epw == 5u ? NL_PADW12 : ~0ull;   // a word of padding entries
   for (uint32_t u = threadIdx.x; u < ntile; u += blockDim.x) {
     uint32_t li, gi, run;
     tile_particle(W, u, li, gi, run);
-    // Kept entries are packed four to a 64-bit word in registers; word m of
+    // Kept entries are packed epw to a 64-bit word in registers; word m of
     // slot gi is written whole at wrow[m * npad].  Words past the capacity
-    // land in the slack word wcap (the allocation holds maxnb + 16 entries
+    // land in the slack word wcap (the allocation holds maxnb + 48 entries
     // per slot); an overflow is reported and the list rebuilt larger.
     uint64_t *wrow = reinterpret_cast<uint64_t *>(nl) + gi;
-    const uint32_t wcap = maxnb >> 2;
+    const uint32_t wcap = maxnb / epw;
     uint64_t word = 0;
-    uint32_t cnt = 0;
+    uint32_t cnt = 0, kw = 0;
     auto append = [&](uint32_t q) {
-      word |= (uint64_t)q << (16u * (cnt & 3u));
+      word |= (uint64_t)q << (eb * kw);
       ++cnt;
-      if ((cnt & 3u) == 0u) {
-        wrow[(size_t)min((cnt >> 2) - 1u, wcap) * npad] = word;
+      if (++kw == epw) {
+        wrow[(size_t)min(cnt / epw - 1u, wcap) * npad] = word;
         word = 0;
+        kw = 0;
       }
     };
     // staged candidates [q0, q1) except li: 32 distance tests into a bit
@@ -468,12 +507,12 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
         }
         return dx * dx + dy * dy + dz * dz < rb2;
       };
-      for (uint32_t c0 = q0; c0 < q1; c0 += 32) {
+      for (uint32_t c0 = q0; c0 < q1; c0 += BUILD_GROUP) {
         uint32_t m = 0;
-#pragma unroll 4
-        for (uint32_t k = 0; k < 32; ++k) m |= (near(c0 + k) ? 1u : 0u) << k;
-        if (q1 - c0 < 32u) m &= (1u << (q1 - c0)) - 1u;
-        if (li - c0 < 32u) m &= ~(1u << (li - c0));   // li < c0 wraps past 32
+#pragma unroll BUILD_UNROLL
+        for (uint32_t k = 0; k < BUILD_GROUP; ++k) m |= (near(c0 + k) ? 1u : 0u) << k;
+        if (q1 - c0 < BUILD_GROUP) m &= (1u << (q1 - c0)) - 1u;
+        if (li - c0 < BUILD_GROUP) m &= ~(1u << (li - c0));   // li < c0 wraps past the group
         while (m) {
           append(c0 + (uint32_t)__builtin_ctz(m));
           m &= m - 1u;
@@ -482,9 +521,9 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
     };
     // window read from global memory (window larger than the stage)
     auto scan_global = [&](uint32_t q0, uint32_t q1, bool mi) {
-      const double4 pi = window_global(W, R, r, li);
+      const double4 pi = window_global(W, G, R, r, li);
       for (uint32_t q = q0; q < q1; ++q) {
-        const double4 pj = window_global(W, R, r, q);
+        const double4 pj = window_global(W, G, R, r, q);
         double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         if (mi) {
           dx -= R.L[0] * rint(dx * R.Linv[0]);
@@ -520,17 +559,21 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
           scan(W.eoff[e0], W.eoff[e0 + 3], false);   // three z slots
         }
     }
-    // close the partial word and pad to whole batches of 16 with the inert
-    // entry (nn holds the padded length)
+    // close the partial word and pad to whole batches with the inert entry
+    // (nn holds the padded length)
     const uint32_t used = cnt < maxnb ? cnt : maxnb;
-    const uint32_t padded = (used + 15u) & ~15u;
-    if ((cnt & 3u) != 0u && cnt < maxnb)
-      wrow[(size_t)(cnt >> 2) * npad] = word | (~0ull << (16u * (cnt & 3u)));
-    for (uint32_t m = (used + 3u) >> 2; m < (padded >> 2); ++m) wrow[(size_t)m * npad] = ~0ull;
+    const uint32_t batch = epw * LPA;
+    const uint32_t padded = (used + batch - 1u) / batch * batch;
+    if (kw != 0u && cnt < maxnb)
+      wrow[(size_t)(cnt / epw) * npad] = word | (padw & (~0ull << (eb * kw)));
+    for (uint32_t m = (used + epw - 1u) / epw; m < padded / epw; ++m) wrow[(size_t)m * npad] = padw;
     nn[gi] = padded;
     if (cnt > maxnb) atomicMax(&counter[CNT_OVF], cnt);
     mymax = max(mymax, cnt);
   }
+  };
+  if (fmt12) tile_rows(UConst<5u>{});
+  else tile_rows(UConst<4u>{});
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mymax = max(mymax, (uint32_t)__shfl_xor((int)mymax, o, 64));
   if (pb_lane() == 0 && mymax) atomicMax(&counter[CNT_MAXNN], mymax);
@@ -540,7 +583,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
 struct StepArgs {
   const double4 *rin, *vin;
   double4 *rout, *vout, *Fout;
-  const uint64_t *nl;       // 4 u16 entries per word (see k_build_list)
+  const uint64_t *nl;       // 5 12-bit or 4 16-bit entries per word (see k_build_list)
   const uint32_t *nn;
   const float4 *rbuild;
   const uint32_t *cstart;
@@ -551,6 +594,8 @@ struct StepArgs {
   uint32_t *counter;
   double *energy;       // records of 8*nrep doubles
   uint32_t *mon;        // host-mapped ring (8 slots x 4 words) or null
+  uint32_t *queue;      // persistent form: 8 work-queue counters + exit counter (else null)
+  uint32_t nsegs;       // segments (work units) of the launch
   uint32_t step;        // index of this step inside the run
   uint32_t npad, nrep;
   double dt, rc2;
@@ -583,33 +628,34 @@ __device__ __forceinline__ void publish_prev(const StepArgs &A) {
 }
 
 // Pairs and integration of one segment's tile particles.  4 lanes per
-// particle, 16 particles per wavefront: lane sl takes list entries
-// 16 b + 4 j + sl (j < 4) of batch b, so every lane has 4 independent pairs
-// in flight and the next batch's list entries are loaded while the current
+// particle, 16 particles per wavefront: lane sl takes list word 4 b + sl of
+// batch b (entries 20 b + 5 sl + j, j < 5), so every lane has 5 independent
+// pairs in flight and the next batch's list word is loaded while the current
 // one is evaluated.  Indices past the end of a row are clamped (no extra
 // HBM lines) and replaced by the padding slot, so with FAST (window staged,
 // no minimum image) a pair is a straight-line sequence: the cutoff is a
 // select.  A butterfly over the 4 lanes sums the partial forces and lane 0
 // of each group integrates.
 template <bool FAST, bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
-__device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, const RepInfo &R,
-                                         bool staged, bool mi, double &ke, double &pe_b,
+__device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, const ListSeg &G,
+                                         const RepInfo &R, bool staged, bool mi, double &ke, double &pe_b,
                                          double &vir_b, float &d2m, float &v2m) {
   const uint32_t ntile = S.W.rpref[S.W.nrun];
   const double L0 = R.L[0], L1 = R.L[1], L2 = R.L[2];
   const double I0 = R.Linv[0], I1 = R.Linv[1], I2 = R.Linv[2];
-  const uint32_t lane = pb_lane(), sl = lane & 3u, grp = lane >> 2;
+  const uint32_t lane = pb_lane(), sl = lane & (LPA - 1u), grp = lane / LPA;
+  constexpr uint32_t APW = 64u / LPA;            // particles per wavefront
   const uint32_t wave = threadIdx.x / 64, nwv = blockDim.x / 64;
-  const uint32_t PAD = FAST ? (uint32_t)WCAP : NL_PAD;
   // replica constants read once, before any store (stores could alias them)
   const double half_skin2 = R.half_skin2;
   const float inv_hs2 = R.inv_hs2;
   const size_t ld = A.npad;
   // Software pipeline across particle groups: while group u is evaluated,
   // the next group's slot, list length, first list batch and integration
-  // state are already in flight.  The first batch (entries 0..15) is read
+  // state are already in flight.  The first batch (words 0..LPA-1) is read
   // without knowing the list length: those words lie inside the list array
-  // for any slot (capacity >= 16 entries) and are not used past the length.
+  // for any slot (capacity >= 20 * LPA entries) and are not used past the
+  // length.
   uint32_t nli = 0, ngi = 0, nnn = 0;
   uint64_t nq = 0;
   double4 np_own = make_double4(0, 0, 0, 0), nv_own = np_own;
@@ -637,19 +683,19 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
       nrb_own = A.rbuild[ngi];
     }
   };
-  fetch_list(wave * 16);
-  fetch_state(wave * 16);
-  for (uint32_t ub = wave * 16; ub < ntile; ub += nwv * 16) {     // wave-uniform
+  fetch_list(wave * APW);
+  fetch_state(wave * APW);
+  for (uint32_t ub = wave * APW; ub < ntile; ub += nwv * APW) {   // wave-uniform
     const uint32_t u = ub + grp;
     const bool act = u < ntile;
     const uint32_t li = nli, gi = ngi, nni = nnn;
     uint64_t qw = nq;
     const double4 p_own = np_own, v_own = nv_own;
     const float4 rb_own = nrb_own;
-    fetch_list(ub + nwv * 16);
+    fetch_list(ub + nwv * APW);
     double4 pi;
     if (FAST) pi = make_double4(S.x[li], S.y[li], S.z[li], (double)S.t[li]);
-    else pi = act ? window_pos(S, R, A.rin, staged, li) : make_double4(0, 0, 0, 0);
+    else pi = act ? window_pos(S, G, R, A.rin, staged, li) : make_double4(0, 0, 0, 0);
     const uint32_t ti = ONE_TYPE ? 0u : (uint32_t)pi.w;
     // forces and energies accumulate in FP64 in both modes; MIXED evaluates
     // each pair in FP32 on the FP64 displacement after the FP64 cutoff test
@@ -658,7 +704,7 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
     auto pair = [&](uint32_t q) {
       double4 pj;
       if (FAST) pj = make_double4(S.x[q], S.y[q], S.z[q], ONE_TYPE ? 0.0 : (double)S.t[q]);
-      else pj = window_pos(S, R, A.rin, staged, q);
+      else pj = window_pos(S, G, R, A.rin, staged, q);
       double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
       if (!FAST && mi) {
         dx -= L0 * rint(dx * I0);
@@ -678,10 +724,13 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
         const float e24 = ONE_TYPE ? A.e24f : A.lj->e24f[t];
         float ir2 = __builtin_amdgcn_rcpf(r2);
         ir2 = ir2 * (2.0f - r2 * ir2);               // one Newton step: ~0.5 ulp
+        ir2 = in ? ir2 : 0.0f;                       // outside rc: f/r = 0 (one FP32 select)
         float s6 = s2 * ir2;
         s6 = s6 * s6 * s6;
-        float fr = e24 * s6 * (2.0f * s6 - 1.0f) * ir2;   // kernelinline.hh:90-95
-        fr = in ? fr : 0.0f;
+        // f/r = 24 eps s6 (2 s6 - 1) / r^2 (kernelinline.hh:90-95), written as
+        // ir2 s6 (48 eps s6 - 24 eps): one multiply fewer a pair
+        const float e48 = e24 + e24;                 // exact; hoisted with one type
+        const float fr = ir2 * s6 * fmaf(e48, s6, -e24);
         const double frd = (double)fr;              // f/r, applied to the FP64 displacement
         fx = fma(frd, dx, fx);
         fy = fma(frd, dy, fy);
@@ -714,25 +763,29 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
         }
       }
     };
-    // batch b: lane sl takes list word 4 b + sl (entries 16 b + 4 sl .. + 3);
+    // batch b: lane sl takes list word 4 b + sl (entries 20 b + 5 sl .. + 4);
     // lists are padded with the inert entry to whole batches
-    const uint64_t *wp = A.nl + (size_t)(4u + sl) * ld + gi;   // word of the next batch
-    const size_t wstep = 4 * ld;
-    const uint32_t nbat = act ? nni >> 4 : 0u;
+    const uint64_t *wp = A.nl + (size_t)(LPA + sl) * ld + gi;  // word of the next batch
+    const size_t wstep = LPA * ld;
+    constexpr uint32_t EPW = FAST ? 5u : 4u, EB = FAST ? 12u : 16u;
+    constexpr uint32_t EMASK = (1u << EB) - 1u;
+    const uint32_t nbat = act ? nni / (EPW * LPA) : 0u;
     for (uint32_t b = 0; b < nbat; ++b) {
-      const uint32_t lo = (uint32_t)qw, hi = (uint32_t)(qw >> 32);
+      const uint64_t w64 = qw;
       qw = *wp;                                   // one batch ahead (slack words at the end)
       wp += wstep;
-      uint32_t q[4] = {lo & 0xFFFFu, lo >> 16, hi & 0xFFFFu, hi >> 16};
 #pragma unroll
-      for (int j = 0; j < 4; ++j) pair(FAST ? min(q[j], PAD) : q[j]);
+      for (uint32_t j = 0; j < EPW; ++j) {
+        const uint32_t qj = (uint32_t)(w64 >> (EB * j)) & EMASK;
+        pair(qj);
+      }
     }
     if (MIXED && ENERGY) {
       pe = (double)pef;
       vir = (double)virf;
     }
 #pragma unroll
-    for (int o = 2; o > 0; o >>= 1) {
+    for (int o = (int)LPA / 2; o > 0; o >>= 1) {
       fx += __shfl_xor(fx, o, 64);
       fy += __shfl_xor(fy, o, 64);
       fz += __shfl_xor(fz, o, 64);
@@ -741,7 +794,7 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
         vir += __shfl_xor(vir, o, 64);
       }
     }
-    fetch_state(ub + nwv * 16);
+    fetch_state(ub + nwv * APW);
     if (!act || sl != 0) continue;
 
     if (INTEGRATE) {
@@ -750,11 +803,12 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
       const float4 rb = rb_own;
       // the list is valid for this step iff the positions the pair loop used
       // are within skin/2 of their positions at the last rebuild
-      {
-        const float ex = (float)(p.x - (double)rb.x), ey = (float)(p.y - (double)rb.y),
-                    ez = (float)(p.z - (double)rb.z);
-        if ((double)(ex * ex + ey * ey + ez * ez) > half_skin2) atomicOr(&A.counter[CNT_SKIN], 1u);
-      }
+      auto disp2 = [&](const double4 &x) {
+        const float ex = (float)(x.x - (double)rb.x), ey = (float)(x.y - (double)rb.y),
+                    ez = (float)(x.z - (double)rb.z);
+        return ex * ex + ey * ey + ez * ez;
+      };
+      if ((double)disp2(p) > half_skin2) atomicOr(&A.counter[CNT_SKIN], 1u);
       // kick-drift form of propagateVelocityVerlet_F4 (kernelfuncs.cu:
       // 347-365): v holds v(t - dt/2) = v_ref(t-dt) + Fold dt/2m
       const double dtm2 = 0.5 * A.dt * vi.w;
@@ -769,10 +823,8 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
       A.rout[gi] = p;
       A.vout[gi] = vi;
       // displacement of the new positions feeds the rebuild schedule
-      const float ddx = (float)(p.x - (double)rb.x), ddy = (float)(p.y - (double)rb.y),
-                  ddz = (float)(p.z - (double)rb.z);
       // in units of the replica's half skin (replicas may have different skins)
-      d2m = fmaxf(d2m, (ddx * ddx + ddy * ddy + ddz * ddz) * inv_hs2);
+      d2m = fmaxf(d2m, disp2(p) * inv_hs2);
       v2m = fmaxf(v2m, (float)(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z) * inv_hs2);
     }
     if (WRITE_F) A.Fout[gi] = make_double4(fx, fy, fz, 0.0);
@@ -783,11 +835,11 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
   }
 }
 
+// One segment: stage its window, pairs + integration of its tile
+// particles, block maxima and energy partials.
 template <bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
-__device__ __forceinline__ void lj_step_impl(const StepArgs &A) {
+__device__ __forceinline__ void lj_segment(const StepArgs &A, uint32_t seg) {
   __shared__ Stage S;
-  publish_prev(A);
-  const uint32_t seg = pb_xcd_block();
   const ListSeg G = A.segs[seg];
   const RepInfo &R = A.reps[G.rep];
   stage_window(S, G, R, A.cstart, A.rin);
@@ -797,10 +849,10 @@ __device__ __forceinline__ void lj_step_impl(const StepArgs &A) {
   double ke = 0.0, pe_b = 0.0, vir_b = 0.0;
   float d2m = 0.f, v2m = 0.f;
   if (staged && !mi)          // block-uniform: the common case, branch-free pairs
-    lj_tiles<true, MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE>(A, S, R, staged, mi, ke, pe_b, vir_b,
+    lj_tiles<true, MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE>(A, S, G, R, staged, mi, ke, pe_b, vir_b,
                                                                 d2m, v2m);
   else
-    lj_tiles<false, MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE>(A, S, R, staged, mi, ke, pe_b,
+    lj_tiles<false, MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE>(A, S, G, R, staged, mi, ke, pe_b,
                                                                  vir_b, d2m, v2m);
 
   __shared__ float smx[2][TBS / 64];
@@ -855,17 +907,60 @@ __device__ __forceinline__ void lj_step_impl(const StepArgs &A) {
   }
 }
 
+template <bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE, bool PERSIST>
+__device__ __forceinline__ void lj_step_impl(const StepArgs &A) {
+  publish_prev(A);
+  if constexpr (PERSIST) {
+    // Persistent form: a grid of (compute units x resident workgroups) drains
+    // the segments from eight work queues, one per slab of consecutive
+    // segments (the slabs pb_xcd_block gives each XCD); a workgroup takes
+    // from the queue of the XCD it runs on first (workgroups are dispatched
+    // round-robin over the 8 XCDs), then from the others.  The last
+    // workgroup to finish resets the queues for the next launch (stream
+    // order makes that visible), so the launch can be replayed in a graph.
+    __shared__ uint32_t ticket;
+    const uint32_t n = A.nsegs, per = n >> 3, rmd = n & 7u;
+    const uint32_t home = blockIdx.x & 7u;
+    uint32_t k = 0;
+    for (;;) {
+      // queue, slab start and segment index are block-uniform (scalar registers)
+      const uint32_t q = __builtin_amdgcn_readfirstlane((home + k) & 7u);
+      __syncthreads();                   // the last segment is done with LDS and `ticket`
+      if (threadIdx.x == 0) ticket = atomicAdd(&A.queue[q], 1u);
+      __syncthreads();
+      const uint32_t t = __builtin_amdgcn_readfirstlane(ticket);
+      const uint32_t cnt = per + (q < rmd ? 1u : 0u);
+      if (t >= cnt) {                    // this queue is drained: the next one
+        if (++k == 8) break;
+        continue;
+      }
+      const uint32_t seg = __builtin_amdgcn_readfirstlane(q * per + (q < rmd ? q : rmd) + t);
+      lj_segment<MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE>(A, seg);
+    }
+    if (threadIdx.x == 0) {
+      __threadfence();
+      if (atomicAdd(&A.queue[8], 1u) == gridDim.x - 1) {
+        // every other workgroup has finished taking work: reset
+        for (int q = 0; q < 9; ++q) __hip_atomic_store(&A.queue[q], 0u, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    return;
+  }
+  lj_segment<MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE>(A, pb_xcd_block());
+}
+
 // Two entry points over the same body: the common variants fit 128 VGPRs
 // (4 wavefronts per SIMD, two 512-thread workgroups per CU); the FP64 and
 // multi-type energy variants need more registers and run without the cap
 // (no spills) at lower occupancy.
-template <bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
+template <bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE, bool PERSIST>
 __global__ void __launch_bounds__(TBS) __attribute__((amdgpu_waves_per_eu(4))) k_lj_step(StepArgs A) {
-  lj_step_impl<MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE>(A);
+  lj_step_impl<MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE, PERSIST>(A);
 }
-template <bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
+template <bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE, bool PERSIST>
 __global__ void __launch_bounds__(TBS) k_lj_step_wide(StepArgs A) {
-  lj_step_impl<MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE>(A);
+  lj_step_impl<MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE, PERSIST>(A);
 }
 
 // ------------------------------------------------------------- host side
@@ -877,7 +972,7 @@ static uint32_t expected_maxnb(const pb_sys *s) {
     mean = std::max(mean, 4.0 / 3.0 * M_PI * R.rlist * R.rlist * R.rlist * rho);
   }
   uint32_t m = (uint32_t)std::ceil(1.5 * mean + 48.0);
-  return (m + 15u) & ~15u;
+  return (m + NL_ROW - 1u) / NL_ROW * NL_ROW;
 }
 
 // List cells, radii and segments for cutoff rc with skin[q] per replica.
@@ -962,8 +1057,9 @@ static int ensure_list_buffers(pb_sys *s, uint32_t maxnb) {
       return rc;
   }
   if (s->maxnb < maxnb) {
-    // + 16 entries per slot: the step kernel prefetches one batch past the end
-    if ((rc = pb_alloc(s->ctx, &s->nl, (size_t)(maxnb + 16) * s->npad))) return rc;
+    // + 48 entries per slot: the step kernel prefetches one batch (4 words)
+    // past the end
+    if ((rc = pb_alloc(s->ctx, &s->nl, (size_t)(maxnb + 48) * s->npad))) return rc;
     s->maxnb = maxnb;
   }
   return PB_OK;
@@ -971,7 +1067,9 @@ static int ensure_list_buffers(pb_sys *s, uint32_t maxnb) {
 
 // rebuild: cell ids -> counting sort -> reorder into cell order -> list.
 // F is not carried: every caller rewrites it before it is read again.
-static int rebuild(pb_sys *s) {
+// carry_a = false inside a fused run, where Fold (a) is dead between the
+// conversion to the kick-drift state and the final one (which rewrites it).
+static int rebuild(pb_sys *s, bool carry_a) {
   hipStream_t st = s->ctx->stream;
   const uint32_t n = s->npad;
   hipLaunchKernelGGL(k_list_cellid, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->r, s->orig,
@@ -980,7 +1078,7 @@ static int rebuild(pb_sys *s) {
   if (int rc = pb_counting_sort(s, s->list_ncell, s->orig)) return rc;
   ReorderArgs A;
   A.perm = s->perm;
-  A.r = s->r; A.v = s->v; A.a = s->a;
+  A.r = s->r; A.v = s->v; A.a = carry_a ? s->a : nullptr;
   A.orig = s->orig; A.cellid = s->cellid; A.type = s->type; A.move = s->move;
   A.r2 = s->r2buf; A.v2 = s->v2buf; A.a2 = s->a2buf;
   A.orig2 = s->orig2; A.cell2 = s->rank; A.type2 = s->type2; A.move2 = s->move2;
@@ -991,7 +1089,7 @@ static int rebuild(pb_sys *s) {
   PB_LAUNCH_CHECK(s->ctx, "k_reorder");
   std::swap(s->r, s->r2buf);
   std::swap(s->v, s->v2buf);
-  std::swap(s->a, s->a2buf);
+  if (carry_a) std::swap(s->a, s->a2buf);
   std::swap(s->orig, s->orig2);
   std::swap(s->type, s->type2);
   std::swap(s->move, s->move2);
@@ -1015,6 +1113,7 @@ static StepArgs make_args(pb_sys *s, double dt) {
   A.lj = s->d_lj;
   A.partials = s->partials;
   A.counter = s->d_counter;
+  A.nsegs = s->list_nsegs;
   A.npad = s->npad;
   A.nrep = s->nrep;
   A.dt = dt;
@@ -1045,34 +1144,50 @@ __global__ void __launch_bounds__(256) k_energy_reduce(const double *partials, c
 
 typedef void (*StepKernel)(StepArgs);
 
-template <bool MIXED, bool ONE, bool E, bool W, bool I>
+template <bool MIXED, bool ONE, bool E, bool W, bool I, bool P>
 static StepKernel pick1() {
   // variants that do not fit 128 VGPRs without spilling
   if constexpr ((E && (!MIXED || !ONE)) || (!MIXED && !ONE))
-    return k_lj_step_wide<MIXED, E, W, ONE, I>;
+    return k_lj_step_wide<MIXED, E, W, ONE, I, P>;
   else
-    return k_lj_step<MIXED, E, W, ONE, I>;
+    return k_lj_step<MIXED, E, W, ONE, I, P>;
+}
+
+template <bool MIXED, bool ONE, bool P>
+static StepKernel pick3(bool energy, bool wf, bool integ) {
+  if (integ) {
+    if (energy) return wf ? pick1<MIXED, ONE, true, true, true, P>() : pick1<MIXED, ONE, true, false, true, P>();
+    return wf ? pick1<MIXED, ONE, false, true, true, P>() : pick1<MIXED, ONE, false, false, true, P>();
+  }
+  return energy ? pick1<MIXED, ONE, true, true, false, false>() : pick1<MIXED, ONE, false, true, false, false>();
 }
 
 template <bool MIXED, bool ONE>
-static StepKernel pick3(bool energy, bool wf, bool integ) {
-  if (integ) {
-    if (energy) return wf ? pick1<MIXED, ONE, true, true, true>() : pick1<MIXED, ONE, true, false, true>();
-    return wf ? pick1<MIXED, ONE, false, true, true>() : pick1<MIXED, ONE, false, false, true>();
-  }
-  return energy ? pick1<MIXED, ONE, true, true, false>() : pick1<MIXED, ONE, false, true, false>();
+static StepKernel pick2(bool energy, bool wf, bool integ, bool persist) {
+  return persist ? pick3<MIXED, ONE, true>(energy, wf, integ) : pick3<MIXED, ONE, false>(energy, wf, integ);
 }
 
-static StepKernel pick_kernel(bool mixed, bool one, bool energy, bool wf, bool integ) {
-  if (mixed) return one ? pick3<true, true>(energy, wf, integ) : pick3<true, false>(energy, wf, integ);
-  return one ? pick3<false, true>(energy, wf, integ) : pick3<false, false>(energy, wf, integ);
+// persistent variants exist for the integrating steps of a fused run
+static StepKernel pick_kernel(bool mixed, bool one, bool energy, bool wf, bool integ, bool persist) {
+  if (mixed) return one ? pick2<true, true>(energy, wf, integ, persist) : pick2<true, false>(energy, wf, integ, persist);
+  return one ? pick2<false, true>(energy, wf, integ, persist) : pick2<false, false>(energy, wf, integ, persist);
 }
 
+// With events, their timestamps are the step kernel's own start and end
+// (taken by the dispatch, hipExtLaunchKernelGGL): a marker packet of
+// hipEventRecord between kernels costs about 10 us of idle GPU per step.
 static int launch_step(pb_sys *s, const StepArgs &A, bool mixed, bool energy, bool wf,
-                       bool integ) {
+                       bool integ, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
   const bool one = s->lj.ntypes == 1;
-  StepKernel k = pick_kernel(mixed, one, energy, wf, integ);
-  hipLaunchKernelGGL(k, dim3(s->list_nsegs), dim3(TBS), 0, s->ctx->stream, A);
+  StepKernel k = pick_kernel(mixed, one, energy, wf, integ, A.queue != nullptr);
+  // persistent form: two 512-thread workgroups per compute unit (the step
+  // kernel's LDS and register use), never more than there are segments
+  const uint32_t grid = A.queue ? std::min<uint32_t>(s->list_nsegs, 2u * (uint32_t)std::max(s->ctx->cu_count, 1))
+                                : s->list_nsegs;
+  if (ev0 && ev1)
+    hipExtLaunchKernelGGL(k, dim3(grid), dim3(TBS), 0, s->ctx->stream, ev0, ev1, 0u, A);
+  else
+    hipLaunchKernelGGL(k, dim3(grid), dim3(TBS), 0, s->ctx->stream, A);
   if (energy)
     hipLaunchKernelGGL(k_energy_reduce, dim3(1), dim3(256), 0, s->ctx->stream, A.partials, A.reps,
                        A.nrep, A.energy, A.counter);
@@ -1102,12 +1217,12 @@ static int reset_counters(pb_sys *s) {
 }
 
 // build (or grow) the list so that no particle overflows it
-static int build_checked(pb_sys *s) {
+static int build_checked(pb_sys *s, bool carry_a) {
   uint32_t want = std::max(s->maxnb, expected_maxnb(s));
   for (int attempt = 0; attempt < 4; ++attempt) {
     if (int rc = ensure_list_buffers(s, want)) return rc;
     if (int rc = reset_counters(s)) return rc;
-    if (int rc = rebuild(s)) return rc;
+    if (int rc = rebuild(s, carry_a)) return rc;
     uint32_t c[16];
     if (int rc = read_counters(s, c)) return rc;
     if (c[CNT_WIN])
@@ -1116,7 +1231,7 @@ static int build_checked(pb_sys *s) {
       s->last_maxnn = c[CNT_MAXNN];
       return PB_OK;
     }
-    want = (c[CNT_OVF] * 5 / 4 + 16u) & ~15u;
+    want = (c[CNT_OVF] * 5 / 4 + 2u * NL_ROW - 1u) / NL_ROW * NL_ROW;
   }
   return pb_fail(s->ctx, PB_ERR_NLIST, "neighbour list keeps overflowing");
 }
@@ -1127,7 +1242,7 @@ extern "C" int pb_lj_force(pb_sys *s, int precision, double *pe, double *virial)
   PB_HIP(s->ctx, hipSetDevice(s->ctx->device));
   const double rc = std::sqrt(s->rc2);
   if (int e = setup_list_cells(s, rc, std::vector<double>(s->nrep, rc * 1e-12))) return e;
-  if (int e = build_checked(s)) return e;
+  if (int e = build_checked(s, true)) return e;
   StepArgs A = make_args(s, 0.0);
   A.rin = s->r;
   A.vin = s->v;
@@ -1182,6 +1297,75 @@ __global__ void k_from_half(double4 *v, double4 *Fold, const double4 *F, uint32_
   Fold[s] = make_double4(f.x, f.y, f.z, 0.0);
 }
 
+// Fingerprint of the positions and the storage order (what the Verlet list
+// and the cell order depend on): a wrapping sum over slots of a 64-bit mix of
+// the slot index, the raw bits of r and orig; order-independent, so the
+// atomics leave it deterministic.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+// Sum over slots of a per-slot hash (order-free, so any grid gives the same
+// value): a grid-stride loop and one atomic per workgroup, since atomics of
+// every wavefront on one address serialise (one per wavefront took 200 us at
+// 1M slots).
+__global__ void __launch_bounds__(256) k_state_hash(const double4 *r, const uint32_t *orig,
+                                                    uint32_t n, unsigned long long *out) {
+  __shared__ uint64_t part[4];
+  uint64_t acc = 0;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    const double4 p = r[s];
+    uint64_t h = mix64((uint64_t)s * 0x9E3779B97F4A7C15ull ^ (uint64_t)orig[s]);
+    h = mix64(h ^ (uint64_t)__double_as_longlong(p.x));
+    h = mix64(h ^ (uint64_t)__double_as_longlong(p.y));
+    h = mix64(h ^ (uint64_t)__double_as_longlong(p.z));
+    h = mix64(h ^ (uint64_t)__double_as_longlong(p.w));
+    acc += h;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += (uint64_t)__shfl_xor((long long)acc, o, 64);
+  if (pb_lane() == 0) part[threadIdx.x / 64] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint64_t t = part[0] + part[1] + part[2] + part[3];
+    if (t) atomicAdd(out, (unsigned long long)t);
+  }
+}
+
+// A continued run: the periodic wrap at the end of the previous run moved
+// particles that left the box by one box length; the list and the window
+// images expect every particle near its position at the last list build,
+// so each is put back at the image nearest to that position.
+__global__ void k_unwrap_to_build(double4 *r, const float4 *rbuild, const uint32_t *orig,
+                                  const uint32_t *waveRep, const RepInfo *reps, uint32_t n) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n || orig[s] == PB_NONE) return;
+  const RepInfo &R = reps[waveRep[s / PB_WAVE]];
+  double4 p = r[s];
+  const float4 b = rbuild[s];
+  p.x -= R.L[0] * rint((p.x - (double)b.x) * R.Linv[0]);
+  p.y -= R.L[1] * rint((p.y - (double)b.y) * R.Linv[1]);
+  p.z -= R.L[2] * rint((p.z - (double)b.z) * R.Linv[2]);
+  r[s] = p;
+}
+
+static int state_hash(pb_sys *s, uint64_t *h) {
+  if (!s->d_hash) {
+    if (int rc = pb_alloc(s->ctx, &s->d_hash, 1)) return rc;
+  }
+  hipStream_t st = s->ctx->stream;
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t *)s->d_hash, 2u, 0u);
+  const uint32_t nb = std::max<uint32_t>(
+      1u, std::min<uint32_t>(pb_blocks(s->npad, 256), 4u * (uint32_t)s->ctx->cu_count));
+  hipLaunchKernelGGL(k_state_hash, dim3(nb), dim3(256), 0, st, s->r, s->orig, s->npad, s->d_hash);
+  unsigned long long v = 0;
+  PB_HIP(s->ctx, hipMemcpyAsync(&v, s->d_hash, 8, hipMemcpyDeviceToHost, st));
+  PB_HIP(s->ctx, hipStreamSynchronize(st));
+  *h = v;
+  return PB_OK;
+}
+
 extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
   if (!s || !o) return PB_ERR_ARG;
   if (int rc = check_lj_ready(s)) return rc;
@@ -1230,8 +1414,19 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
   int rc = PB_OK;
   uint32_t rebuilds = 0;
   int skin_bad = 0;
+  bool cont = false;
+  int64_t end_since = 0;             // steps since the last list build at the end
   // optional HIP-event timing on the context's stream
-  std::vector<hipEvent_t> ev_step, ev_reb, ev_graph;   // graph periods: R steps + rebuild
+  std::vector<hipEvent_t> ev_step, ev_graph;   // graph periods: R steps + rebuild
+  // Eager steps: opts.timing = k times every k-th step and the step after
+  // each rebuild, with the kernel's own timestamps; a rebuild is timed from
+  // the end of the step before it (or a marker when that step was not
+  // timed) to the start of the step after it.  A timed dispatch idles the
+  // GPU about 9 us before it, so k > 1 keeps that cost off most steps.
+  const uint32_t tstride = o->timing > 0 ? (uint32_t)o->timing : 0u;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> reb_spans;
+  std::vector<hipEvent_t> ev_mark;                     // rebuild-start markers
+  hipEvent_t reb_from = nullptr, prev_stop = nullptr;
   auto mk = [&](std::vector<hipEvent_t> &v) -> hipEvent_t {
     hipEvent_t e = nullptr;
     if (hipEventCreate(&e) == hipSuccess) v.push_back(e);
@@ -1253,17 +1448,42 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
     }
   }
   for (int k = 0; k < 32; ++k) ((volatile uint32_t *)s->hmon)[k] = 0u;
+  // Continue with the list of the previous run when nothing has changed the
+  // positions or the storage order since (in graph mode the first captured
+  // period then starts where the eager schedule would rebuild)
+  cont = false;
+  if (s->lj_cont && s->lj_cont_rc2 == s->rc2 && s->lj_cont_skins == skins &&
+      s->lj_cont_tile_w == s->tile_w && s->lj_cont_fill == s->tile_fill) {
+    uint64_t h = 0;
+    if ((rc = state_hash(s, &h))) goto done;
+    cont = h == s->lj_cont_hash;
+  }
+  s->lj_cont = false;
   // reference VV state -> kick-drift state
   hipLaunchKernelGGL(k_to_half, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->v, s->a,
                      s->npad, o->dt);
-  if ((rc = build_checked(s))) goto done;
-  ++rebuilds;
+  if (!cont) {
+    if ((rc = build_checked(s, false))) goto done;
+    ++rebuilds;
+  } else {
+    hipLaunchKernelGGL(k_unwrap_to_build, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->r,
+                       s->rbuild, s->orig, s->waveRep, s->d_reps, s->npad);
+  }
   if ((rc = reset_counters(s))) goto done;
+  if (o->persistent && !s->lj_queue) {
+    // zeroed once; the persistent kernel resets the counters after each launch
+    if ((rc = pb_alloc(s->ctx, &s->lj_queue, 16))) goto done;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s->lj_queue, 16u, 0u);
+  }
   {
     StepArgs A = make_args(s, o->dt);
     A.energy = ebuf;
     A.mon = s->dmon;
-    uint32_t last_build = 0;
+    A.queue = o->persistent ? s->lj_queue : nullptr;
+    // steps are counted from the list build: a continued list is `since`
+    // steps old at step 0
+    const int64_t since0 = cont ? (int64_t)s->lj_cont_since : 0;
+    int64_t last_build = -since0;
     uint32_t step = 0;
     // Graph mode: whole rebuild periods -- R steps followed by the list
     // rebuild that prepares the next period -- are captured once as a
@@ -1273,13 +1493,114 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
     // the other ping-pong buffer, so up to two graphs (one per buffer
     // assignment) are kept.  The remaining steps (at least the last one,
     // which writes F) run eagerly.
+    auto rebuild_now = [&]() -> int {
+      // rebuilds after the first use the capacity found by the first one;
+      // an overflow is reported at the end of the run
+      if (tstride) {
+        reb_from = prev_stop;
+        if (!reb_from) {
+          reb_from = mk(ev_mark);
+          if (reb_from) (void)hipEventRecord(reb_from, st);
+        }
+      }
+      if (int e = rebuild(s, false)) return e;
+      hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s->d_counter + CNT_DISP, 2u, 0u);
+      ++rebuilds;
+      last_build = step;
+      A.nl = (const uint64_t *)s->nl;
+      A.nn = s->nn;
+      return PB_OK;
+    };
+    // eager steps [step, to) with the host-polled rebuild schedule
+    auto eager = [&](uint32_t to) -> int {
+    for (; step < to; ++step) {
+      // Rebuild schedule: at most every rebuild_every steps, and earlier when
+      // the particles may reach skin/2.  Kernel `step-1` published the
+      // maxima after step `step-2` when it started, so reading them keeps one
+      // kernel in flight; the one step of motion in between is bounded by
+      // 1.5 * vmax * dt.
+      bool reb = false;
+      if (step == 0 && cont) {
+        // the previous run's last kernel measured the displacement of these
+        // positions (at most d0); kernel 0 uses them and kernel 1 the positions
+        // one step later, and the first report of this run (kernel 0's output,
+        // read at step 2) covers kernel 2: the same one-step bound as below
+        float d2, v2;
+        std::memcpy(&d2, &s->lj_cont_d2, 4);
+        std::memcpy(&v2, &s->lj_cont_v2, 4);
+        reb = since0 >= (int64_t)o->rebuild_every ||
+              std::sqrt((double)d2) + 1.5 * std::sqrt((double)v2) * o->dt >= 1.0;
+      } else if (step > 0) {
+        if ((int64_t)step - last_build >= (int64_t)o->rebuild_every) {
+          reb = true;
+        } else if (step >= 2 && (int64_t)step - last_build >= 2) {
+          const uint32_t slot = ((step - 2) & 7u) * 4u;
+          volatile uint32_t *mon = (volatile uint32_t *)s->hmon;
+          const auto t0 = std::chrono::steady_clock::now();
+          while (mon[slot + 3] != step - 1) {
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(30)) {
+              return pb_fail(s->ctx, PB_ERR_HIP, "pb_lj_run: step report timed out");
+            }
+            std::this_thread::yield();
+          }
+          std::atomic_thread_fence(std::memory_order_acquire);
+          const uint32_t ud = mon[slot + 0], uv = mon[slot + 1];
+          float d2, v2;
+          std::memcpy(&d2, &ud, 4);
+          std::memcpy(&v2, &uv, 4);
+          // maxima are in units of each replica's half skin
+          if (std::sqrt((double)d2) + 1.5 * std::sqrt((double)v2) * o->dt >= 1.0) reb = true;
+        }
+      }
+      if (reb) {
+        if (int e = rebuild_now()) return e;
+      }
+      const bool energy = ne && (step % ne == 0);
+      const bool last = step + 1 == o->nsteps;
+      A.rin = s->r;
+      A.vin = s->v;
+      A.rout = s->r2buf;
+      A.vout = s->v2buf;
+      A.Fout = s->F;
+      A.step = step;
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      if (tstride && (reb_from || step % tstride == 0)) {
+        if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+          ev_step.push_back(e0);
+          ev_step.push_back(e1);
+          if (reb_from) reb_spans.emplace_back(reb_from, e0);
+        } else {
+          if (e0) (void)hipEventDestroy(e0);
+          e0 = e1 = nullptr;
+        }
+        reb_from = nullptr;
+      }
+      if (int e = launch_step(s, A, mixed, energy, last, true, e0, e1)) return e;
+      prev_stop = e1;
+      std::swap(s->r, s->r2buf);
+      std::swap(s->v, s->v2buf);
+    }
+    return PB_OK;
+    };
     if (o->use_graph) {
       const uint32_t R = o->rebuild_every;
       if (ne && R % ne) {
         rc = pb_fail(s->ctx, PB_ERR_ARG, "pb_lj_run: with use_graph, energy_every must divide rebuild_every");
         goto done;
       }
-      const uint32_t periods = (o->nsteps - 1) / R;
+      // a continued list: eager steps until it is R steps old (the eager
+      // schedule's forced rebuild), then the rebuild, so every period starts
+      // from a fresh list as in a run that built at its start
+      if (cont) {
+        const uint32_t prefix = (uint32_t)std::min<int64_t>(
+            o->nsteps, std::max<int64_t>(0, (int64_t)R - since0));
+        if ((rc = eager(prefix))) goto done;
+        if (step < o->nsteps && (rc = rebuild_now())) goto done;
+      }
+      const uint32_t periods = step < o->nsteps ? (o->nsteps - step - 1) / R : 0u;
+      // energies are recorded at steps s with s % ne == 0 in both schedules;
+      // R % ne == 0, so every period records at the same offsets
+      const uint32_t step0 = step;
       hipGraphExec_t gx[2] = {nullptr, nullptr};
       hipGraph_t gg[2] = {nullptr, nullptr};
       // a graph is valid for the buffer assignment it was captured with
@@ -1296,11 +1617,11 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
           A.vout = s->v2buf;
           A.Fout = s->F;
           A.step = 0;
-          e = launch_step(s, A, mixed, ne && (l % ne == 0), false, true);
+          e = launch_step(s, A, mixed, ne && ((step0 + l) % ne == 0), false, true);
           std::swap(s->r, s->r2buf);
           std::swap(s->v, s->v2buf);
         }
-        if (e == PB_OK) e = rebuild(s);
+        if (e == PB_OK) e = rebuild(s, false);
         if (e == PB_OK)
           hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s->d_counter + CNT_DISP, 2u, 0u);
         hipGraph_t g = nullptr;
@@ -1350,9 +1671,8 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
           std::swap(s->r, s->r2buf);
           std::swap(s->v, s->v2buf);
         }
-        std::swap(s->r, s->r2buf);          // the rebuild's reorder
+        std::swap(s->r, s->r2buf);          // the rebuild's reorder (Fold is not carried)
         std::swap(s->v, s->v2buf);
-        std::swap(s->a, s->a2buf);
         std::swap(s->orig, s->orig2);
         std::swap(s->type, s->type2);
         std::swap(s->move, s->move2);
@@ -1367,62 +1687,8 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
       if (rc) goto done;
       A.mon = s->dmon;
     }
-    for (; step < o->nsteps; ++step) {
-      // Rebuild schedule: at most every rebuild_every steps, and earlier when
-      // the particles may reach skin/2.  Kernel `step-1` published the
-      // maxima after step `step-2` when it started, so reading them keeps one
-      // kernel in flight; the one step of motion in between is bounded by
-      // 1.5 * vmax * dt.
-      bool reb = false;
-      if (step > 0) {
-        if (step - last_build >= o->rebuild_every) {
-          reb = true;
-        } else if (step - last_build >= 2) {
-          const uint32_t slot = ((step - 2) & 7u) * 4u;
-          volatile uint32_t *mon = (volatile uint32_t *)s->hmon;
-          const auto t0 = std::chrono::steady_clock::now();
-          while (mon[slot + 3] != step - 1) {
-            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(30)) {
-              rc = pb_fail(s->ctx, PB_ERR_HIP, "pb_lj_run: step report timed out");
-              goto done;
-            }
-            std::this_thread::yield();
-          }
-          std::atomic_thread_fence(std::memory_order_acquire);
-          const uint32_t ud = mon[slot + 0], uv = mon[slot + 1];
-          float d2, v2;
-          std::memcpy(&d2, &ud, 4);
-          std::memcpy(&v2, &uv, 4);
-          // maxima are in units of each replica's half skin
-          if (std::sqrt((double)d2) + 1.5 * std::sqrt((double)v2) * o->dt >= 1.0) reb = true;
-        }
-      }
-      if (reb) {
-        // rebuilds after the first use the capacity found by the first one;
-        // an overflow is reported at the end of the run
-        rec(ev_reb);
-        if ((rc = rebuild(s))) goto done;
-        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, s->d_counter + CNT_DISP, 2u, 0u);
-        rec(ev_reb);
-        ++rebuilds;
-        last_build = step;
-        A.nl = (const uint64_t *)s->nl;
-        A.nn = s->nn;
-      }
-      const bool energy = ne && (step % ne == 0);
-      const bool last = step + 1 == o->nsteps;
-      A.rin = s->r;
-      A.vin = s->v;
-      A.rout = s->r2buf;
-      A.vout = s->v2buf;
-      A.Fout = s->F;
-      A.step = step;
-      rec(ev_step);
-      if ((rc = launch_step(s, A, mixed, energy, last, true))) goto done;
-      rec(ev_step);
-      std::swap(s->r, s->r2buf);
-      std::swap(s->v, s->v2buf);
-    }
+    if ((rc = eager(o->nsteps))) goto done;
+    end_since = (int64_t)o->nsteps - last_build;
   }
   // kick-drift state -> reference VV state: v(t) and Fold = F(t)
   hipLaunchKernelGGL(k_from_half, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->v, s->a,
@@ -1442,6 +1708,20 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
       rc = pb_fail(s->ctx, PB_ERR_NLIST, "neighbour list overflow during the run");
       goto done;
     }
+    if (!skin_bad) {
+      // the list and cell order stay valid for a next run from this state
+      uint64_t h = 0;
+      if ((rc = state_hash(s, &h))) goto done;
+      s->lj_cont = true;
+      s->lj_cont_hash = h;
+      s->lj_cont_rc2 = s->rc2;
+      s->lj_cont_skins = skins;
+      s->lj_cont_tile_w = s->tile_w;
+      s->lj_cont_fill = s->tile_fill;
+      s->lj_cont_since = (uint32_t)std::min<int64_t>(end_since, 0x7FFFFFFF);
+      s->lj_cont_d2 = c[CNT_DISP];
+      s->lj_cont_v2 = c[CNT_VMAX];
+    }
   }
   if (energies && nrec) {
     std::vector<double> h((size_t)nrec * 8 * s->nrep);
@@ -1460,14 +1740,14 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
   s->last_skin_violation = skin_bad;
   s->last_step_ms = s->last_rebuild_ms = 0.0;
   s->last_nstep_timed = (uint32_t)(ev_step.size() / 2);
-  s->last_nrebuild_timed = (uint32_t)(ev_reb.size() / 2);
+  s->last_nrebuild_timed = (uint32_t)reb_spans.size();
   for (size_t k = 0; k + 1 < ev_step.size(); k += 2) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, ev_step[k], ev_step[k + 1]) == hipSuccess) s->last_step_ms += ms;
   }
-  for (size_t k = 0; k + 1 < ev_reb.size(); k += 2) {
+  for (const auto &sp : reb_spans) {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, ev_reb[k], ev_reb[k + 1]) == hipSuccess) s->last_rebuild_ms += ms;
+    if (hipEventElapsedTime(&ms, sp.first, sp.second) == hipSuccess) s->last_rebuild_ms += ms;
   }
   // a replayed period counts as rebuild_every steps (its rebuild included)
   for (size_t k = 0; k + 1 < ev_graph.size(); k += 2) {
@@ -1478,7 +1758,7 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
   if (skin_bad) rc = pb_fail(s->ctx, PB_ERR_SKIN, "skin violated: increase skin or rebuild more often");
 done:
   for (hipEvent_t e : ev_step) (void)hipEventDestroy(e);
-  for (hipEvent_t e : ev_reb) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ev_mark) (void)hipEventDestroy(e);
   for (hipEvent_t e : ev_graph) (void)hipEventDestroy(e);
   pb_free(ebuf);
   return rc;

@@ -156,7 +156,9 @@ __global__ void k_scatter(const uint32_t *cellid, const uint32_t *rank, const ui
   if (s < n) perm[start[cellid[s]] + rank[s]] = s;
 }
 
-// 256 consecutive cells per workgroup own one contiguous range of perm.
+// CS_CELLS consecutive cells per workgroup own one contiguous range of perm
+// (64 cells: about 900 particles at liquid density, so a 1M-particle sort
+// is ~1200 workgroups and fills the chip).
 // The range is sorted in passes: each pass stages the longest run of the
 // remaining cells whose particles fit the LDS stage (at least one cell).
 // Every element then finds its cell and its rank in it -- the number of
@@ -165,9 +167,10 @@ __global__ void k_scatter(const uint32_t *cellid, const uint32_t *rank, const ui
 // reads, so a crowded cell costs O(n) per element in parallel instead of a
 // serial O(n^2) insertion sort.  A single cell larger than the stage (a
 // pathological pile-up) is insertion-sorted in global memory by its thread.
-#define CS_CELLS 256
-#define CS_CAP 6144
-__global__ void __launch_bounds__(CS_CELLS) k_cell_sort(const uint32_t *start, uint32_t ncell,
+#define CS_CELLS 64
+#define CS_THREADS 256
+#define CS_CAP 3072
+__global__ void __launch_bounds__(CS_THREADS) k_cell_sort(const uint32_t *start, uint32_t ncell,
                                                         const uint32_t *key, uint32_t *perm) {
   __shared__ uint32_t sv[CS_CAP], sk[CS_CAP];
   __shared__ uint32_t cst[CS_CELLS + 1];
@@ -257,6 +260,7 @@ int pb_ensure_sort_ws(pb_sys *s, uint32_t ncell) {
 }
 
 int pb_counting_sort(pb_sys *s, uint32_t ncell, const uint32_t *key) {
+  s->lj_cont = false;             // the fused LJ run's cell order is gone
   if (int rc = pb_ensure_sort_ws(s, ncell)) return rc;
   hipStream_t st = s->ctx->stream;
   const uint32_t n = s->npad;
@@ -268,8 +272,8 @@ int pb_counting_sort(pb_sys *s, uint32_t ncell, const uint32_t *key) {
   if (int rc = pb_exclusive_scan(s, s->ccount, s->cstart, ncell)) return rc;
   hipLaunchKernelGGL(k_scatter, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->cellid, s->rank,
                      s->cstart, n, s->perm);
-  hipLaunchKernelGGL(k_cell_sort, dim3(pb_blocks(ncell, CS_CELLS)), dim3(CS_CELLS), 0, st, s->cstart,
-                     ncell, key, s->perm);
+  hipLaunchKernelGGL(k_cell_sort, dim3(pb_blocks(ncell, CS_CELLS)), dim3(CS_THREADS), 0, st,
+                     s->cstart, ncell, key, s->perm);
   PB_LAUNCH_CHECK(s->ctx, "pb_counting_sort");
   return PB_OK;
 }
