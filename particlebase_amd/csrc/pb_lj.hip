@@ -478,15 +478,21 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
     // per slot); an overflow is reported and the list rebuilt larger.
     uint64_t *wrow = reinterpret_cast<uint64_t *>(nl) + gi;
     const uint32_t wcap = maxnb / epw;
+    // Entries enter the word at its top and move down eb bits per append
+    // (constant shifts; a word of epw entries then holds the first at bit
+    // 0).  The store pointer advances one word per completed word up to the
+    // slack word.
     uint64_t word = 0;
-    uint32_t cnt = 0, kw = 0;
+    uint32_t cnt = 0, kw = 0, mw = 0;
+    uint64_t *wp = wrow;
     auto append = [&](uint32_t q) {
-      word |= (uint64_t)q << (eb * kw);
+      word = (word >> eb) | ((uint64_t)q << (eb * (epw - 1u)));
       ++cnt;
       if (++kw == epw) {
-        wrow[(size_t)min(cnt / epw - 1u, wcap) * npad] = word;
-        word = 0;
+        *wp = word;
         kw = 0;
+        if (mw < wcap) wp += npad;
+        ++mw;
       }
     };
     // staged candidates [q0, q1) except li: 32 distance tests into a bit
@@ -508,14 +514,18 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
         return dx * dx + dy * dy + dz * dz < rb2;
       };
       for (uint32_t c0 = q0; c0 < q1; c0 += BUILD_GROUP) {
+        // test k shifts in at bit 0, so candidate c0 + k ends at bit
+        // BUILD_GROUP - 1 - k
         uint32_t m = 0;
 #pragma unroll BUILD_UNROLL
-        for (uint32_t k = 0; k < BUILD_GROUP; ++k) m |= (near(c0 + k) ? 1u : 0u) << k;
-        if (q1 - c0 < BUILD_GROUP) m &= (1u << (q1 - c0)) - 1u;
-        if (li - c0 < BUILD_GROUP) m &= ~(1u << (li - c0));   // li < c0 wraps past the group
+        for (uint32_t k = 0; k < BUILD_GROUP; ++k) m = m + m + (near(c0 + k) ? 1u : 0u);
+        constexpr uint32_t top = 1u << (BUILD_GROUP - 1u);
+        if (q1 - c0 < BUILD_GROUP) m &= ~((top >> (q1 - c0 - 1u)) - 1u);
+        if (li - c0 < BUILD_GROUP) m &= ~(top >> (li - c0));   // li < c0 wraps past the group
         while (m) {
-          append(c0 + (uint32_t)__builtin_ctz(m));
-          m &= m - 1u;
+          const uint32_t k = (uint32_t)__builtin_clz(m) - (32u - BUILD_GROUP);
+          append(c0 + k);
+          m &= ~(top >> k);
         }
       }
     };
@@ -565,7 +575,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
     const uint32_t batch = epw * LPA;
     const uint32_t padded = (used + batch - 1u) / batch * batch;
     if (kw != 0u && cnt < maxnb)
-      wrow[(size_t)(cnt / epw) * npad] = word | (padw & (~0ull << (eb * kw)));
+      wrow[(size_t)(cnt / epw) * npad] = (word >> (eb * (epw - kw))) | (padw & (~0ull << (eb * kw)));
     for (uint32_t m = (used + epw - 1u) / epw; m < padded / epw; ++m) wrow[(size_t)m * npad] = padw;
     nn[gi] = padded;
     if (cnt > maxnb) atomicMax(&counter[CNT_OVF], cnt);
