@@ -559,18 +559,6 @@ __global__ void k_euler_step(double4 *r, double4 *v, const double4 *a, const uin
   v[s] = V;
 }
 
-__device__ __forceinline__ double wrap1(double x, double c0, double c1) {
-  // applyPeriodicCondition_F4 (kernelfuncs.cu:95-129) in FP64; a value that
-  // rounds onto the upper face is put on the lower one.
-  const double L = c1 - c0;
-  const double d0 = c0 - x;
-  if (d0 > 0) x += (double)((int)(d0 / L) + 1) * L;
-  const double d1 = x - c1;
-  if (d1 >= 0) x -= (double)((int)(d1 / L) + 1) * L;
-  if (x >= c1) x = c0;
-  return x;
-}
-
 __global__ void k_wrap(double4 *r, const uint32_t *orig, const uint32_t *waveRep,
                        const RepInfo *reps, uint32_t n) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;

@@ -326,3 +326,15 @@ __device__ __forceinline__ void pb_block_reduce_reps(const double *partials, int
     __syncthreads();
   }
 }
+
+// applyPeriodicCondition_F4 (kernelfuncs.cu:95-129) in FP64; a value that
+// rounds onto the upper face is put on the lower one.
+__device__ __forceinline__ double wrap1(double x, double c0, double c1) {
+  const double L = c1 - c0;
+  const double d0 = c0 - x;
+  if (d0 > 0) x += (double)((int)(d0 / L) + 1) * L;
+  const double d1 = x - c1;
+  if (d1 >= 0) x -= (double)((int)(d1 / L) + 1) * L;
+  if (x >= c1) x = c0;
+  return x;
+}

@@ -26,8 +26,6 @@
 int pb_ensure_sort_ws(pb_sys *s, uint32_t ncell);
 int pb_counting_sort(pb_sys *s, uint32_t ncell, const uint32_t *key);
 __global__ void k_fill_u32(uint32_t *x, uint32_t n, uint32_t val);
-__global__ void k_wrap(double4 *r, const uint32_t *orig, const uint32_t *waveRep,
-                       const RepInfo *reps, uint32_t n);
 
 enum { CNT_LAST = 0, CNT_SKIN = 1, CNT_OVF = 2, CNT_MAXNN = 3, CNT_REC = 4, CNT_DISP = 5,
        CNT_VMAX = 6, CNT_WIN = 7 };
@@ -1282,31 +1280,6 @@ extern "C" int pb_lj_force(pb_sys *s, int precision, double *pe, double *virial)
   return PB_OK;
 }
 
-// v(t-dt/2) <-> reference VV state conversions
-__global__ void k_to_half(double4 *v, const double4 *Fold, uint32_t n, double dt) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  double4 V = v[s];
-  const double4 f = Fold[s];
-  const double dtm2 = 0.5 * dt * V.w;
-  V.x += f.x * dtm2;
-  V.y += f.y * dtm2;
-  V.z += f.z * dtm2;
-  v[s] = V;
-}
-__global__ void k_from_half(double4 *v, double4 *Fold, const double4 *F, uint32_t n, double dt) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  double4 V = v[s];
-  const double4 f = F[s];
-  const double dtm2 = 0.5 * dt * V.w;
-  V.x -= f.x * dtm2;
-  V.y -= f.y * dtm2;
-  V.z -= f.z * dtm2;
-  v[s] = V;
-  Fold[s] = make_double4(f.x, f.y, f.z, 0.0);
-}
-
 // Fingerprint of the positions and the storage order (what the Verlet list
 // and the cell order depend on): a wrapping sum over slots of a 64-bit mix of
 // the slot index, the raw bits of r and orig; order-independent, so the
@@ -1316,23 +1289,15 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   return z ^ (z >> 31);
 }
-// Sum over slots of a per-slot hash (order-free, so any grid gives the same
-// value): a grid-stride loop and one atomic per workgroup, since atomics of
-// every wavefront on one address serialise (one per wavefront took 200 us at
-// 1M slots).
-__global__ void __launch_bounds__(256) k_state_hash(const double4 *r, const uint32_t *orig,
-                                                    uint32_t n, unsigned long long *out) {
+__device__ __forceinline__ uint64_t slot_hash(uint32_t s, uint32_t o, const double4 &p) {
+  uint64_t h = mix64((uint64_t)s * 0x9E3779B97F4A7C15ull ^ (uint64_t)o);
+  h = mix64(h ^ (uint64_t)__double_as_longlong(p.x));
+  h = mix64(h ^ (uint64_t)__double_as_longlong(p.y));
+  h = mix64(h ^ (uint64_t)__double_as_longlong(p.z));
+  return mix64(h ^ (uint64_t)__double_as_longlong(p.w));
+}
+__device__ __forceinline__ void hash_flush(uint64_t acc, unsigned long long *out) {
   __shared__ uint64_t part[4];
-  uint64_t acc = 0;
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    const double4 p = r[s];
-    uint64_t h = mix64((uint64_t)s * 0x9E3779B97F4A7C15ull ^ (uint64_t)orig[s]);
-    h = mix64(h ^ (uint64_t)__double_as_longlong(p.x));
-    h = mix64(h ^ (uint64_t)__double_as_longlong(p.y));
-    h = mix64(h ^ (uint64_t)__double_as_longlong(p.z));
-    h = mix64(h ^ (uint64_t)__double_as_longlong(p.w));
-    acc += h;
-  }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) acc += (uint64_t)__shfl_xor((long long)acc, o, 64);
   if (pb_lane() == 0) part[threadIdx.x / 64] = acc;
@@ -1342,15 +1307,69 @@ __global__ void __launch_bounds__(256) k_state_hash(const double4 *r, const uint
     if (t) atomicAdd(out, (unsigned long long)t);
   }
 }
+// Sum over slots of a per-slot hash (order-free, so any grid gives the same
+// value): a grid-stride loop and one atomic per workgroup, since atomics of
+// every wavefront on one address serialise (one per wavefront took 200 us at
+// 1M slots).
+__global__ void __launch_bounds__(256) k_state_hash(const double4 *r, const uint32_t *orig,
+                                                    uint32_t n, unsigned long long *out) {
+  uint64_t acc = 0;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    acc += slot_hash(s, orig[s], r[s]);
+  }
+  hash_flush(acc, out);
+}
 
-// A continued run: the periodic wrap at the end of the previous run moved
-// particles that left the box by one box length; the list and the window
-// images expect every particle near its position at the last list build,
-// so each is put back at the image nearest to that position.
-__global__ void k_unwrap_to_build(double4 *r, const float4 *rbuild, const uint32_t *orig,
-                                  const uint32_t *waveRep, const RepInfo *reps, uint32_t n) {
+// End of a fused run in one pass: kick-drift state -> reference VV state
+// (v(t), Fold = F(t)), the periodic wrap (treatPeriodicCondition), and the
+// fingerprint of the wrapped state that a next run compares against.
+__global__ void __launch_bounds__(256) k_run_end(double4 *r, double4 *v, double4 *Fold,
+                                                 const double4 *F, const uint32_t *orig,
+                                                 const uint32_t *waveRep, const RepInfo *reps,
+                                                 uint32_t n, double dt, unsigned long long *hash) {
+  uint64_t acc = 0;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    double4 V = v[s];
+    const double4 f = F[s];
+    const double dtm2 = 0.5 * dt * V.w;
+    V.x -= f.x * dtm2;
+    V.y -= f.y * dtm2;
+    V.z -= f.z * dtm2;
+    v[s] = V;
+    Fold[s] = make_double4(f.x, f.y, f.z, 0.0);
+    const uint32_t o = orig[s];
+    double4 p = r[s];
+    if (o != PB_NONE) {
+      const RepInfo &R = reps[waveRep[s / PB_WAVE]];
+      p.x = wrap1(p.x, R.lo[0], R.hi[0]);
+      p.y = wrap1(p.y, R.lo[1], R.hi[1]);
+      p.z = wrap1(p.z, R.lo[2], R.hi[2]);
+      r[s] = p;
+    }
+    acc += slot_hash(s, o, p);
+  }
+  hash_flush(acc, hash);
+}
+
+// Start of a fused run: reference VV state -> kick-drift state, and for a
+// continued list the unwrap: the periodic wrap at the end of the previous
+// run moved particles that left the box by one box length, while the list
+// and the window images expect every particle near its position at the
+// last list build, so each is put back at the image nearest to that
+// position.
+__global__ void k_run_start(double4 *v, const double4 *Fold, double4 *r, const float4 *rbuild,
+                            const uint32_t *orig, const uint32_t *waveRep, const RepInfo *reps,
+                            uint32_t n, double dt, int unwrap) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n || orig[s] == PB_NONE) return;
+  if (s >= n) return;
+  double4 V = v[s];
+  const double4 f = Fold[s];
+  const double dtm2 = 0.5 * dt * V.w;
+  V.x += f.x * dtm2;
+  V.y += f.y * dtm2;
+  V.z += f.z * dtm2;
+  v[s] = V;
+  if (!unwrap || orig[s] == PB_NONE) return;
   const RepInfo &R = reps[waveRep[s / PB_WAVE]];
   double4 p = r[s];
   const float4 b = rbuild[s];
@@ -1360,15 +1379,19 @@ __global__ void k_unwrap_to_build(double4 *r, const float4 *rbuild, const uint32
   r[s] = p;
 }
 
+static uint32_t hash_blocks(const pb_sys *s) {
+  return std::max<uint32_t>(
+      1u, std::min<uint32_t>(pb_blocks(s->npad, 256), 4u * (uint32_t)s->ctx->cu_count));
+}
+
 static int state_hash(pb_sys *s, uint64_t *h) {
   if (!s->d_hash) {
     if (int rc = pb_alloc(s->ctx, &s->d_hash, 1)) return rc;
   }
   hipStream_t st = s->ctx->stream;
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t *)s->d_hash, 2u, 0u);
-  const uint32_t nb = std::max<uint32_t>(
-      1u, std::min<uint32_t>(pb_blocks(s->npad, 256), 4u * (uint32_t)s->ctx->cu_count));
-  hipLaunchKernelGGL(k_state_hash, dim3(nb), dim3(256), 0, st, s->r, s->orig, s->npad, s->d_hash);
+  hipLaunchKernelGGL(k_state_hash, dim3(hash_blocks(s)), dim3(256), 0, st, s->r, s->orig,
+                     s->npad, s->d_hash);
   unsigned long long v = 0;
   PB_HIP(s->ctx, hipMemcpyAsync(&v, s->d_hash, 8, hipMemcpyDeviceToHost, st));
   PB_HIP(s->ctx, hipStreamSynchronize(st));
@@ -1426,6 +1449,8 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
   int skin_bad = 0;
   bool cont = false;
   int64_t end_since = 0;             // steps since the last list build at the end
+  std::vector<double> ehost;         // energy records read back at the end
+  unsigned long long end_hash = 0;   // fingerprint of the final state
   // optional HIP-event timing on the context's stream
   std::vector<hipEvent_t> ev_step, ev_graph;   // graph periods: R steps + rebuild
   // Eager steps: opts.timing = k times every k-th step and the step after
@@ -1469,15 +1494,13 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
     cont = h == s->lj_cont_hash;
   }
   s->lj_cont = false;
-  // reference VV state -> kick-drift state
-  hipLaunchKernelGGL(k_to_half, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->v, s->a,
-                     s->npad, o->dt);
+  // reference VV state -> kick-drift state (a continued list: particles
+  // put back at the images nearest their list-build positions)
+  hipLaunchKernelGGL(k_run_start, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->v, s->a,
+                     s->r, s->rbuild, s->orig, s->waveRep, s->d_reps, s->npad, o->dt, cont ? 1 : 0);
   if (!cont) {
     if ((rc = build_checked(s, false))) goto done;
     ++rebuilds;
-  } else {
-    hipLaunchKernelGGL(k_unwrap_to_build, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->r,
-                       s->rbuild, s->orig, s->waveRep, s->d_reps, s->npad);
   }
   if ((rc = reset_counters(s))) goto done;
   if (o->persistent && !s->lj_queue) {
@@ -1700,11 +1723,26 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
     if ((rc = eager(o->nsteps))) goto done;
     end_since = (int64_t)o->nsteps - last_build;
   }
-  // kick-drift state -> reference VV state: v(t) and Fold = F(t)
-  hipLaunchKernelGGL(k_from_half, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->v, s->a,
-                     s->F, s->npad, o->dt);
-  hipLaunchKernelGGL(k_wrap, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->r, s->orig,
-                     s->waveRep, s->d_reps, s->npad);
+  // kick-drift state -> reference VV state (v(t), Fold = F(t)), the wrap
+  // and the fingerprint; the fingerprint, the energies and the counters
+  // come back with one synchronisation
+  if (!s->d_hash && (rc = pb_alloc(s->ctx, &s->d_hash, 1))) goto done;
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t *)s->d_hash, 2u, 0u);
+  hipLaunchKernelGGL(k_run_end, dim3(hash_blocks(s)), dim3(256), 0, st, s->r, s->v, s->a, s->F,
+                     s->orig, s->waveRep, s->d_reps, s->npad, o->dt, s->d_hash);
+  {
+    hipError_t e = hipSuccess;
+    if (energies && nrec) {
+      ehost.resize((size_t)nrec * 8 * s->nrep);
+      e = hipMemcpyAsync(ehost.data(), ebuf, sizeof(double) * ehost.size(), hipMemcpyDeviceToHost,
+                         st);
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&end_hash, s->d_hash, 8, hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) {
+      rc = pb_hip_check(s->ctx, e, "pb_lj_run results");
+      goto done;
+    }
+  }
   {
     uint32_t c[16];
     if ((rc = read_counters(s, c))) goto done;
@@ -1720,10 +1758,8 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
     }
     if (!skin_bad) {
       // the list and cell order stay valid for a next run from this state
-      uint64_t h = 0;
-      if ((rc = state_hash(s, &h))) goto done;
       s->lj_cont = true;
-      s->lj_cont_hash = h;
+      s->lj_cont_hash = end_hash;
       s->lj_cont_rc2 = s->rc2;
       s->lj_cont_skins = skins;
       s->lj_cont_tile_w = s->tile_w;
@@ -1733,18 +1769,11 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
       s->lj_cont_v2 = c[CNT_VMAX];
     }
   }
-  if (energies && nrec) {
-    std::vector<double> h((size_t)nrec * 8 * s->nrep);
-    hipError_t e = hipMemcpyAsync(h.data(), ebuf, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) {
-      rc = pb_hip_check(s->ctx, e, "pb_lj_run energies");
-      goto done;
-    }
+  if (energies && nrec) {   // copied before the counters' synchronisation
     for (uint32_t r = 0; r < nrec; ++r)
       for (uint32_t q = 0; q < s->nrep; ++q)
         for (int c = 0; c < 3; ++c)
-          energies[((size_t)r * s->nrep + q) * 3 + c] = h[((size_t)r * s->nrep + q) * 8 + c];
+          energies[((size_t)r * s->nrep + q) * 3 + c] = ehost[((size_t)r * s->nrep + q) * 8 + c];
   }
   s->last_rebuilds = rebuilds;
   s->last_skin_violation = skin_bad;
