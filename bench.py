@@ -220,6 +220,15 @@ DEM_ALG_BYTES = 156.0        # per particle-step: FP64 r, v, w/L read + written,
 DEM_ALG_BYTES_CONTACT = 52.0  # per contact: column index + 3-double history in and out
 
 
+def write_dem_input(w, path):
+    """The dem_settle driver's input file: n (uint32), then r, radius, mass
+    and v as FP64 arrays."""
+    with open(path, "wb") as f:
+        f.write(np.array([w.n], dtype=np.uint32).tobytes())
+        for x in (w.r, w.rad, w.m, np.zeros_like(w.r)):
+            f.write(np.ascontiguousarray(x, dtype=np.float64).tobytes())
+
+
 def bench_dem(a, law="linear"):
     """DEM settle as the workload states it: 200,000 spheres, radii U[0.9,
     1.1] mm, random non-overlapping insertion in 0.1 x 0.1 x 0.4 m, at rest;
@@ -236,10 +245,8 @@ def bench_dem(a, law="linear"):
     w = workloads.dem_settle_config(20000 if a.quick else 200_000)
     exe = os.path.join(ROOT, "particlebase_amd", "bin", "dem_settle")
     with tempfile.NamedTemporaryFile(suffix=".bin", delete=False) as f:
-        f.write(np.array([w.n], dtype=np.uint32).tobytes())
-        for x in (w.r, w.rad, w.m, np.zeros_like(w.r)):
-            f.write(np.ascontiguousarray(x, dtype=np.float64).tobytes())
         path = f.name
+    write_dem_input(w, path)
     try:
         p = subprocess.run([exe, str(w.n), str(a.dem_steps), law, path], capture_output=True,
                            text=True, timeout=600)
@@ -269,12 +276,19 @@ def bench_dem(a, law="linear"):
 
 
 def load_traffic():
+    """HBM bytes per k_lj_step launch from the committed rocprofv3 counter
+    passes (FETCH_SIZE / WRITE_SIZE, calibrated by a copy of known size):
+    a stored profile figure, not measured in this run, with its source."""
     path = os.path.join(ROOT, "profiles", "lj_step_1M_traffic.json")
     if os.path.exists(path):
         with open(path) as f:
             d = json.load(f)
-        return d.get("hbm_bytes_per_launch")
-    return None
+        t = d.get("tiles", {})
+        src = (f"profiles/lj_step_1M_traffic.json: {d.get('measured', 'rocprofv3 counters')}, "
+               f"tiles {t.get('tile_w')}/{t.get('tile_fill')}, "
+               f"{d.get('kernel_avg_us', 0):.1f} us a launch there")
+        return d.get("hbm_bytes_per_launch"), src
+    return None, None
 
 
 def main():
@@ -306,12 +320,38 @@ def main():
                     help="only g(r) of the 256 batched replicas, 3 times (for counter runs)")
     ap.add_argument("--sph-only", action="store_true",
                     help="only the two SPH dam-break legs (for kernel traces / counter runs)")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only the FP32 / FP64 FMA and INT32 issue-rate probe, 3 launches each "
+                         "(calibration of counter-based VALU utilisation)")
+    ap.add_argument("--dem-input", metavar="PATH",
+                    help="write the DEM settle workload for the dem_settle driver to PATH and "
+                         "exit (no GPU; profiles run the driver on it directly)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {a.gpus} is not supported "
               f"(the multi-GPU decomposition is a design, not part of this build)",
               file=sys.stderr)
         sys.exit(2)
+
+    if a.dem_input:
+        w = workloads.dem_settle_config(20000 if a.quick else 200_000)
+        write_dem_input(w, a.dem_input)
+        log(f"wrote the DEM settle workload (N={w.n}) to {a.dem_input}")
+        return
+
+    if a.valu_probe:
+        ctx = pb.Context(0)
+        _, cus = ctx.info()
+        for kind, name in ((0, "FP32 FMA"), (1, "FP64 FMA"), (2, "INT32 xor+add"),
+                           (3, "INT32 cmp+cndmask+add")):
+            ctx.valu_probe(kind, 256)                     # warm-up
+            for _ in range(3):
+                ms, wi = ctx.valu_probe(kind, 4096)
+                rate = wi / (ms * 1e-3)
+                log(f"valu probe {name}: {wi:.4g} wave instructions in "
+                    f"{ms:.3f} ms = {rate:.4g}/s = {rate / (cus * 2.4e9):.3f} per CU-cycle at "
+                    f"2.4 GHz")
+        return
 
     if a.gr_only:
         ctx = pb.Context(0)
@@ -405,6 +445,7 @@ def main():
     log(f"fp64: {value64:.4g} atom-steps/s, step kernel {kern64 * 1e3:.1f} us")
     del s
 
+    traffic, traffic_src = load_traffic()
     out = {
         "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1e3 * el / a.steps,
@@ -415,7 +456,8 @@ def main():
                                f"re-sort every {a.rebuild_every} steps",
                    "global_batch": w.n, "seq_len": 1, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "k_lj_step (mixed, fused force+VV)",
                      "kernel_us": kern_ms * 1e3, "alg_bytes_per_atom": ALG_BYTES},
         "plain_fp64": {"value": value64, "ms_per_step": 1e3 * el64 / a.steps, "tiles": tiles64,

@@ -437,6 +437,16 @@ int pb_softcore_force(pb_sys *sys, double rc, double f0, double rmax2);
  * movable particles. */
 int pb_euler_step(pb_sys *sys, double dt);
 
+/* (no reference counterpart; diagnostic) Vector-ALU issue probe: kind 0, 1,
+ * 2 or 3 selects FP32 FMAs, FP64 FMAs, 32-bit integer xor-adds or
+ * compare-select-adds; eight
+ * independent chains per lane for iters iterations of 64 operations, 8
+ * wavefronts per SIMD.  Returns the launch's event time and the number of
+ * wave64 VALU instructions it executed, so a caller (or rocprofv3 on the same
+ * launch) calibrates the issue rate that counter-based VALU utilisations are
+ * divided by. */
+int pb_diag_valu_probe(pb_ctx *ctx, int kind, uint32_t iters, double *ms, double *wave_instr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -52,7 +52,7 @@ EXPORTS = [
     "pb_gt_step", "pb_gt_xi", "pb_adjust_velocities", "pb_const_temp",
     "pb_nlist_build", "pb_nlist_get", "pb_lj_force_nlist", "pb_softcore_force", "pb_euler_step",
     "pb_dem_set_contacts", "pb_lj_set_tiles", "pb_dem_rewind", "pb_dem_set_linear",
-    "pb_dem_history_width", "pb_gr_hist_noalloc",
+    "pb_dem_history_width", "pb_gr_hist_noalloc", "pb_diag_valu_probe",
 ]
 
 
@@ -169,6 +169,7 @@ def _extra_signatures():
         "pb_dem_set_linear": (i32, [vp, d, d, d, d, P(d), d]),
         "pb_dem_history_width": (i32, [vp, P(u32)]),
         "pb_gr_hist_noalloc": (i32, [vp, P(C.c_float), u32, P(C.c_uint64)]),
+        "pb_diag_valu_probe": (i32, [vp, i32, u32, P(d), P(d)]),
         "pb_lj_autotune": (i32, [vp, P(pb_run_opts), u32, P(d), P(u32), P(d)]),
     }
 
@@ -228,6 +229,15 @@ class Context:
 
     def sync(self):
         self.check(self.lib.pb_ctx_sync(self.h))
+
+    def valu_probe(self, kind=0, iters=4096):
+        """pb_diag_valu_probe: (ms, wave64 VALU instructions) of a launch
+        filling every SIMD with FP32 FMAs (kind 0), FP64 FMAs (1), integer
+        xor-adds (2) or compare-select-adds (3): issue-rate calibration."""
+        ms, wi = C.c_double(), C.c_double()
+        self.check(self.lib.pb_diag_valu_probe(self.h, int(kind), int(iters), C.byref(ms),
+                                               C.byref(wi)))
+        return ms.value, wi.value
 
     def close(self):
         if getattr(self, "h", None):
