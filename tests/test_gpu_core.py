@@ -265,3 +265,16 @@ def test_context_destroy_order(ctx):
     if rc == pb.PB_OK:
         c2.h = None          # the handle is gone: the Context must not destroy it again
     assert rc == pb.PB_OK
+
+
+@pytest.mark.parametrize("kind,per_step", [(0, 1), (1, 1), (2, 2), (3, 3)])
+def test_valu_probe(ctx, kind, per_step):
+    """pb_diag_valu_probe: the instruction count it reports is the loop's
+    (8 waves per SIMD, 64 chain steps an iteration, 1, 2 or 3 instructions a
+    step) and the rate is below two wave instructions per CU-cycle at the
+    2.4 GHz peak clock."""
+    _, cus = ctx.info()
+    ms, wi = ctx.valu_probe(kind, 64)
+    assert wi == 8 * cus * 4 * 64 * 64 * per_step
+    assert ms > 0
+    assert wi / (ms * 1e-3) / (cus * 2.4e9) < 2.0

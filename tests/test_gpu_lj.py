@@ -155,6 +155,28 @@ def test_continued_runs_match_oracle_loop(ctx, orc, persistent):
     assert s.lj_run_stats()["rebuilds"] == 1
 
 
+def test_sampled_timing(ctx):
+    """opts.timing = 4 times steps 0, 4, 8, ... and the step after each
+    rebuild (forced every 5 steps here), and every rebuild; the results are
+    those of an untimed run bit for bit."""
+    say("sampled step timing (timing=4), 20 steps, rebuild every 5, vs untimed")
+    w = workloads.lj_config1()
+    runs = []
+    for timing in (4, 0):
+        s = lj_sys(ctx, w)
+        s.clear_force(True)
+        s.lj_run(0.005, 20, precision=pb.PB_MIXED, skin=0.8, rebuild_every=5, energy_every=0,
+                 timing=timing)
+        assert s.lj_run_stats()["rebuilds"] == 4       # the start + steps 5, 10, 15
+        runs.append((s.get("r"), s.get("v"), s.lj_run_timing() if timing else None))
+        s.close()
+    t = runs[0][2]
+    assert t["nstep"] == 8                             # steps 0 4 5 8 10 12 15 16
+    assert t["nrebuild"] == 3
+    assert t["step_ms"] > 0 and t["rebuild_ms"] > 0
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+
+
 def test_fused_run_equals_unfused_api(ctx):
     say("fused run vs calcForce/TimeEvolution/treatPeriodic loop, 30 steps")
     w = workloads.lj_fcc(6, seed=7)
