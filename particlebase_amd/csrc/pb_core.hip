@@ -237,6 +237,7 @@ extern "C" int pb_sys_set_cell(pb_sys *s, const double *cells) {
   }
   std::memcpy(s->cells.data(), cells, sizeof(double) * 6 * s->nrep);
   s->cell_set = true;
+  s->lj_cont = false;             // the fused LJ run's list cells follow the box
   return upload_reps(s);
 }
 

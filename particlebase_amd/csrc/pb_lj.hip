@@ -1926,6 +1926,7 @@ out:
     s->tile_w = w0;
     s->tile_fill = f0;
   }
+  s->lj_cont = false;             // the restored state has no list of its own
   (void)hipStreamSynchronize(st);
   pb_free(sr); pb_free(sv); pb_free(sa); pb_free(sf); pb_free(so); pb_free(sty); pb_free(smv);
   return rc;

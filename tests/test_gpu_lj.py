@@ -153,6 +153,16 @@ def test_continued_runs_match_oracle_loop(ctx, orc, persistent):
     s.lj_run(dt, 1, precision=pb.PB_FP64, skin=0.8, rebuild_every=40, energy_every=0,
              persistent=persistent)
     assert s.lj_run_stats()["rebuilds"] == 1
+    # the box changed (positions untouched): the list cells follow the box
+    s.lj_run(dt, 1, precision=pb.PB_FP64, skin=0.8, rebuild_every=40, energy_every=0,
+             persistent=persistent)
+    assert s.lj_run_stats()["rebuilds"] == 0
+    cell = np.array(w.cell, dtype=np.float64).copy()
+    cell[1::2] += 0.05
+    s.set_cell(cell)
+    s.lj_run(dt, 1, precision=pb.PB_FP64, skin=0.8, rebuild_every=40, energy_every=0,
+             persistent=persistent)
+    assert s.lj_run_stats()["rebuilds"] == 1
 
 
 def test_sampled_timing(ctx):
