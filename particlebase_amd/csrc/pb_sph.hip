@@ -151,10 +151,17 @@ __device__ __forceinline__ void sph_pairs(const SphArgs &A, uint32_t zseg, SphFi
   __shared__ double L[NF][CAP];
   __shared__ float4 Fq[CAP + 64];             // filter view; 64 spare for the last step
   const uint32_t nz = A.g.nb[2], ny = A.g.nb[1];
-  const uint32_t col = blockIdx.x;
+  const uint32_t ncol = A.g.nb[0] * ny;
+  // 1-D grid of (z segment, column), columns fastest.  Workgroups go to the
+  // XCDs round-robin; within every 64 workgroups XCD x takes 8 consecutive
+  // columns, which share neighbour columns in its L2, while the fine
+  // interleave keeps dense and empty regions spread over all XCDs.
+  const uint32_t b = blockIdx.x, full = gridDim.x & ~63u;
+  const uint32_t lb = b < full ? (b & ~63u) + (b & 7u) * 8u + ((b >> 3) & 7u) : b;
+  const uint32_t col = lb % ncol;
   const int ci = (int)(col / ny), cj = (int)(col % ny);
   // this workgroup's cells: z in [zs, ze) of column col
-  const uint32_t zs = blockIdx.y * zseg, ze = zs + zseg < nz ? zs + zseg : nz;
+  const uint32_t zs = (lb / ncol) * zseg, ze = zs + zseg < nz ? zs + zseg : nz;
   const uint32_t s0 = A.cstart[col * nz + zs], s1 = A.cstart[col * nz + ze];
   if (s0 >= s1) return;
   const double4 O = A.rs[s0];                 // origin of the filter coordinates
@@ -426,7 +433,7 @@ static dim3 sph_pair_grid(pb_sys *s, uint32_t *zseg) {
   uint32_t z = (uint32_t)std::lround(64.0 / (occ > 1e-9 ? occ : 1e-9));
   z = z < 1 ? 1 : (z > nz ? nz : z);
   *zseg = z;
-  return dim3(s->blocknum[0] * s->blocknum[1], (nz + z - 1) / z);
+  return dim3(s->blocknum[0] * s->blocknum[1] * ((nz + z - 1) / z));
 }
 
 // FP32 filter bound: r2max plus 1e-6 Lmax^2, where Lmax bounds the filter
