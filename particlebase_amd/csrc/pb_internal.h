@@ -265,14 +265,14 @@ __device__ __forceinline__ double pb_wave_max(double x) {
   return x;
 }
 
-// XCD-aware block index: consecutive logical blocks go to the same XCD so
-// each XCD's L2 holds one contiguous slab of particles.  Bijective for any
-// grid size (MI355X dispatches workgroups round-robin over 8 XCDs).
-__device__ __forceinline__ uint32_t pb_xcd_block() {
-  const uint32_t b = blockIdx.x, nb = gridDim.x;
-  const uint32_t q = nb >> 3, rmd = nb & 7u, x = b & 7u;
-  const uint32_t start = x * q + (x < rmd ? x : rmd);
-  return start + (b >> 3);
+// XCD-aware logical block of a 1-D grid (MI355X dispatches workgroups
+// round-robin over its 8 XCDs, each with its own L2): within every 64
+// workgroups, XCD x takes 8 consecutive logical blocks (the tail past the
+// last whole 64 keeps its order).  Neighbouring blocks share an XCD's L2 in eights while
+// dense and sparse regions stay spread over all XCDs.  Bijective.
+__device__ __forceinline__ uint32_t pb_xcd_chunk8() {
+  const uint32_t b = blockIdx.x, full = gridDim.x & ~63u;
+  return b < full ? (b & ~63u) + (b & 7u) * 8u + ((b >> 3) & 7u) : b;
 }
 
 __device__ __forceinline__ double pb_load_relaxed(const double *p) {

@@ -156,8 +156,7 @@ __device__ __forceinline__ void sph_pairs(const SphArgs &A, uint32_t zseg, SphFi
   // XCDs round-robin; within every 64 workgroups XCD x takes 8 consecutive
   // columns, which share neighbour columns in its L2, while the fine
   // interleave keeps dense and empty regions spread over all XCDs.
-  const uint32_t b = blockIdx.x, full = gridDim.x & ~63u;
-  const uint32_t lb = b < full ? (b & ~63u) + (b & 7u) * 8u + ((b >> 3) & 7u) : b;
+  const uint32_t lb = pb_xcd_chunk8();
   const uint32_t col = lb % ncol;
   const int ci = (int)(col / ny), cj = (int)(col % ny);
   // this workgroup's cells: z in [zs, ze) of column col
