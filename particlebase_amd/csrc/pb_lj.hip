@@ -216,6 +216,8 @@ struct Stage {              // pair kernel: FP64 positions with shifts applied
   double x[WCAP + 1], y[WCAP + 1], z[WCAP + 1];   // slot WCAP: padding target
   uint8_t t[WCAP + 1];
   WinTable W;
+  unsigned long long hsum[16];  // last step of a run: fingerprint sums (lane mod 16, so
+                                // four lanes of a wavefront share an LDS atomic address)
 };
 #define WCAPB WCAP          // build kernel stage (FP32, 16 B a particle): three workgroups per CU
 struct StageB {             // list build: FP32 window-relative positions, 12 B a
@@ -501,6 +503,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
     auto scan_staged = [&](uint32_t q0, uint32_t q1, bool mi) {
       const float2 pixy = S.xy[li];
       const float piz = S.z[li];
+      // |r_iq|^2 - rb2 as raw bits: the sign bit is set when q is near
       auto near = [&](uint32_t q) {
         const float2 pj = S.xy[q];
         float dx = pixy.x - pj.x, dy = pixy.y - pj.y, dz = piz - S.z[q];
@@ -509,14 +512,15 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
           dy -= Ly * rintf(dy * Iy);
           dz -= Lz * rintf(dz * Iz);
         }
-        return dx * dx + dy * dy + dz * dz < rb2;
+        return __float_as_uint(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, -rb2))));
       };
       for (uint32_t c0 = q0; c0 < q1; c0 += BUILD_GROUP) {
         // test k shifts in at bit 0, so candidate c0 + k ends at bit
-        // BUILD_GROUP - 1 - k
+        // BUILD_GROUP - 1 - k: alignbit(m, t, 31) = (m << 1) | (t >> 31),
+        // one instruction a candidate for the compare and the mask
         uint32_t m = 0;
 #pragma unroll BUILD_UNROLL
-        for (uint32_t k = 0; k < BUILD_GROUP; ++k) m = m + m + (near(c0 + k) ? 1u : 0u);
+        for (uint32_t k = 0; k < BUILD_GROUP; ++k) m = __builtin_amdgcn_alignbit(m, near(c0 + k), 31);
         constexpr uint32_t top = 1u << (BUILD_GROUP - 1u);
         if (q1 - c0 < BUILD_GROUP) m &= ~((top >> (q1 - c0 - 1u)) - 1u);
         if (li - c0 < BUILD_GROUP) m &= ~(top >> (li - c0));   // li < c0 wraps past the group
@@ -587,6 +591,39 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
   if (pb_lane() == 0 && mymax) atomicMax(&counter[CNT_MAXNN], mymax);
 }
 
+// Fingerprint of the positions and the storage order (what the Verlet list
+// and the cell order depend on): a wrapping sum over slots of a 64-bit mix of
+// the slot index, the raw bits of r and orig; order-independent, so the
+// atomics leave it deterministic.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ uint64_t slot_hash(uint32_t s, uint32_t o, const double4 &p) {
+  uint64_t h = mix64((uint64_t)s * 0x9E3779B97F4A7C15ull ^ (uint64_t)o);
+  h = mix64(h ^ (uint64_t)__double_as_longlong(p.x));
+  h = mix64(h ^ (uint64_t)__double_as_longlong(p.y));
+  h = mix64(h ^ (uint64_t)__double_as_longlong(p.z));
+  return mix64(h ^ (uint64_t)__double_as_longlong(p.w));
+}
+// The sum is spread over HASH_SLOTS addresses (workgroup b adds to slot
+// b mod HASH_SLOTS) and the host adds the slots: atomics on one address
+// serialise in its L2 channel.  Any workgroup size up to 1024.
+#define HASH_SLOTS 64u
+__device__ __forceinline__ void hash_flush(uint64_t acc, unsigned long long *out) {
+  __shared__ uint64_t part[16];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) acc += (uint64_t)__shfl_xor((long long)acc, o, 64);
+  if (pb_lane() == 0) part[threadIdx.x / 64] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (uint32_t w = 0; w < blockDim.x / 64; ++w) t += part[w];
+    if (t) atomicAdd(out + blockIdx.x % HASH_SLOTS, (unsigned long long)t);
+  }
+}
+
 // ------------------------------------------------------------- step kernel
 struct StepArgs {
   const double4 *rin, *vin;
@@ -610,6 +647,10 @@ struct StepArgs {
   // single-type parameters
   double s2, e24, e4, shift;
   float s2f, e24f, e4f, shiftf;
+  // last step of a run
+  double4 *Aout;            // F(t) again (the VV state's Fold)
+  const uint32_t *orig;     // slot -> particle (fingerprint)
+  unsigned long long *hash; // HASH_SLOTS fingerprint sums
 };
 
 __device__ __forceinline__ double rcp64(double x) {
@@ -647,7 +688,8 @@ __device__ __forceinline__ void publish_prev(const StepArgs &A) {
 template <bool FAST, bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
 __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, const ListSeg &G,
                                          const RepInfo &R, bool staged, bool mi, double &ke, double &pe_b,
-                                         double &vir_b, float &d2m, float &v2m) {
+                                         double &vir_b, float &d2m, float &v2m,
+                                         unsigned long long *hsum) {
   const uint32_t ntile = S.W.rpref[S.W.nrun];
   const double L0 = R.L[0], L1 = R.L[1], L2 = R.L[2];
   const double I0 = R.Linv[0], I1 = R.Linv[1], I2 = R.Linv[2];
@@ -684,11 +726,13 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
   };
   // the next group's integration state, needed at the end of its iteration:
   // requested at the end of the current one (one set of registers live)
+  uint32_t norig = 0;
   auto fetch_state = [&](uint32_t ub) {
     if (INTEGRATE && sl == 0 && ub + grp < ntile) {
       np_own = A.rin[ngi];
       nv_own = A.vin[ngi];
       nrb_own = A.rbuild[ngi];
+      if (WRITE_F) norig = A.orig[ngi];
     }
   };
   fetch_list(wave * APW);
@@ -700,6 +744,7 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
     uint64_t qw = nq;
     const double4 p_own = np_own, v_own = nv_own;
     const float4 rb_own = nrb_own;
+    const uint32_t o_own = norig;
     fetch_list(ub + nwv * APW);
     double4 pi;
     if (FAST) pi = make_double4(S.x[li], S.y[li], S.z[li], (double)S.t[li]);
@@ -828,12 +873,26 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
       p.x += vi.x * A.dt;
       p.y += vi.y * A.dt;
       p.z += vi.z * A.dt;
-      A.rout[gi] = p;
-      A.vout[gi] = vi;
+      if (!WRITE_F) {
+        A.rout[gi] = p;
+        A.vout[gi] = vi;
+      }
       // displacement of the new positions feeds the rebuild schedule
       // in units of the replica's half skin (replicas may have different skins)
       d2m = fmaxf(d2m, disp2(p) * inv_hs2);
       v2m = fmaxf(v2m, (float)(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z) * inv_hs2);
+      if (WRITE_F) {
+        // last step of a run: the reference VV state r(t+dt), v(t), Fold =
+        // F(t), positions wrapped into the box (treatPeriodicCondition), and
+        // the fingerprint of the slot that a next run compares against
+        p.x = wrap1(p.x, R.lo[0], R.hi[0]);
+        p.y = wrap1(p.y, R.lo[1], R.hi[1]);
+        p.z = wrap1(p.z, R.lo[2], R.hi[2]);
+        A.rout[gi] = p;
+        A.vout[gi] = make_double4(vx, vy, vz, vi.w);
+        A.Aout[gi] = make_double4(fx, fy, fz, 0.0);
+        atomicAdd(hsum + (lane & 15u), (unsigned long long)slot_hash(gi, o_own, p));   // LDS
+      }
     }
     if (WRITE_F) A.Fout[gi] = make_double4(fx, fy, fz, 0.0);
     if (ENERGY) {
@@ -850,6 +909,7 @@ __device__ __forceinline__ void lj_segment(const StepArgs &A, uint32_t seg) {
   __shared__ Stage S;
   const ListSeg G = A.segs[seg];
   const RepInfo &R = A.reps[G.rep];
+  if (WRITE_F && INTEGRATE && threadIdx.x < 16) S.hsum[threadIdx.x] = 0;   // visible after the staging
   stage_window(S, G, R, A.cstart, A.rin);
   const bool staged = S.W.total <= WCAP;
   const bool mi = G.whole != 0;
@@ -858,10 +918,10 @@ __device__ __forceinline__ void lj_segment(const StepArgs &A, uint32_t seg) {
   float d2m = 0.f, v2m = 0.f;
   if (staged && !mi)          // block-uniform: the common case, branch-free pairs
     lj_tiles<true, MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE>(A, S, G, R, staged, mi, ke, pe_b, vir_b,
-                                                                d2m, v2m);
+                                                                d2m, v2m, S.hsum);
   else
     lj_tiles<false, MIXED, ENERGY, WRITE_F, ONE_TYPE, INTEGRATE>(A, S, G, R, staged, mi, ke, pe_b,
-                                                                 vir_b, d2m, v2m);
+                                                                 vir_b, d2m, v2m, S.hsum);
 
   __shared__ float smx[2][TBS / 64];
   __shared__ double sen[3][TBS / 64];
@@ -889,6 +949,11 @@ __device__ __forceinline__ void lj_segment(const StepArgs &A, uint32_t seg) {
   __syncthreads();
   if (threadIdx.x == 0) {
     const uint32_t nw = blockDim.x / 64;
+    if (WRITE_F && INTEGRATE) {
+      unsigned long long t = 0;
+      for (int k = 0; k < 16; ++k) t += S.hsum[k];
+      if (t) atomicAdd(A.hash + blockIdx.x % HASH_SLOTS, t);
+    }
     if (INTEGRATE) {
       float bd = 0.f, bv = 0.f;
       for (uint32_t w = 0; w < nw; ++w) {
@@ -1283,33 +1348,6 @@ extern "C" int pb_lj_force(pb_sys *s, int precision, double *pe, double *virial)
   return PB_OK;
 }
 
-// Fingerprint of the positions and the storage order (what the Verlet list
-// and the cell order depend on): a wrapping sum over slots of a 64-bit mix of
-// the slot index, the raw bits of r and orig; order-independent, so the
-// atomics leave it deterministic.
-__device__ __forceinline__ uint64_t mix64(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-  return z ^ (z >> 31);
-}
-__device__ __forceinline__ uint64_t slot_hash(uint32_t s, uint32_t o, const double4 &p) {
-  uint64_t h = mix64((uint64_t)s * 0x9E3779B97F4A7C15ull ^ (uint64_t)o);
-  h = mix64(h ^ (uint64_t)__double_as_longlong(p.x));
-  h = mix64(h ^ (uint64_t)__double_as_longlong(p.y));
-  h = mix64(h ^ (uint64_t)__double_as_longlong(p.z));
-  return mix64(h ^ (uint64_t)__double_as_longlong(p.w));
-}
-__device__ __forceinline__ void hash_flush(uint64_t acc, unsigned long long *out) {
-  __shared__ uint64_t part[4];
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) acc += (uint64_t)__shfl_xor((long long)acc, o, 64);
-  if (pb_lane() == 0) part[threadIdx.x / 64] = acc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint64_t t = part[0] + part[1] + part[2] + part[3];
-    if (t) atomicAdd(out, (unsigned long long)t);
-  }
-}
 // Sum over slots of a per-slot hash (order-free, so any grid gives the same
 // value): a grid-stride loop and one atomic per workgroup, since atomics of
 // every wavefront on one address serialise (one per wavefront took 200 us at
@@ -1318,40 +1356,10 @@ __global__ void __launch_bounds__(256) k_state_hash(const double4 *r, const uint
                                                     uint32_t n, unsigned long long *out) {
   uint64_t acc = 0;
   for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    acc += slot_hash(s, orig[s], r[s]);
+    const uint32_t o = orig[s];
+    if (o != PB_NONE) acc += slot_hash(s, o, r[s]);
   }
   hash_flush(acc, out);
-}
-
-// End of a fused run in one pass: kick-drift state -> reference VV state
-// (v(t), Fold = F(t)), the periodic wrap (treatPeriodicCondition), and the
-// fingerprint of the wrapped state that a next run compares against.
-__global__ void __launch_bounds__(256) k_run_end(double4 *r, double4 *v, double4 *Fold,
-                                                 const double4 *F, const uint32_t *orig,
-                                                 const uint32_t *waveRep, const RepInfo *reps,
-                                                 uint32_t n, double dt, unsigned long long *hash) {
-  uint64_t acc = 0;
-  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-    double4 V = v[s];
-    const double4 f = F[s];
-    const double dtm2 = 0.5 * dt * V.w;
-    V.x -= f.x * dtm2;
-    V.y -= f.y * dtm2;
-    V.z -= f.z * dtm2;
-    v[s] = V;
-    Fold[s] = make_double4(f.x, f.y, f.z, 0.0);
-    const uint32_t o = orig[s];
-    double4 p = r[s];
-    if (o != PB_NONE) {
-      const RepInfo &R = reps[waveRep[s / PB_WAVE]];
-      p.x = wrap1(p.x, R.lo[0], R.hi[0]);
-      p.y = wrap1(p.y, R.lo[1], R.hi[1]);
-      p.z = wrap1(p.z, R.lo[2], R.hi[2]);
-      r[s] = p;
-    }
-    acc += slot_hash(s, o, p);
-  }
-  hash_flush(acc, hash);
 }
 
 // Start of a fused run: reference VV state -> kick-drift state, and for a
@@ -1382,23 +1390,37 @@ __global__ void k_run_start(double4 *v, const double4 *Fold, double4 *r, const f
   r[s] = p;
 }
 
+// grid of the hash pass: up to 4 workgroups of 256 per CU, a grid-stride
+// loop (measured faster than one slot a thread: 10.4 against 16.4 us at 1M)
 static uint32_t hash_blocks(const pb_sys *s) {
   return std::max<uint32_t>(
       1u, std::min<uint32_t>(pb_blocks(s->npad, 256), 4u * (uint32_t)s->ctx->cu_count));
 }
 
-static int state_hash(pb_sys *s, uint64_t *h) {
+static int hash_slots_ready(pb_sys *s) {
   if (!s->d_hash) {
-    if (int rc = pb_alloc(s->ctx, &s->d_hash, 1)) return rc;
+    if (int rc = pb_alloc(s->ctx, &s->d_hash, HASH_SLOTS)) return rc;
   }
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(2 * HASH_SLOTS), 0, s->ctx->stream,
+                     (uint32_t *)s->d_hash, 2u * HASH_SLOTS, 0u);
+  return PB_OK;
+}
+
+static uint64_t hash_sum(const unsigned long long *slots) {
+  uint64_t h = 0;
+  for (uint32_t k = 0; k < HASH_SLOTS; ++k) h += slots[k];
+  return h;
+}
+
+static int state_hash(pb_sys *s, uint64_t *h) {
+  if (int rc = hash_slots_ready(s)) return rc;
   hipStream_t st = s->ctx->stream;
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t *)s->d_hash, 2u, 0u);
   hipLaunchKernelGGL(k_state_hash, dim3(hash_blocks(s)), dim3(256), 0, st, s->r, s->orig,
                      s->npad, s->d_hash);
-  unsigned long long v = 0;
-  PB_HIP(s->ctx, hipMemcpyAsync(&v, s->d_hash, 8, hipMemcpyDeviceToHost, st));
+  unsigned long long v[HASH_SLOTS];
+  PB_HIP(s->ctx, hipMemcpyAsync(v, s->d_hash, sizeof(v), hipMemcpyDeviceToHost, st));
   PB_HIP(s->ctx, hipStreamSynchronize(st));
-  *h = v;
+  *h = hash_sum(v);
   return PB_OK;
 }
 
@@ -1453,7 +1475,7 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
   bool cont = false;
   int64_t end_since = 0;             // steps since the last list build at the end
   std::vector<double> ehost;         // energy records read back at the end
-  unsigned long long end_hash = 0;   // fingerprint of the final state
+  unsigned long long end_slots[HASH_SLOTS];   // fingerprint of the final state (slot sums)
   // optional HIP-event timing on the context's stream
   std::vector<hipEvent_t> ev_step, ev_graph;   // graph periods: R steps + rebuild
   // Eager steps: opts.timing = k times every k-th step and the step after
@@ -1497,6 +1519,8 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
     cont = h == s->lj_cont_hash;
   }
   s->lj_cont = false;
+  // the run's last step adds the fingerprint of the final state here
+  if ((rc = hash_slots_ready(s))) goto done;
   // reference VV state -> kick-drift state (a continued list: particles
   // put back at the images nearest their list-build positions)
   hipLaunchKernelGGL(k_run_start, dim3(pb_blocks(s->npad, 256)), dim3(256), 0, st, s->v, s->a,
@@ -1598,6 +1622,9 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
       A.rout = s->r2buf;
       A.vout = s->v2buf;
       A.Fout = s->F;
+      A.Aout = s->a;
+      A.orig = s->orig;
+      A.hash = s->d_hash;
       A.step = step;
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (tstride && (reb_from || step % tstride == 0)) {
@@ -1726,13 +1753,9 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
     if ((rc = eager(o->nsteps))) goto done;
     end_since = (int64_t)o->nsteps - last_build;
   }
-  // kick-drift state -> reference VV state (v(t), Fold = F(t)), the wrap
-  // and the fingerprint; the fingerprint, the energies and the counters
-  // come back with one synchronisation
-  if (!s->d_hash && (rc = pb_alloc(s->ctx, &s->d_hash, 1))) goto done;
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t *)s->d_hash, 2u, 0u);
-  hipLaunchKernelGGL(k_run_end, dim3(hash_blocks(s)), dim3(256), 0, st, s->r, s->v, s->a, s->F,
-                     s->orig, s->waveRep, s->d_reps, s->npad, o->dt, s->d_hash);
+  // the last step left the reference VV state, the wrap and the
+  // fingerprint; the fingerprint, the energies and the counters come back
+  // with one synchronisation
   {
     hipError_t e = hipSuccess;
     if (energies && nrec) {
@@ -1740,7 +1763,8 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
       e = hipMemcpyAsync(ehost.data(), ebuf, sizeof(double) * ehost.size(), hipMemcpyDeviceToHost,
                          st);
     }
-    if (e == hipSuccess) e = hipMemcpyAsync(&end_hash, s->d_hash, 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(end_slots, s->d_hash, sizeof(end_slots), hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) {
       rc = pb_hip_check(s->ctx, e, "pb_lj_run results");
       goto done;
@@ -1762,7 +1786,7 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
     if (!skin_bad) {
       // the list and cell order stay valid for a next run from this state
       s->lj_cont = true;
-      s->lj_cont_hash = end_hash;
+      s->lj_cont_hash = hash_sum(end_slots);
       s->lj_cont_rc2 = s->rc2;
       s->lj_cont_skins = skins;
       s->lj_cont_tile_w = s->tile_w;
