@@ -168,7 +168,14 @@ __global__ void k_reorder(ReorderArgs A) {
 struct ListSeg {
   uint32_t rep, ci, cj, ck0, nz, tx, ty, whole;
 };
-#define TB 384            // list-build workgroup (6 waves)
+// list-build workgroup: 8 waves.  A segment holds about as many tile
+// particles as the step kernel's 512 lanes take in one pass (415 on the 1M
+// fluid), one particle a lane here as well: with 384 lanes the build made a
+// second pass for the last few particles of most segments (340 us at 1M
+// atoms; 448 lanes: 326 us; 512: 288 us)
+#ifndef TB
+#define TB 512
+#endif
 #ifndef BUILD_GROUP
 #define BUILD_GROUP 32u   // candidates tested per mask step of the list build (<= 32)
 #endif
@@ -219,7 +226,7 @@ struct Stage {              // pair kernel: FP64 positions with shifts applied
   unsigned long long hsum[16];  // last step of a run: fingerprint sums (lane mod 16, so
                                 // four lanes of a wavefront share an LDS atomic address)
 };
-#define WCAPB WCAP          // build kernel stage (FP32, 16 B a particle): three workgroups per CU
+#define WCAPB WCAP          // build kernel stage (FP32, 12 B a particle): four workgroups per CU
 struct StageB {             // list build: FP32 window-relative positions, 12 B a
   float2 xy[WCAPB + 32];     // particle (four workgroups per CU); 32 slack
   float z[WCAPB + 32];       // entries read by the last group of 32 tests
@@ -426,9 +433,13 @@ __device__ __forceinline__ void tile_particle(const WinTable &W, uint32_t u, uin
 // the format's inert entry to whole batches (nn holds the padded length).
 // 12-bit entries move 20 % fewer list bytes per step than 16-bit ones
 // (measured: the step kernel 159 -> 151 us at 1M atoms).
-// six wavefronts per SIMD (<= 80 VGPRs with the granule of 8): with the
-// 38.6 KB stage, four workgroups per CU
-__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_build_list(const double4 *r, const uint32_t *cstart,
+// eight wavefronts per SIMD (64 VGPRs; the few values spilled are per
+// particle, outside the candidate loops): with the 38.6 KB stage, four
+// 512-thread workgroups per CU (six per SIMD, three workgroups: 297 us)
+#ifndef BUILD_WPE
+#define BUILD_WPE 8
+#endif
+__global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BUILD_WPE))) k_build_list(const double4 *r, const uint32_t *cstart,
                                                    const ListSeg *segs, const RepInfo *reps,
                                                    uint32_t npad, uint32_t maxnb, uint16_t *nl,
                                                    uint32_t *nn, uint32_t *counter) {
@@ -473,26 +484,42 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
     uint32_t li, gi, run;
     tile_particle(W, u, li, gi, run);
     // Kept entries are packed epw to a 64-bit word in registers; word m of
-    // slot gi is written whole at wrow[m * npad].  Words past the capacity
-    // land in the slack word wcap (the allocation holds maxnb + 48 entries
-    // per slot); an overflow is reported and the list rebuilt larger.
+    // slot gi is written whole at wrow[m * npad], for m < maxnb / epw (the
+    // allocation holds maxnb + 48 entries per slot: slack words for the
+    // step kernel's read one batch ahead).
     uint64_t *wrow = reinterpret_cast<uint64_t *>(nl) + gi;
     const uint32_t wcap = maxnb / epw;
-    // Entries enter the word at its top and move down eb bits per append
-    // (constant shifts; a word of epw entries then holds the first at bit
-    // 0).  The store pointer advances one word per completed word up to the
-    // slack word.
-    uint64_t word = 0;
-    uint32_t cnt = 0, kw = 0, mw = 0;
+    // 12-bit format: an entry enters the word at bit 0 and the word moves up
+    // 12 bits per append, behind a marker bit that starts at bit 0; after
+    // five appends the marker reaches bit 60, above the five fields (readers
+    // mask the fields), and the word is complete: one compare of the high
+    // half instead of an entry counter.  16-bit format (no spare bit):
+    // entries enter at the top, move down 16 bits, and a counter completes
+    // the word.  The store pointer advances one word per completed word;
+    // words past the capacity are not stored (an overflow is reported and
+    // the list rebuilt larger).  cnt is advanced per group of tests
+    // (popcount).
+    uint64_t word = epw == 5u ? 1ull : 0ull;
+    uint32_t cnt = 0, kw = 0;
     uint64_t *wp = wrow;
+    const uint64_t *const wend = wrow + (size_t)wcap * npad;
+    auto store_word = [&]() {
+      if (wp < wend) *wp = word;
+      wp += npad;
+    };
     auto append = [&](uint32_t q) {
-      word = (word >> eb) | ((uint64_t)q << (eb * (epw - 1u)));
-      ++cnt;
-      if (++kw == epw) {
-        *wp = word;
-        kw = 0;
-        if (mw < wcap) wp += npad;
-        ++mw;
+      if constexpr (epw == 5u) {
+        word = (word << 12) + q;                 // the low field is zero after the shift
+        if ((uint32_t)(word >> 32) >= (1u << 28)) {
+          store_word();
+          word = 1ull;
+        }
+      } else {
+        word = (word >> eb) | ((uint64_t)q << (eb * (epw - 1u)));
+        if (++kw == epw) {
+          store_word();
+          kw = 0;
+        }
       }
     };
     // staged candidates [q0, q1) except li: 32 distance tests into a bit
@@ -524,6 +551,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
         constexpr uint32_t top = 1u << (BUILD_GROUP - 1u);
         if (q1 - c0 < BUILD_GROUP) m &= ~((top >> (q1 - c0 - 1u)) - 1u);
         if (li - c0 < BUILD_GROUP) m &= ~(top >> (li - c0));   // li < c0 wraps past the group
+        cnt += __popc(m);
         while (m) {
           const uint32_t k = (uint32_t)__builtin_clz(m) - (32u - BUILD_GROUP);
           append(c0 + k);
@@ -542,7 +570,10 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
           dy -= R.L[1] * rint(dy * R.Linv[1]);
           dz -= R.L[2] * rint(dz * R.Linv[2]);
         }
-        if (dx * dx + dy * dy + dz * dz < rlist2) append(q);
+        if (dx * dx + dy * dy + dz * dz < rlist2) {
+          ++cnt;
+          append(q);
+        }
       }
     };
     // candidates [q0, q1) except the particle itself
@@ -576,8 +607,19 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(6))) k_
     const uint32_t used = cnt < maxnb ? cnt : maxnb;
     const uint32_t batch = epw * LPA;
     const uint32_t padded = (used + batch - 1u) / batch * batch;
-    if (kw != 0u && cnt < maxnb)
-      wrow[(size_t)(cnt / epw) * npad] = (word >> (eb * (epw - kw))) | (padw & (~0ull << (eb * kw)));
+    if (cnt % epw != 0u && cnt < maxnb) {
+      // the partial word: its kr entries moved to the word's top fields, the
+      // rest filled with the inert entry
+      const uint32_t kr = cnt % epw;
+      uint64_t last;
+      if constexpr (epw == 5u) {
+        const uint32_t sh = 12u * (5u - kr);
+        last = (word << sh) | (padw & ((1ull << sh) - 1ull));
+      } else {
+        last = (word >> (eb * (epw - kr))) | (padw & (~0ull << (eb * kr)));
+      }
+      wrow[(size_t)(cnt / epw) * npad] = last;
+    }
     for (uint32_t m = (used + epw - 1u) / epw; m < padded / epw; ++m) wrow[(size_t)m * npad] = padw;
     nn[gi] = padded;
     if (cnt > maxnb) atomicMax(&counter[CNT_OVF], cnt);
