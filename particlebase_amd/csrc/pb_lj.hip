@@ -183,7 +183,9 @@ struct ListSeg {
 #define BUILD_UNROLL 4
 #endif
 #define TBS 512           // step workgroup (8 waves; two per CU at <= 128 VGPRs)
+#ifndef WCAP
 #define WCAP 2816         // staged window particles per workgroup (LDS)
+#endif
 #define MAXCOL 16         // (tx+2)*(ty+2) window columns, tx, ty <= 2
 #define MAXZ 16           // nz <= MAXZ
 #define MAXE (MAXCOL * (MAXZ + 2))
@@ -1134,7 +1136,7 @@ static int setup_list_cells(pb_sys *s, double rc, const std::vector<double> &ski
     if (R.nb[0] < 3 || R.nb[1] < 3 || R.nb[2] < 3) {
       if (R.natoms > WCAP)
         return pb_fail(s->ctx, PB_ERR_CELLS,
-                       "a replica thinner than 3 list cells must hold at most 2816 particles");
+                       "a replica thinner than 3 list cells must fit one LDS window");
       segs.push_back(ListSeg{q, 0, 0, 0, 1, 1, 1, 1});
     } else {
       const uint32_t tw = s->tile_w;
@@ -1143,11 +1145,19 @@ static int setup_list_cells(pb_sys *s, double rc, const std::vector<double> &ski
       int nz = (int)std::floor(s->tile_fill * WCAP /
                                ((tx + 2) * (ty + 2) * std::max(per_cell, 1e-3))) - 2;
       nz = std::max(1, std::min<int>({nz, MAXZ, (int)R.nb[2] - 2}));
+      // the column split into the fewest z runs of at most nz cells, of
+      // equal length to within one cell (a short last run would be a
+      // workgroup with a few busy wavefronts)
+      const uint32_t nrun = (R.nb[2] + (uint32_t)nz - 1u) / (uint32_t)nz;
+      const uint32_t zb = R.nb[2] / nrun, zr = R.nb[2] % nrun;
       for (uint32_t ci = 0; ci < R.nb[0]; ci += tx)
         for (uint32_t cj = 0; cj < R.nb[1]; cj += ty)
-          for (uint32_t ck = 0; ck < R.nb[2]; ck += (uint32_t)nz)
-            segs.push_back(ListSeg{q, ci, cj, ck, std::min<uint32_t>((uint32_t)nz, R.nb[2] - ck),
-                                   std::min(tx, R.nb[0] - ci), std::min(ty, R.nb[1] - cj), 0});
+          for (uint32_t k = 0, ck = 0; k < nrun; ++k) {
+            const uint32_t len = zb + (k < zr ? 1u : 0u);
+            segs.push_back(ListSeg{q, ci, cj, ck, len, std::min(tx, R.nb[0] - ci),
+                                   std::min(ty, R.nb[1] - cj), 0});
+            ck += len;
+          }
     }
     R.nseg = (uint32_t)segs.size() - R.seg0;
   }
