@@ -1534,14 +1534,21 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
   // each rebuild, with the kernel's own timestamps; a rebuild is timed from
   // the end of the step before it (or a marker when that step was not
   // timed) to the start of the step after it.  A timed dispatch idles the
-  // GPU about 9 us before it, so k > 1 keeps that cost off most steps.
+  // GPU several microseconds before it (about 2 us less with the
+  // timing-only events below), so k > 1 keeps that cost off most steps.
   const uint32_t tstride = o->timing > 0 ? (uint32_t)o->timing : 0u;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> reb_spans;
   std::vector<hipEvent_t> ev_mark;                     // rebuild-start markers
   hipEvent_t reb_from = nullptr, prev_stop = nullptr;
+  // timing-only events: no system-scope fence (no cache write-back and
+  // invalidation around the timed work); they are read after the run's
+  // synchronisation
+  auto tev = [](hipEvent_t *e) {
+    return hipEventCreateWithFlags(e, hipEventDisableSystemFence) == hipSuccess;
+  };
   auto mk = [&](std::vector<hipEvent_t> &v) -> hipEvent_t {
     hipEvent_t e = nullptr;
-    if (hipEventCreate(&e) == hipSuccess) v.push_back(e);
+    if (tev(&e)) v.push_back(e);
     return e;
   };
   auto rec = [&](std::vector<hipEvent_t> &v) {
@@ -1680,7 +1687,7 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
       A.step = step;
       hipEvent_t e0 = nullptr, e1 = nullptr;
       if (tstride && (reb_from || step % tstride == 0)) {
-        if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        if (tev(&e0) && tev(&e1)) {
           ev_step.push_back(e0);
           ev_step.push_back(e1);
           if (reb_from) reb_spans.emplace_back(reb_from, e0);
