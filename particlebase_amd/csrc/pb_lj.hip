@@ -355,25 +355,83 @@ __device__ __forceinline__ double4 window_global(const WinTable &W, const ListSe
   return p;
 }
 
-// Stage the window in FP64 (pair kernel) when it fits.
-// Copy every window entry (cell) into its LDS range, image shift applied:
-// 16 lanes per entry, 4 entries per wavefront at a time, so the copies of
-// different cells are in flight together (no per-particle entry search).
+// Copy the window into LDS (image shifts applied), calling f(q, p) for
+// window-local particle q.  The copy is latency bound.  The cells of a
+// window column are consecutive in storage except across the periodic wrap
+// in z, so a column is one or two contiguous runs: with columns of
+// STAGE_RUN_MIN particles or more on average, one wavefront copies a
+// column, 64 particles a load and up to four loads of a lane in flight
+// before their stores (the 1M-atom fluid's step kernel, 153 particles a
+// column: 125.3 -> 121.5 us).  Windows of shorter columns (the 256 batched
+// replicas, about 100 a column, whose columns also wrap more often) are
+// copied entry by entry, 16 lanes to an entry and four entries a wavefront
+// (the run form was 3 % slower there).  A whole-replica window (one entry)
+// is copied by all threads.
+#ifndef STAGE_RUN_MIN
+#define STAGE_RUN_MIN 128u
+#endif
 template <typename F>
 __device__ __forceinline__ void for_window_particles(const WinTable &W, const RepInfo &R,
                                                      const double4 *r, F &&f) {
-  const uint32_t lane = pb_lane(), g = lane >> 4, l16 = lane & 15u;
-  const uint32_t wave = threadIdx.x / 64, nwv = blockDim.x / 64;
-  for (uint32_t e0 = wave * 4; e0 < W.nent; e0 += nwv * 4) {
-    const uint32_t e = e0 + g;
-    if (e >= W.nent) continue;
-    const uint32_t c = W.ec[e], src = W.eg0[e], dst = W.eoff[e], sh = W.esh[e];
+  constexpr uint32_t U = 4;
+  auto copy = [&](uint32_t src, uint32_t dst, uint32_t n, uint32_t sh, uint32_t t0,
+                  uint32_t stride) {
     const double sx = (double)((int)(sh & 3u) - 1) * R.L[0];
     const double sy = (double)((int)((sh >> 2) & 3u) - 1) * R.L[1];
     const double sz = (double)((int)((sh >> 4) & 3u) - 1) * R.L[2];
-    for (uint32_t k = l16; k < c; k += 16) {
-      const double4 p = r[src + k];
-      f(dst + k, make_double4(p.x + sx, p.y + sy, p.z + sz, p.w));
+    for (uint32_t k0 = t0; k0 < n; k0 += U * stride) {
+      double4 p[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t k = k0 + u * stride;
+        if (k < n) p[u] = r[src + k];
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t k = k0 + u * stride;
+        if (k < n) f(dst + k, make_double4(p[u].x + sx, p[u].y + sy, p[u].z + sz, p[u].w));
+      }
+    }
+  };
+  if (W.nent == 1) {                       // whole replica
+    copy(W.eg0[0], W.eoff[0], W.ec[0], W.esh[0], threadIdx.x, blockDim.x);
+    return;
+  }
+  const uint32_t lane = pb_lane(), wave = threadIdx.x / 64, nwv = blockDim.x / 64;
+  if (W.total < STAGE_RUN_MIN * W.ncol) {   // block-uniform: short columns, entry-wise
+    const uint32_t g = lane >> 4, l16 = lane & 15u;
+    for (uint32_t e0 = wave * 4; e0 < W.nent; e0 += nwv * 4) {
+      const uint32_t e = e0 + g;
+      if (e >= W.nent) continue;
+      const uint32_t c = W.ec[e], src = W.eg0[e], dst = W.eoff[e], sh = W.esh[e];
+      const double sx = (double)((int)(sh & 3u) - 1) * R.L[0];
+      const double sy = (double)((int)((sh >> 2) & 3u) - 1) * R.L[1];
+      const double sz = (double)((int)((sh >> 4) & 3u) - 1) * R.L[2];
+      for (uint32_t k = l16; k < c; k += 16) {
+        const double4 p = r[src + k];
+        f(dst + k, make_double4(p.x + sx, p.y + sy, p.z + sz, p.w));
+      }
+    }
+    return;
+  }
+  const uint32_t zs = W.zs;                // entries of a column (<= MAXZ + 2 < 64)
+  for (uint32_t col = wave; col < W.ncol; col += nwv) {   // wave-uniform
+    const uint32_t e0 = col * zs;
+    // entry e0 + t starts a run: the column's first, or not contiguous in
+    // storage with the entry before it, or another image shift
+    bool start = false;
+    if (lane < zs) {
+      const uint32_t e = e0 + lane;
+      start = lane == 0 || W.esh[e] != W.esh[e - 1] || W.eg0[e] != W.eg0[e - 1] + W.ec[e - 1];
+    }
+    uint64_t runs = __ballot(start) | (1ull << zs);
+    while (true) {
+      const uint32_t t0 = (uint32_t)__builtin_ctzll(runs);
+      if (t0 >= zs) break;
+      runs &= runs - 1ull;
+      const uint32_t t1 = (uint32_t)__builtin_ctzll(runs);
+      const uint32_t ea = e0 + t0, eb = e0 + t1;
+      copy(W.eg0[ea], W.eoff[ea], W.eoff[eb] - W.eoff[ea], W.esh[ea], lane, 64u);
     }
   }
 }
