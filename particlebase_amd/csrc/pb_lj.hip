@@ -487,10 +487,11 @@ __device__ __forceinline__ void tile_particle(const WinTable &W, uint32_t u, uin
 // rc + skin.  A window larger than the stage reads FP64 from global memory;
 // a window of more than 65534 particles raises CNT_WIN.
 // Layout: the entries of a slot are grouped in words of 8 bytes (five 12-bit
-// entries in staged segments, four 16-bit entries otherwise); word m of slot
-// gi is at m * npad + gi, so the step kernel's lanes load whole words and
-// the particles of a wavefront read adjacent words.  Rows are padded with
-// the format's inert entry to whole batches (nn holds the padded length).
+// entries in staged segments, four 16-bit entries otherwise), and the words
+// in pairs: words 2j and 2j+1 of slot gi are the 16 bytes at pair j * npad +
+// gi, so a lane of the step kernel loads two words at a time and the
+// particles of a wavefront read adjacent pairs.  Rows are padded with the
+// format's inert entry to whole word pairs (nn holds the padded length).
 // 12-bit entries move 20 % fewer list bytes per step than 16-bit ones
 // (measured: the step kernel 159 -> 151 us at 1M atoms).
 // eight wavefronts per SIMD (64 VGPRs; the few values spilled are per
@@ -544,10 +545,13 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BUILD_W
     uint32_t li, gi, run;
     tile_particle(W, u, li, gi, run);
     // Kept entries are packed epw to a 64-bit word in registers; word m of
-    // slot gi is written whole at wrow[m * npad], for m < maxnb / epw (the
+    // slot gi is written whole at waddr(m), for m < maxnb / epw (the
     // allocation holds maxnb + 48 entries per slot: slack words for the
-    // step kernel's read one batch ahead).
-    uint64_t *wrow = reinterpret_cast<uint64_t *>(nl) + gi;
+    // step kernel's read one word pair ahead).
+    uint64_t *const wrow = reinterpret_cast<uint64_t *>(nl) + 2u * gi;
+    auto waddr = [&](uint32_t m) -> uint64_t * {
+      return wrow + (size_t)(m >> 1) * 2u * npad + (m & 1u);
+    };
     const uint32_t wcap = maxnb / epw;
     // 12-bit format: an entry enters the word at bit 0 and the word moves up
     // 12 bits per append, behind a marker bit that starts at bit 0; after
@@ -560,12 +564,13 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BUILD_W
     // the list rebuilt larger).  cnt is advanced per group of tests
     // (popcount).
     uint64_t word = epw == 5u ? 1ull : 0ull;
-    uint32_t cnt = 0, kw = 0;
+    uint32_t cnt = 0, kw = 0, odd = 0;
     uint64_t *wp = wrow;
-    const uint64_t *const wend = wrow + (size_t)wcap * npad;
+    const uint64_t *const wend = waddr(wcap);
     auto store_word = [&]() {
       if (wp < wend) *wp = word;
-      wp += npad;
+      wp += odd ? 2u * npad - 1u : 1u;
+      odd ^= 1u;
     };
     auto append = [&](uint32_t q) {
       if constexpr (epw == 5u) {
@@ -665,7 +670,7 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BUILD_W
     // close the partial word and pad to whole batches with the inert entry
     // (nn holds the padded length)
     const uint32_t used = cnt < maxnb ? cnt : maxnb;
-    const uint32_t batch = epw * LPA;
+    const uint32_t batch = 2u * epw * LPA;      // whole word pairs
     const uint32_t padded = (used + batch - 1u) / batch * batch;
     if (cnt % epw != 0u && cnt < maxnb) {
       // the partial word: its kr entries moved to the word's top fields, the
@@ -678,9 +683,9 @@ __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BUILD_W
       } else {
         last = (word >> (eb * (epw - kr))) | (padw & (~0ull << (eb * kr)));
       }
-      wrow[(size_t)(cnt / epw) * npad] = last;
+      *waddr(cnt / epw) = last;
     }
-    for (uint32_t m = (used + epw - 1u) / epw; m < padded / epw; ++m) wrow[(size_t)m * npad] = padw;
+    for (uint32_t m = (used + epw - 1u) / epw; m < padded / epw; ++m) *waddr(m) = padw;
     nn[gi] = padded;
     if (cnt > maxnb) atomicMax(&counter[CNT_OVF], cnt);
     mymax = max(mymax, cnt);
@@ -778,15 +783,14 @@ __device__ __forceinline__ void publish_prev(const StepArgs &A) {
   __hip_atomic_store(&A.mon[slot + 3], A.step, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// Pairs and integration of one segment's tile particles.  4 lanes per
-// particle, 16 particles per wavefront: lane sl takes list word 4 b + sl of
-// batch b (entries 20 b + 5 sl + j, j < 5), so every lane has 5 independent
-// pairs in flight and the next batch's list word is loaded while the current
-// one is evaluated.  Indices past the end of a row are clamped (no extra
-// HBM lines) and replaced by the padding slot, so with FAST (window staged,
-// no minimum image) a pair is a straight-line sequence: the cutoff is a
-// select.  A butterfly over the 4 lanes sums the partial forces and lane 0
-// of each group integrates.
+// Pairs and integration of one segment's tile particles: one lane a
+// particle (LPA = 1), 64 particles a wavefront.  With FAST (window staged,
+// no minimum image) a lane takes a word pair (ten pairs) a step, the next
+// pair's 16-byte load in flight while they are evaluated; rows are padded
+// with the inert LDS slot, so a pair is a straight-line sequence and the
+// cutoff is a select.  Other segments take one word (four pairs) a step.
+// (LPA > 1 lanes a particle would sum partial forces over the lanes and
+// let the first lane integrate.)
 template <bool FAST, bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
 __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, const ListSeg &G,
                                          const RepInfo &R, bool staged, bool mi, double &ke, double &pe_b,
@@ -804,12 +808,11 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
   const size_t ld = A.npad;
   // Software pipeline across particle groups: while group u is evaluated,
   // the next group's slot, list length, first list batch and integration
-  // state are already in flight.  The first batch (words 0..LPA-1) is read
-  // without knowing the list length: those words lie inside the list array
-  // for any slot (capacity >= 20 * LPA entries) and are not used past the
-  // length.
+  // state are already in flight.  The first word pair is read without
+  // knowing the list length: those words lie inside the list array for any
+  // slot (capacity >= 20 entries) and are not used past the length.
   uint32_t nli = 0, ngi = 0, nnn = 0;
-  uint64_t nq = 0;
+  ulonglong2 nqq = make_ulonglong2(0, 0);
   double4 np_own = make_double4(0, 0, 0, 0), nv_own = np_own;
   float4 nrb_own = make_float4(0, 0, 0, 0);
   // the next group's slot, length and first list word, needed when its
@@ -823,7 +826,7 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
     if (u < ntile) {
       tile_particle(S.W, u, nli, ngi, run);
       nnn = A.nn[ngi];
-      nq = A.nl[(size_t)sl * ld + ngi];
+      nqq = reinterpret_cast<const ulonglong2 *>(A.nl)[ngi];
     }
   };
   // the next group's integration state, needed at the end of its iteration:
@@ -843,7 +846,7 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
     const uint32_t u = ub + grp;
     const bool act = u < ntile;
     const uint32_t li = nli, gi = ngi, nni = nnn;
-    uint64_t qw = nq;
+    ulonglong2 qq = nqq;
     const double4 p_own = np_own, v_own = nv_own;
     const float4 rb_own = nrb_own;
     const uint32_t o_own = norig;
@@ -918,21 +921,54 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
         }
       }
     };
-    // batch b: lane sl takes list word 4 b + sl (entries 20 b + 5 sl .. + 4);
-    // lists are padded with the inert entry to whole batches
-    const uint64_t *wp = A.nl + (size_t)(LPA + sl) * ld + gi;  // word of the next batch
-    const size_t wstep = LPA * ld;
+    // rows are padded with the inert entry to whole word pairs
     constexpr uint32_t EPW = FAST ? 5u : 4u, EB = FAST ? 12u : 16u;
     constexpr uint32_t EMASK = (1u << EB) - 1u;
     const uint32_t nbat = act ? nni / (EPW * LPA) : 0u;
-    for (uint32_t b = 0; b < nbat; ++b) {
-      const uint64_t w64 = qw;
-      qw = *wp;                                   // one batch ahead (slack words at the end)
-      wp += wstep;
+    auto batch = [&](uint64_t w64) {
 #pragma unroll
       for (uint32_t j = 0; j < EPW; ++j) {
         const uint32_t qj = (uint32_t)(w64 >> (EB * j)) & EMASK;
         pair(qj);
+      }
+    };
+    static_assert(LPA == 1u, "word-pair rows assume one lane a particle");
+    if (!FAST) {
+      // one word a batch (the minimum-image and unstaged paths use more
+      // registers a pair): word m of the row at 2 gi + (m >> 1) 2 npad + (m & 1)
+      uint64_t qw = qq.x;
+      const uint64_t *wp = A.nl + 2u * (size_t)gi + 1u;
+      for (uint32_t b = 0; b < nbat; ++b) {
+        const uint64_t w64 = qw;
+        qw = *wp;                                 // one word ahead (slack words at the end)
+        wp += (b & 1u) ? 1u : 2u * ld - 1u;
+        batch(w64);
+      }
+    } else {
+      // one 16-byte load brings the next two words while ten pairs are
+      // evaluated
+      const ulonglong2 *pp = reinterpret_cast<const ulonglong2 *>(A.nl) + ld + gi;
+      const uint32_t npair = act ? nni / (2u * EPW) : 0u;
+      for (uint32_t b = 0; b < npair; ++b) {
+        // unpack both words first, so the load lands in their registers and
+        // is waited for only when the next step unpacks it
+        uint32_t e[2 * EPW];
+#pragma unroll
+        for (uint32_t j = 0; j < EPW; ++j) {
+          e[j] = (uint32_t)(qq.x >> (EB * j)) & EMASK;
+          e[EPW + j] = (uint32_t)(qq.y >> (EB * j)) & EMASK;
+        }
+        // the entries are computed here (an empty asm that reads and writes
+        // them), so the old words are dead when the load is issued
+#pragma unroll
+        for (uint32_t j = 0; j < 2 * EPW; ++j) asm volatile("" : "+v"(e[j]));
+        qq = *pp;                                 // one pair ahead (slack words at the end)
+        pp += ld;
+#pragma unroll
+        for (uint32_t j = 0; j < EPW; ++j) pair(e[j]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (uint32_t j = 0; j < EPW; ++j) pair(e[EPW + j]);
       }
     }
     if (MIXED && ENERGY) {
