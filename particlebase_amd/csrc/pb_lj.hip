@@ -437,9 +437,9 @@ __device__ __forceinline__ void for_window_particles(const WinTable &W, const Re
 }
 
 // Stage the window in FP64 (pair kernel) when it fits.
-__device__ void stage_window(Stage &S, const ListSeg &G, const RepInfo &R, const uint32_t *cstart,
-                             const double4 *r) {
-  window_table(S.W, G, R, cstart);
+// (the window table is built before: the step kernel issues its first list
+// and state loads between the two)
+__device__ void stage_window(Stage &S, const RepInfo &R, const double4 *r) {
   if (S.W.total <= WCAP)
     for_window_particles(S.W, R, r, [&](uint32_t q, const double4 &p) {
       S.x[q] = p.x;
@@ -792,7 +792,7 @@ __device__ __forceinline__ void publish_prev(const StepArgs &A) {
 // (LPA > 1 lanes a particle would sum partial forces over the lanes and
 // let the first lane integrate.)
 template <bool FAST, bool MIXED, bool ENERGY, bool WRITE_F, bool ONE_TYPE, bool INTEGRATE>
-__device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, const ListSeg &G,
+__device__ __forceinline__ void lj_tiles(const StepArgs &A, Stage &S, const ListSeg &G,
                                          const RepInfo &R, bool staged, bool mi, double &ke, double &pe_b,
                                          double &vir_b, float &d2m, float &v2m,
                                          unsigned long long *hsum) {
@@ -840,8 +840,11 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, const Stage &S, cons
       if (WRITE_F) norig = A.orig[ngi];
     }
   };
+  // the first group's list and state loads are in flight while the window
+  // is staged
   fetch_list(wave * APW);
   fetch_state(wave * APW);
+  stage_window(S, R, A.rin);
   for (uint32_t ub = wave * APW; ub < ntile; ub += nwv * APW) {   // wave-uniform
     const uint32_t u = ub + grp;
     const bool act = u < ntile;
@@ -1048,7 +1051,7 @@ __device__ __forceinline__ void lj_segment(const StepArgs &A, uint32_t seg) {
   const ListSeg G = A.segs[seg];
   const RepInfo &R = A.reps[G.rep];
   if (WRITE_F && INTEGRATE && threadIdx.x < 16) S.hsum[threadIdx.x] = 0;   // visible after the staging
-  stage_window(S, G, R, A.cstart, A.rin);
+  window_table(S.W, G, R, A.cstart);
   const bool staged = S.W.total <= WCAP;
   const bool mi = G.whole != 0;
 
