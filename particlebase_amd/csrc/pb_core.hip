@@ -207,7 +207,7 @@ extern "C" void pb_sys_destroy(pb_sys *s) {
     pb_free(s->cm_row[b]); pb_free(s->cm_col[b]); pb_free(s->cm_val[b]);
   }
   pb_free(s->gr_hist);
-  pb_free(s->list_segs);
+  pb_free(s->list_segs); pb_free(s->list_wtab);
   if (s->hmon) (void)hipHostFree(s->hmon);
   if (s->ctx) --s->ctx->nsys;
   delete s;

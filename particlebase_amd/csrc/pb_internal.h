@@ -121,8 +121,10 @@ struct pb_sys {
   double4 *a2buf = nullptr;
   uint32_t list_ncell = 0;
   double list_rlist = 0.0;         // largest list radius over the replicas
-  uint32_t *list_segs = nullptr;   // list-build work units (5 words each)
+  uint32_t *list_segs = nullptr;   // list-build work units (8 words each)
   uint32_t list_nsegs = 0;
+  uint4 *list_wtab = nullptr;      // per-segment window tables (cell order of the last sort)
+  uint32_t list_wtab_cap = 0;      // segments list_wtab holds
   uint32_t tile_w = 2;             // list-cell columns per tile side (1 or 2)
   double tile_fill = 0.72;         // target fill of the LDS window stage
   // Continuation of the fused LJ run: the cell order, Verlet list and list

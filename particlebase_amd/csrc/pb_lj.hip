@@ -192,7 +192,7 @@ struct ListSeg {
 
 // entry table of a window: entry e = (window column, z slot) -> LDS offset,
 // storage start, count, packed image shift; plus the runs of tile particles
-struct WinTable {
+struct alignas(16) WinTable {
   uint32_t eoff[MAXE + 1], eg0[MAXE], ec[MAXE], esh[MAXE];
   uint32_t nent, total, ncol, zs;
   uint32_t nrun, rloc[4], rpref[5], rg0[4], rcol[4];
@@ -224,7 +224,7 @@ static_assert(WCAP < 0xFFF, "12-bit list entries address the staged window and i
 struct Stage {              // pair kernel: FP64 positions with shifts applied
   double x[WCAP + 1], y[WCAP + 1], z[WCAP + 1];   // slot WCAP: padding target
   uint8_t t[WCAP + 1];
-  WinTable W;
+  alignas(16) WinTable W;
   unsigned long long hsum[16];  // last step of a run: fingerprint sums (lane mod 16, so
                                 // four lanes of a wavefront share an LDS atomic address)
 };
@@ -232,7 +232,7 @@ struct Stage {              // pair kernel: FP64 positions with shifts applied
 struct StageB {             // list build: FP32 window-relative positions, 12 B a
   float2 xy[WCAPB + 32];     // particle (four workgroups per CU); 32 slack
   float z[WCAPB + 32];       // entries read by the last group of 32 tests
-  WinTable W;
+  alignas(16) WinTable W;
 };
 
 // exclusive block scan (blockDim multiple of 64, <= TBS)
@@ -329,6 +329,27 @@ __device__ void window_table(WinTable &W, const ListSeg &G, const RepInfo &R,
       W.rpref[n] = pref;
     }
   }
+  __syncthreads();
+}
+
+// The window tables depend only on the segments and the cell starts of the
+// last sort, so the list build, which builds them, also stores them (one
+// per segment) and each step workgroup copies its table into LDS with one
+// coalesced load round instead of reading the segment descriptor and then
+// its cells' starts (two dependent rounds) and scanning.
+static_assert(sizeof(WinTable) % 16 == 0, "window tables are copied 16 bytes a thread");
+constexpr uint32_t WTAB_U4 = sizeof(WinTable) / 16;
+
+__device__ __forceinline__ void store_window_table(const WinTable &W, uint4 *tabs, uint32_t seg) {
+  const uint4 *src = reinterpret_cast<const uint4 *>(&W);
+  uint4 *dst = tabs + (size_t)seg * WTAB_U4;
+  for (uint32_t i = threadIdx.x; i < WTAB_U4; i += blockDim.x) dst[i] = src[i];
+}
+
+__device__ __forceinline__ void load_window_table(WinTable &W, const uint4 *tabs, uint32_t seg) {
+  const uint4 *src = tabs + (size_t)seg * WTAB_U4;
+  uint4 *dst = reinterpret_cast<uint4 *>(&W);
+  for (uint32_t i = threadIdx.x; i < WTAB_U4; i += blockDim.x) dst[i] = src[i];
   __syncthreads();
 }
 
@@ -503,12 +524,14 @@ __device__ __forceinline__ void tile_particle(const WinTable &W, uint32_t u, uin
 __global__ void __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BUILD_WPE))) k_build_list(const double4 *r, const uint32_t *cstart,
                                                    const ListSeg *segs, const RepInfo *reps,
                                                    uint32_t npad, uint32_t maxnb, uint16_t *nl,
-                                                   uint32_t *nn, uint32_t *counter) {
+                                                   uint32_t *nn, uint32_t *counter, uint4 *wtab) {
   __shared__ StageB S;
-  const ListSeg G = segs[pb_xcd_chunk8()];
+  const uint32_t seg = pb_xcd_chunk8();
+  const ListSeg G = segs[seg];
   const RepInfo &R = reps[G.rep];
   WinTable &W = S.W;
   window_table(W, G, R, cstart);
+  store_window_table(W, wtab, seg);              // for the step kernels
   if (W.total >= NL_PAD) {                       // block-uniform
     if (threadIdx.x == 0) atomicOr(&counter[CNT_WIN], 1u);
     for (uint32_t u = threadIdx.x; u < W.rpref[W.nrun]; u += blockDim.x) {
@@ -739,6 +762,7 @@ struct StepArgs {
   const uint32_t *nn;
   const float4 *rbuild;
   const uint32_t *cstart;
+  const uint4 *wtab;        // per-segment window tables
   const ListSeg *segs;
   const RepInfo *reps;
   const LJTable *lj;
@@ -1051,7 +1075,7 @@ __device__ __forceinline__ void lj_segment(const StepArgs &A, uint32_t seg) {
   const ListSeg G = A.segs[seg];
   const RepInfo &R = A.reps[G.rep];
   if (WRITE_F && INTEGRATE && threadIdx.x < 16) S.hsum[threadIdx.x] = 0;   // visible after the staging
-  window_table(S.W, G, R, A.cstart);
+  load_window_table(S.W, A.wtab, seg);
   const bool staged = S.W.total <= WCAP;
   const bool mi = G.whole != 0;
 
@@ -1260,6 +1284,12 @@ static int setup_list_cells(pb_sys *s, double rc, const std::vector<double> &ski
   }
   s->list_nsegs = (uint32_t)segs.size();
   if (int rc = pb_alloc(s->ctx, &s->list_segs, 8 * segs.size())) return rc;
+  // the window tables persist while the segments do (a continued list uses
+  // the ones its build stored)
+  if (segs.size() > s->list_wtab_cap) {
+    if (int rc = pb_alloc(s->ctx, &s->list_wtab, (size_t)WTAB_U4 * segs.size())) return rc;
+    s->list_wtab_cap = (uint32_t)segs.size();
+  }
   PB_HIP(s->ctx, hipMemcpyAsync(s->list_segs, segs.data(), sizeof(ListSeg) * segs.size(),
                                 hipMemcpyHostToDevice, s->ctx->stream));
   // per-segment energy partials
@@ -1320,7 +1350,7 @@ static int rebuild(pb_sys *s, bool carry_a) {
   std::swap(s->move, s->move2);
   hipLaunchKernelGGL(k_build_list, dim3(s->list_nsegs), dim3(TB), 0, st, s->r, s->cstart,
                      (const ListSeg *)s->list_segs, s->d_reps, n, s->maxnb, s->nl, s->nn,
-                     s->d_counter);
+                     s->d_counter, s->list_wtab);
   PB_LAUNCH_CHECK(s->ctx, "k_build_list");
   s->ref_binned = false;
   return PB_OK;
@@ -1333,6 +1363,7 @@ static StepArgs make_args(pb_sys *s, double dt) {
   A.nn = s->nn;
   A.rbuild = s->rbuild;
   A.cstart = s->cstart;
+  A.wtab = s->list_wtab;
   A.segs = (const ListSeg *)s->list_segs;
   A.reps = s->d_reps;
   A.lj = s->d_lj;
