@@ -239,6 +239,7 @@ int pb_exclusive_scan(pb_sys *sys, const uint32_t *in, uint32_t *out, uint32_t n
 // cellid[s] given (PB_NONE never); builds perm (sorted slot -> storage slot)
 // ordered by (cell, key[s]); counts/starts in sys->ccount / sys->cstart.
 int pb_counting_sort(pb_sys *sys, uint32_t ncell, const uint32_t *key);
+int pb_counting_sort_counted(pb_sys *sys, uint32_t ncell, const uint32_t *key);
 // per-replica reduction of K per-wave partial components into out[rep*8+c]
 int pb_reduce_waves(pb_sys *sys, int K);
 // storage slot -> original index (PB_NONE for padding), downloaded
@@ -250,6 +251,31 @@ int pb_ensure_radius(pb_sys *sys);
 
 // ------------------------------------------------------------ device util
 __device__ __forceinline__ uint32_t pb_lane() { return __lane_id(); }
+
+// Rank of a slot among the slots of cell c counted so far: lanes holding the
+// same cell as their left neighbour form a run, whose head does one
+// atomicAdd on counts[c] for the whole run and broadcasts the base (one
+// atomic per run when the input is nearly cell-ordered).  All lanes of the
+// wavefront call it; invalid lanes pass valid = false.
+__device__ __forceinline__ uint32_t pb_count_rank(uint32_t c, bool valid, uint32_t *counts) {
+  const uint32_t lane = pb_lane();
+  const uint32_t left = __shfl_up(c, 1, 64);
+  const bool head = valid && (lane == 0 || left != c);
+  const uint64_t heads = __ballot(head);
+  const uint64_t validm = __ballot(valid);
+  // my run head: highest head lane <= lane
+  const uint64_t le = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
+  const uint32_t hl = le ? (63u - (uint32_t)__clzll((long long)le)) : 0u;
+  // run length (for heads): distance to the next head or the end of valid lanes
+  uint32_t base = 0;
+  if (head) {
+    const uint64_t gt = heads & ~((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
+    const uint32_t nextl = gt ? (uint32_t)__ffsll((long long)gt) - 1u : (uint32_t)__popcll(validm);
+    base = atomicAdd(&counts[c], nextl - lane);
+  }
+  base = __shfl(base, (int)hl, 64);
+  return base + (lane - hl);
+}
 
 __device__ __forceinline__ double pb_wave_sum(double x) {
 #pragma unroll

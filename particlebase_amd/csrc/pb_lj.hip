@@ -25,6 +25,7 @@
 
 int pb_ensure_sort_ws(pb_sys *s, uint32_t ncell);
 int pb_counting_sort(pb_sys *s, uint32_t ncell, const uint32_t *key);
+int pb_counting_sort_counted(pb_sys *s, uint32_t ncell, const uint32_t *key);
 __global__ void k_fill_u32(uint32_t *x, uint32_t n, uint32_t val);
 
 enum { CNT_LAST = 0, CNT_SKIN = 1, CNT_OVF = 2, CNT_MAXNN = 3, CNT_REC = 4, CNT_DISP = 5,
@@ -96,33 +97,42 @@ __device__ __forceinline__ uint32_t cell_axis(double x, double lo, uint32_t nb, 
 }
 
 __global__ void k_list_cellid(const double4 *r, const uint32_t *orig, const uint32_t *waveRep,
-                              const RepInfo *reps, uint32_t n, uint32_t *cellid) {
+                              const RepInfo *reps, uint32_t n, uint32_t *cellid,
+                              uint32_t *counts, uint32_t *rank) {
+  // the list cell of slot s (padding: the replica's pad cell), counted in
+  // the same pass (the counting sort's k_count)
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const RepInfo &R = reps[waveRep[s / PB_WAVE]];
-  const uint32_t ncr = R.nb[0] * R.nb[1] * R.nb[2];
-  if (orig[s] == PB_NONE) {
-    cellid[s] = R.cell0 + ncr;
-    return;
+  const bool valid = s < n;
+  uint32_t c = PB_NONE;
+  if (valid) {
+    const RepInfo &R = reps[waveRep[s / PB_WAVE]];
+    const uint32_t ncr = R.nb[0] * R.nb[1] * R.nb[2];
+    if (orig[s] == PB_NONE) {
+      c = R.cell0 + ncr;
+    } else {
+      const double4 p = r[s];
+      const double x = wrap_pos(p.x, R.lo[0], R.L[0], R.Linv[0]);
+      const double y = wrap_pos(p.y, R.lo[1], R.L[1], R.Linv[1]);
+      const double z = wrap_pos(p.z, R.lo[2], R.L[2], R.Linv[2]);
+      const uint32_t ci = cell_axis(x, R.lo[0], R.nb[0], R.Linv[0]);
+      const uint32_t cj = cell_axis(y, R.lo[1], R.nb[1], R.Linv[1]);
+      const uint32_t ck = cell_axis(z, R.lo[2], R.nb[2], R.Linv[2]);
+      c = R.cell0 + ck + R.nb[2] * (cj + R.nb[1] * ci);
+    }
+    cellid[s] = c;
   }
-  const double4 p = r[s];
-  const double x = wrap_pos(p.x, R.lo[0], R.L[0], R.Linv[0]);
-  const double y = wrap_pos(p.y, R.lo[1], R.L[1], R.Linv[1]);
-  const double z = wrap_pos(p.z, R.lo[2], R.L[2], R.Linv[2]);
-  const uint32_t ci = cell_axis(x, R.lo[0], R.nb[0], R.Linv[0]);
-  const uint32_t cj = cell_axis(y, R.lo[1], R.nb[1], R.Linv[1]);
-  const uint32_t ck = cell_axis(z, R.lo[2], R.nb[2], R.Linv[2]);
-  cellid[s] = R.cell0 + ck + R.nb[2] * (cj + R.nb[1] * ci);
+  const uint32_t k = pb_count_rank(c, valid, counts);
+  if (valid) rank[s] = k;
 }
 
 struct ReorderArgs {
   const uint32_t *perm;
   const double4 *r, *v, *a;
-  const uint32_t *orig, *cellid;
+  const uint32_t *orig;
   const uint16_t *type;
   const uint8_t *move;
   double4 *r2, *v2, *a2;
-  uint32_t *orig2, *cell2;
+  uint32_t *orig2;
   uint16_t *type2;
   uint8_t *move2;
   float4 *rbuild;
@@ -149,7 +159,6 @@ __global__ void k_reorder(ReorderArgs A) {
   A.v2[k] = A.v[s];
   if (A.a) A.a2[k] = A.a[s];
   A.orig2[k] = o;
-  A.cell2[k] = A.cellid[s];
   A.type2[k] = A.type[s];
   A.move2[k] = A.move[s];
   A.rbuild[k] = make_float4((float)p.x, (float)p.y, (float)p.z, 0.f);
@@ -1327,16 +1336,19 @@ static int ensure_list_buffers(pb_sys *s, uint32_t maxnb) {
 static int rebuild(pb_sys *s, bool carry_a) {
   hipStream_t st = s->ctx->stream;
   const uint32_t n = s->npad;
+  if (int rc = pb_ensure_sort_ws(s, s->list_ncell)) return rc;
+  hipLaunchKernelGGL(k_fill_u32, dim3(pb_blocks(s->list_ncell + 1, 256)), dim3(256), 0, st,
+                     s->ccount, s->list_ncell + 1, 0u);
   hipLaunchKernelGGL(k_list_cellid, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->r, s->orig,
-                     s->waveRep, s->d_reps, n, s->cellid);
+                     s->waveRep, s->d_reps, n, s->cellid, s->ccount, s->rank);
   PB_LAUNCH_CHECK(s->ctx, "k_list_cellid");
-  if (int rc = pb_counting_sort(s, s->list_ncell, s->orig)) return rc;
+  if (int rc = pb_counting_sort_counted(s, s->list_ncell, s->orig)) return rc;
   ReorderArgs A;
   A.perm = s->perm;
   A.r = s->r; A.v = s->v; A.a = carry_a ? s->a : nullptr;
-  A.orig = s->orig; A.cellid = s->cellid; A.type = s->type; A.move = s->move;
+  A.orig = s->orig; A.type = s->type; A.move = s->move;
   A.r2 = s->r2buf; A.v2 = s->v2buf; A.a2 = s->a2buf;
-  A.orig2 = s->orig2; A.cell2 = s->rank; A.type2 = s->type2; A.move2 = s->move2;
+  A.orig2 = s->orig2; A.type2 = s->type2; A.move2 = s->move2;
   A.rbuild = s->rbuild;
   A.nn = s->nn;
   A.waveRep = s->waveRep; A.reps = s->d_reps; A.n = n;

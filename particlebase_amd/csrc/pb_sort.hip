@@ -6,8 +6,9 @@
 // unstable and timing dependent.  Here:
 //   1. k_count   : wave64 atomics on per-cell counters; equal keys in
 //                  neighbouring lanes are aggregated into one atomic.
-//   2. scan      : exclusive prefix sum, LDS-tiled (reduce / scan-of-sums /
-//                  scan-and-add), any length.
+//   2. scan      : exclusive prefix sum, LDS-tiled (block sums, then
+//                  scan-and-add with each block summing the sums before it),
+//                  any length.
 //   3. k_scatter : slot = start[cell] + rank.
 //   4. k_cell_sort: inside every cell, order by a key (the original particle
 //                  index) -- the in-cell order produced by the atomics is
@@ -64,27 +65,15 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_reduce(const uint32_t *in, uin
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// single block: exclusive scan of nb block sums in place; out_total <- sum
-__global__ void __launch_bounds__(SCAN_BS) k_scan_bsums(uint32_t *bsum, uint32_t nb,
-                                                        uint32_t *out_total) {
-  const uint32_t per = (nb + SCAN_BS - 1) / SCAN_BS;
-  const uint32_t b0 = threadIdx.x * per;
-  uint32_t x = 0;
-  for (uint32_t k = 0; k < per; ++k)
-    if (b0 + k < nb) x += bsum[b0 + k];
-  uint32_t tot;
-  uint32_t run = block_excl_scan(x, &tot);
-  for (uint32_t k = 0; k < per; ++k)
-    if (b0 + k < nb) {
-      const uint32_t v = bsum[b0 + k];
-      bsum[b0 + k] = run;
-      run += v;
-    }
-  if (threadIdx.x == 0) *out_total = tot;
-}
-
+// scan-and-add: each block adds the sums of the blocks before it (summed by
+// the block itself, so no separate pass scans the block sums); the last
+// block writes the total to out[n]
 __global__ void __launch_bounds__(SCAN_BS) k_scan_final(const uint32_t *in, uint32_t n,
                                                         const uint32_t *bsum, uint32_t *out) {
+  uint32_t pre = 0;
+  for (uint32_t k = threadIdx.x; k < blockIdx.x; k += blockDim.x) pre += bsum[k];
+  uint32_t before;
+  (void)block_excl_scan(pre, &before);
   const uint32_t base = blockIdx.x * SCAN_CH + threadIdx.x * SCAN_PER;
   uint32_t v[SCAN_PER];
   uint32_t x = 0;
@@ -94,13 +83,14 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_final(const uint32_t *in, uint
     x += v[k];
   }
   uint32_t tot;
-  uint32_t run = block_excl_scan(x, &tot) + bsum[blockIdx.x];
+  uint32_t run = block_excl_scan(x, &tot) + before;
 #pragma unroll
   for (int k = 0; k < SCAN_PER; ++k)
     if (base + k < n) {
       out[base + k] = run;
       run += v[k];
     }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) out[n] = before + tot;
 }
 
 int pb_exclusive_scan(pb_sys *s, const uint32_t *in, uint32_t *out, uint32_t n) {
@@ -117,7 +107,6 @@ int pb_exclusive_scan(pb_sys *s, const uint32_t *in, uint32_t *out, uint32_t n) 
     return PB_OK;
   }
   hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_BS), 0, st, in, n, s->scan_tmp);
-  hipLaunchKernelGGL(k_scan_bsums, dim3(1), dim3(SCAN_BS), 0, st, s->scan_tmp, nb, out + n);
   hipLaunchKernelGGL(k_scan_final, dim3(nb), dim3(SCAN_BS), 0, st, in, n, s->scan_tmp, out);
   PB_LAUNCH_CHECK(s->ctx, "pb_exclusive_scan");
   return PB_OK;
@@ -129,25 +118,10 @@ int pb_exclusive_scan(pb_sys *s, const uint32_t *in, uint32_t *out, uint32_t n) 
 // input is already nearly cell-ordered (the steady state after a re-sort).
 __global__ void k_count(const uint32_t *cellid, uint32_t n, uint32_t *counts, uint32_t *rank) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t lane = pb_lane();
   const bool valid = s < n;
   const uint32_t c = valid ? cellid[s] : PB_NONE;
-  const uint32_t left = __shfl_up(c, 1, 64);
-  const bool head = valid && (lane == 0 || left != c);
-  const uint64_t heads = __ballot(head);
-  const uint64_t validm = __ballot(valid);
-  // my run head: highest head lane <= lane
-  const uint64_t le = heads & ((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
-  const uint32_t hl = le ? (63u - (uint32_t)__clzll((long long)le)) : 0u;
-  // run length (for heads): distance to the next head or the end of valid lanes
-  uint32_t base = 0;
-  if (head) {
-    const uint64_t gt = heads & ~((lane == 63) ? ~0ull : ((2ull << lane) - 1ull));
-    const uint32_t nextl = gt ? (uint32_t)__ffsll((long long)gt) - 1u : (uint32_t)__popcll(validm);
-    base = atomicAdd(&counts[c], nextl - lane);
-  }
-  base = __shfl(base, (int)hl, 64);
-  if (valid) rank[s] = base + (lane - hl);
+  const uint32_t r = pb_count_rank(c, valid, counts);
+  if (valid) rank[s] = r;
 }
 
 __global__ void k_scatter(const uint32_t *cellid, const uint32_t *rank, const uint32_t *start,
@@ -260,7 +234,6 @@ int pb_ensure_sort_ws(pb_sys *s, uint32_t ncell) {
 }
 
 int pb_counting_sort(pb_sys *s, uint32_t ncell, const uint32_t *key) {
-  s->lj_cont = false;             // the fused LJ run's cell order is gone
   if (int rc = pb_ensure_sort_ws(s, ncell)) return rc;
   hipStream_t st = s->ctx->stream;
   const uint32_t n = s->npad;
@@ -269,6 +242,15 @@ int pb_counting_sort(pb_sys *s, uint32_t ncell, const uint32_t *key) {
   hipLaunchKernelGGL(k_count, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->cellid, n, s->ccount,
                      s->rank);
   PB_LAUNCH_CHECK(s->ctx, "k_count");
+  return pb_counting_sort_counted(s, ncell, key);
+}
+
+// the sort after the counting: s->ccount and s->rank filled (by k_count or
+// by a caller's kernel that computes the cell ids and counts in one pass)
+int pb_counting_sort_counted(pb_sys *s, uint32_t ncell, const uint32_t *key) {
+  s->lj_cont = false;             // the fused LJ run's cell order is gone
+  hipStream_t st = s->ctx->stream;
+  const uint32_t n = s->npad;
   if (int rc = pb_exclusive_scan(s, s->ccount, s->cstart, ncell)) return rc;
   hipLaunchKernelGGL(k_scatter, dim3(pb_blocks(n, 256)), dim3(256), 0, st, s->cellid, s->rank,
                      s->cstart, n, s->perm);
