@@ -800,20 +800,34 @@ __device__ __forceinline__ double rcp64(double x) {
   return y;
 }
 
-// Publish the previous step's displacement / speed maxima to the host ring.
-// Runs in workgroup 0 when this kernel starts, i.e. after the previous
-// step's kernel has finished (stream order): the host reads it without a
-// synchronisation.
+// Copy the displacement / speed maxima of the kernels so far (the running
+// maxima since the last rebuild) into the host ring's slot step - 1, marked
+// with `step`.
+__device__ __forceinline__ void publish_maxima(const uint32_t *counter, uint32_t *mon,
+                                               uint32_t step) {
+  const uint32_t slot = ((step - 1) & 7u) * 4u;
+  const uint32_t d = __hip_atomic_load(&counter[CNT_DISP], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t v = __hip_atomic_load(&counter[CNT_VMAX], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const uint32_t k = __hip_atomic_load(&counter[CNT_SKIN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&mon[slot + 0], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&mon[slot + 1], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&mon[slot + 2], k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&mon[slot + 3], step, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Publish the previous step's maxima to the host ring.  Runs in workgroup 0
+// when this kernel starts, i.e. after the previous step's kernel has
+// finished (stream order): the host reads it without a synchronisation.
 __device__ __forceinline__ void publish_prev(const StepArgs &A) {
   if (!A.mon || A.step == 0 || blockIdx.x != 0 || threadIdx.x != 0) return;
-  const uint32_t slot = ((A.step - 1) & 7u) * 4u;
-  const uint32_t d = __hip_atomic_load(&A.counter[CNT_DISP], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const uint32_t v = __hip_atomic_load(&A.counter[CNT_VMAX], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const uint32_t k = __hip_atomic_load(&A.counter[CNT_SKIN], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(&A.mon[slot + 0], d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  __hip_atomic_store(&A.mon[slot + 1], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  __hip_atomic_store(&A.mon[slot + 2], k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-  __hip_atomic_store(&A.mon[slot + 3], A.step, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  publish_maxima(A.counter, A.mon, A.step);
+}
+
+// The same report on demand, launched behind step kernel `step - 1` when the
+// host needs the displacement of the positions step `step` will use before
+// it decides on a rebuild (the report the kernel of step `step` would make).
+__global__ void k_publish_maxima(const uint32_t *counter, uint32_t *mon, uint32_t step) {
+  if (threadIdx.x == 0) publish_maxima(counter, mon, step);
 }
 
 // Pairs and integration of one segment's tile particles: one lane a
@@ -1777,7 +1791,12 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
       // the particles may reach skin/2.  Kernel `step-1` published the
       // maxima after step `step-2` when it started, so reading them keeps one
       // kernel in flight; the one step of motion in between is bounded by
-      // 1.5 * vmax * dt.
+      // 1.5 * vmax * dt.  When that bound reaches skin/2 but one step at the
+      // top speed seen so far does not, the maxima after step `step-1` --
+      // the displacement of the positions kernel `step` uses -- are fetched
+      // instead (one wait for the GPU) and decide alone.  The fastest
+      // particles move ballistically over a rebuild period, so in about
+      // half the periods this puts the rebuild one step later.
       bool reb = false;
       if (step == 0 && cont) {
         // the previous run's last kernel measured the displacement of these
@@ -1808,7 +1827,28 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
           std::memcpy(&d2, &ud, 4);
           std::memcpy(&v2, &uv, 4);
           // maxima are in units of each replica's half skin
-          if (std::sqrt((double)d2) + 1.5 * std::sqrt((double)v2) * o->dt >= 1.0) reb = true;
+          const double dl = std::sqrt((double)d2), vdt = std::sqrt((double)v2) * o->dt;
+          if (dl + vdt >= 1.0) {
+            reb = true;
+          } else if (dl + 1.5 * vdt >= 1.0) {
+            const uint32_t eslot = ((step - 1) & 7u) * 4u;
+            hipLaunchKernelGGL(k_publish_maxima, dim3(1), dim3(64), 0, st, s->d_counter, s->dmon,
+                               step);
+            PB_LAUNCH_CHECK(s->ctx, "k_publish_maxima");
+            const auto t1 = std::chrono::steady_clock::now();
+            while (mon[eslot + 3] != step) {
+              if (std::chrono::steady_clock::now() - t1 > std::chrono::seconds(30)) {
+                return pb_fail(s->ctx, PB_ERR_HIP, "pb_lj_run: step report timed out");
+              }
+              std::this_thread::yield();
+            }
+            std::atomic_thread_fence(std::memory_order_acquire);
+            const uint32_t ue = mon[eslot + 0];
+            float e2;
+            std::memcpy(&e2, &ue, 4);
+            // float rounding of the squared ratio: a margin of 1e-3
+            if (e2 >= 0.999f) reb = true;
+          }
         }
       }
       if (reb) {
