@@ -1797,35 +1797,39 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
       // instead (one wait for the GPU) and decide alone.  The fastest
       // particles move ballistically over a rebuild period, so in about
       // half the periods this puts the rebuild one step later.
+      // A continued list: the previous run's last kernel measured the
+      // displacement of the positions kernel 0 uses (its maxima d0, v0), so
+      // step 0 rebuilds only if they already reach skin/2, and step 1 decides
+      // from d0, v0 as later steps do from the report of step `step-2`.
       bool reb = false;
+      volatile uint32_t *mon = (volatile uint32_t *)s->hmon;
       if (step == 0 && cont) {
-        // the previous run's last kernel measured the displacement of these
-        // positions (at most d0); kernel 0 uses them and kernel 1 the positions
-        // one step later, and the first report of this run (kernel 0's output,
-        // read at step 2) covers kernel 2: the same one-step bound as below
-        float d2, v2;
+        float d2;
         std::memcpy(&d2, &s->lj_cont_d2, 4);
-        std::memcpy(&v2, &s->lj_cont_v2, 4);
-        reb = since0 >= (int64_t)o->rebuild_every ||
-              std::sqrt((double)d2) + 1.5 * std::sqrt((double)v2) * o->dt >= 1.0;
+        // float rounding of the squared ratio: a margin of 1e-3
+        reb = since0 >= (int64_t)o->rebuild_every || d2 >= 0.999f;
       } else if (step > 0) {
         if ((int64_t)step - last_build >= (int64_t)o->rebuild_every) {
           reb = true;
-        } else if (step >= 2 && (int64_t)step - last_build >= 2) {
-          const uint32_t slot = ((step - 2) & 7u) * 4u;
-          volatile uint32_t *mon = (volatile uint32_t *)s->hmon;
-          const auto t0 = std::chrono::steady_clock::now();
-          while (mon[slot + 3] != step - 1) {
-            if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(30)) {
-              return pb_fail(s->ctx, PB_ERR_HIP, "pb_lj_run: step report timed out");
-            }
-            std::this_thread::yield();
-          }
-          std::atomic_thread_fence(std::memory_order_acquire);
-          const uint32_t ud = mon[slot + 0], uv = mon[slot + 1];
+        } else if ((int64_t)step - last_build >= 2 && (step >= 2 || cont)) {
           float d2, v2;
-          std::memcpy(&d2, &ud, 4);
-          std::memcpy(&v2, &uv, 4);
+          if (step >= 2) {
+            const uint32_t slot = ((step - 2) & 7u) * 4u;
+            const auto t0 = std::chrono::steady_clock::now();
+            while (mon[slot + 3] != step - 1) {
+              if (std::chrono::steady_clock::now() - t0 > std::chrono::seconds(30)) {
+                return pb_fail(s->ctx, PB_ERR_HIP, "pb_lj_run: step report timed out");
+              }
+              std::this_thread::yield();
+            }
+            std::atomic_thread_fence(std::memory_order_acquire);
+            const uint32_t ud = mon[slot + 0], uv = mon[slot + 1];
+            std::memcpy(&d2, &ud, 4);
+            std::memcpy(&v2, &uv, 4);
+          } else {
+            std::memcpy(&d2, &s->lj_cont_d2, 4);
+            std::memcpy(&v2, &s->lj_cont_v2, 4);
+          }
           // maxima are in units of each replica's half skin
           const double dl = std::sqrt((double)d2), vdt = std::sqrt((double)v2) * o->dt;
           if (dl + vdt >= 1.0) {
@@ -1846,7 +1850,6 @@ extern "C" int pb_lj_run(pb_sys *s, const pb_run_opts *o, double *energies) {
             const uint32_t ue = mon[eslot + 0];
             float e2;
             std::memcpy(&e2, &ue, 4);
-            // float rounding of the squared ratio: a margin of 1e-3
             if (e2 >= 0.999f) reb = true;
           }
         }

@@ -165,6 +165,38 @@ def test_continued_runs_match_oracle_loop(ctx, orc, persistent):
     assert s.lj_run_stats()["rebuilds"] == 1
 
 
+def test_short_runs_rebuild_only_when_needed(ctx, orc):
+    """Twenty 5-step runs in a row (FP64).  A continued run rebuilds at its
+    first step only if the positions it starts from have reached skin/2,
+    and its second step decides from the previous run's final maxima, so a
+    list outlives the run boundaries: no skin violation, the oracle's
+    100-step loop matched, and about as many rebuilds as one 100-step run."""
+    say("twenty 5-step runs vs one 100-step run and the oracle loop, config1")
+    w = workloads.lj_config1()
+    dt = 0.005
+    s = lj_sys(ctx, w)
+    s.clear_force(True)
+    nreb = 0
+    for _ in range(20):
+        s.lj_run(dt, 5, precision=pb.PB_FP64, skin=0.3, rebuild_every=20, energy_every=0)
+        st = s.lj_run_stats()
+        assert st["skin_violation"] == 0
+        nreb += st["rebuilds"]
+    p = orc.lj_params([1.0, 1.0], 1, RC2, True)
+    r, v, _, _ = orc.lj_run(w.r, w.v, np.zeros_like(w.r), 1.0 / w.m, w.type, w.cell, 2.5,
+                            RC2, p, dt, 100)
+    dr = s.get("r") - r
+    dr -= w.L * np.rint(dr / w.L)
+    assert np.abs(dr).max() < 1e-9
+    assert np.abs(s.get("v") - v).max() < 1e-8
+    one = lj_sys(ctx, w)
+    one.clear_force(True)
+    one.lj_run(dt, 100, precision=pb.PB_FP64, skin=0.3, rebuild_every=20, energy_every=0)
+    n1 = one.lj_run_stats()["rebuilds"]
+    say(f"  rebuilds: {nreb} in 20 runs of 5 steps, {n1} in one run of 100")
+    assert nreb <= n1 + 2
+
+
 def test_sampled_timing(ctx):
     """opts.timing = 4 times steps 0, 4, 8, ... and the step after each
     rebuild (forced every 5 steps here), and every rebuild; the results are
