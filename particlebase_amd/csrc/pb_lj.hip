@@ -1012,7 +1012,9 @@ __device__ __forceinline__ void lj_tiles(const StepArgs &A, Stage &S, const List
         // them), so the old words are dead when the load is issued
 #pragma unroll
         for (uint32_t j = 0; j < 2 * EPW; ++j) asm volatile("" : "+v"(e[j]));
-        qq = *pp;                                 // one pair ahead (slack words at the end)
+        // the row's next pair; none past the row's end (its cache lines
+        // would be fetched for nothing when all their lanes are done)
+        if (b + 1u < npair) qq = *pp;
         pp += ld;
 #pragma unroll
         for (uint32_t j = 0; j < EPW; ++j) pair(e[j]);
